@@ -1,0 +1,118 @@
+/*
+ * swfwi_hip.h -- C ABI of the MI355X acoustic FD propagation and
+ * adjoint-gradient library (libswfwi_hip.so).
+ *
+ * Plain C types only.  Every entry point except the two per-step drop-in
+ * kernels returns a status code (SWFWI_OK on success); swfwi_last_error()
+ * gives the message of the last failure on the calling thread.
+ *
+ * Grid conventions follow the reference (damp4t10d.cpp:138-151,174): a
+ * physical model of nz x nx cells is expanded to nzpad = nz+nb+12 by
+ * nxpad = nx+2nb+12 with the sponge on the left, right and bottom, stored
+ * index ix*nzpad + iz (z contiguous), in transformed units
+ * vel_t = dx^2/(v^2 dt^2).  Source/receiver positions are physical cell
+ * indices (ShotPosition::getx/getz, shot-position.cpp:11-37).  Seismograms
+ * are time-major [nt][ng] (Damp4t10d::recordSeis, damp4t10d.cpp:163-180).
+ */
+#ifndef SWFWI_HIP_H_
+#define SWFWI_HIP_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SWFWI_OK 0
+#define SWFWI_ERR_ARG 1     /* invalid argument or geometry */
+#define SWFWI_ERR_STATE 2   /* call out of order (e.g. no velocity bound) */
+#define SWFWI_ERR_HIP 3     /* HIP runtime failure */
+#define SWFWI_ERR_NUMERIC 4 /* numerical failure (eigen-solve did not converge) */
+#define SWFWI_ERR_RANGE 5   /* size beyond a supported maximum */
+
+#define SWFWI_PREC_EXACT 0 /* the reference's FP32/FP64 expression trees, bit-for-bit */
+#define SWFWI_PREC_FP32 1  /* pure FP32 with fused multiply-adds */
+
+typedef struct swfwi_solver_s *swfwi_solver;       /* Damp4t10d: geometry + bound model */
+typedef struct swfwi_inversion_s *swfwi_inversion; /* FwiFramework/EssFwiFramework state */
+typedef struct swfwi_enkf_s *swfwi_enkf;           /* EnkfAnalyze state */
+
+const char *swfwi_last_error(void);
+const char *swfwi_version(void);
+/* Number of visible HIP devices (0 when none); never initialises a context. */
+int swfwi_device_count(int *count);
+
+/* ---- drop-in per-step kernels (host pointers, in place prev <- next) ----- */
+/* Replaces fd4t10s_damp_zjh_2d_vtrans (fd4t10s-damp-zjh.h:11,
+ * fd4t10s-damp-zjh.c:14-115): nx, nz are the padded dims, nb the damping width
+ * (Damp4t10d passes bx0).  u2 receives the unscaled Laplacian over [5, n-5)
+ * exactly as the reference's scratch does.  Aborts the process on a device
+ * error, as the reference's error paths exit. */
+void fd4t10s_damp_zjh_2d_vtrans(float *prev_wave, const float *curr_wave, const float *vel, float *u2, int nx,
+                                int nz, int nb);
+/* Replaces fd4t10s_zjh_2d_vtrans (fd4t10s-zjh.h:11, fd4t10s-zjh.c:10-56). */
+void fd4t10s_zjh_2d_vtrans(float *prev_wave, const float *curr_wave, const float *vel, float *u2, int nx, int nz);
+/* The same two kernels with a status return and a precision switch. */
+int swfwi_step_damped(float *prev_wave, const float *curr_wave, const float *vel, float *u2, int nx, int nz, int nb,
+                      int precision);
+int swfwi_step_undamped(float *prev_wave, const float *curr_wave, const float *vel, float *u2, int nx, int nz,
+                        int precision);
+
+/* ---- host helpers of the Damp4t10d surface -------------------------------- */
+/* Damp4t10d::expandDomain (damp4t10d.cpp:138-151): v [nx][nz] m/s ->
+ * out [nxpad][nzpad] transformed. */
+int swfwi_expand_domain(const float *v, int nx, int nz, int nb, float dx, float dt, float *out);
+/* rickerWavelet (ricker-wavelet.cpp:15-21) */
+int swfwi_ricker(float *wlt, int nt, float fm, float dt, float amp);
+/* RandomCodes(seed).genPlus1Minus1(n) after `skip` earlier draws
+ * (random-code.cpp:2-22). */
+int swfwi_random_codes(int seed, int skip, int n, int *codes);
+
+/* ---- device-resident propagator (Damp4t10d) ------------------------------ */
+/* Damp4t10d(allSrcPos, allGeoPos, dt, dx, fm, nb, nt) (damp4t10d.cpp:367-376)
+ * on the current HIP device, one stream per handle; handles are not
+ * thread-safe, many may coexist.  nb must be >= 6. */
+int swfwi_solver_create(swfwi_solver *out, int nx, int nz, int nb, float dt, float dx, float fm, int nt, int ns,
+                        const int *sx, const int *sz, int ng, const int *gx, const int *gz, int precision);
+int swfwi_solver_destroy(swfwi_solver h);
+/* dims[0..5] = nxpad, nzpad, bx0, bz0, boundary floats per step, stencil tiles */
+int swfwi_solver_dims(swfwi_solver h, int *dims);
+int swfwi_solver_set_precision(swfwi_solver h, int precision);
+/* Damp4t10d::bindVelocity (damp4t10d.cpp:159-161): padded transformed model */
+int swfwi_solver_bind_velocity(swfwi_solver h, const float *vel_t);
+int swfwi_solver_get_velocity(swfwi_solver h, float *vel_t);
+/* FwiForwardModeling (shot >= 0; encsrc[nt], damp4t10d.cpp:272-289) or
+ * EssForwardModeling (shot < 0; encsrc[nt][ns], damp4t10d.cpp:291-307). */
+int swfwi_forward_modeling(swfwi_solver h, const float *encsrc, int shot, float *dcal);
+/* calgradient (fwiframework.cpp:114-177 for shot >= 0,
+ * essfwiframework.cpp:114-173 for shot < 0): vsrc [nt][ng] adjoint source,
+ * grad [nxpad][nzpad] receives the raw imaging condition (before
+ * scaleGradient/maskGradient). */
+int swfwi_calgradient(swfwi_solver h, const float *encsrc, int shot, const float *vsrc, float *grad);
+/* Observed data of every shot, one batched launch per time step (fm-damp's
+ * shot loop, main-fm-damp.cpp:241-256): wlt [nt] -> dobs [ns][nt][ng]. */
+int swfwi_forward_shots(swfwi_solver h, const float *wlt, float *dobs);
+/* Grid-point updates and stencil launches issued by this handle so far. */
+int swfwi_solver_counters(swfwi_solver h, double *point_updates, long long *launches);
+
+/* ---- inversion drivers (FwiFramework / EssFwiFramework) ------------------ */
+/* wlt [nt], dobs [ns][nt][ng]; UpdateVelOp(vmin, vmax, dx, dt) and
+ * UpdateSteplenOp(fm, velop, nita, maxdv) parameters. */
+int swfwi_inversion_create(swfwi_inversion *out, swfwi_solver h, const float *wlt, const float *dobs, float vmin,
+                           float vmax, int nita, float maxdv);
+int swfwi_inversion_destroy(swfwi_inversion inv);
+/* EssFwiFramework::epoch (essfwiframework.cpp:191-239) */
+int swfwi_ess_epoch(swfwi_inversion inv, int iter);
+/* FwiFramework::epoch (fwiframework.cpp:269-356), all shots batched; the
+ * reference's step length is undefined behaviour, this uses the defined
+ * two-point parabolic rule documented with the library. */
+int swfwi_fwi_epoch(swfwi_inversion inv, int iter);
+/* out[10] = obj1, steplen, alpha2, alpha3, obj2, obj3, updateobj, initobj,
+ * trial forward runs, parabolic flag; ms[2] = gradient, line-search time. */
+int swfwi_inversion_stats(swfwi_inversion inv, float *out, double *ms);
+int swfwi_inversion_gradient(swfwi_inversion inv, float *grad);   /* scaled + masked */
+int swfwi_inversion_direction(swfwi_inversion inv, float *dir);   /* CG direction */
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SWFWI_HIP_H_ */

@@ -1,0 +1,371 @@
+// extern "C" surface of libswfwi_hip.so (declared in include/swfwi_hip.h).
+#include "../../include/swfwi_hip.h"
+
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "hostmath.h"
+#include "solver.h"
+
+using namespace swfwi;
+
+struct swfwi_solver_s {
+  std::unique_ptr<Solver> s;
+};
+struct swfwi_inversion_s {
+  std::unique_ptr<Inversion> inv;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    return SWFWI_OK;
+  } catch (const Error& e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return SWFWI_ERR_RANGE;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return SWFWI_ERR_ARG;
+  }
+}
+
+#define NEED(p)                                             \
+  do {                                                      \
+    if (!(p)) fail(kErrArg, std::string("null argument: ") + #p); \
+  } while (0)
+
+// Device state of the per-step drop-in kernels: one set of buffers grown to
+// the largest grid seen, guarded by a mutex (the reference's static u2 makes
+// these calls non-reentrant; here they are merely serialised).
+struct StepContext {
+  std::mutex mu;
+  std::unique_ptr<Stream> stream;
+  DBuf<float> prev, cur, rv, u2;
+  DampTables damp;
+  int nx = -1, nz = -1, nb = -1;
+  DevPoints none;
+};
+StepContext& step_ctx() {
+  static StepContext c;
+  return c;
+}
+
+int run_step(float* prev, const float* cur, const float* vel, float* u2, int nx, int nz, int nb, bool damped,
+             int precision) {
+  return guarded([&] {
+    NEED(prev);
+    NEED(cur);
+    NEED(vel);
+    if (nx < 2 * kHalo + 1 || nz < 2 * kHalo + 1) fail(kErrArg, "grid smaller than the stencil");
+    if (precision != kExact && precision != kFp32) fail(kErrArg, "unknown precision mode");
+    StepContext& c = step_ctx();
+    std::lock_guard<std::mutex> lock(c.mu);
+    if (!c.stream) c.stream.reset(new Stream());
+    hipStream_t s = c.stream->get();
+    const long long n = (long long)nx * nz;
+    if (damped && (nx != c.nx || nz != c.nz || nb != c.nb)) {
+      build_damping(nx, nz, nb, c.damp, s);
+      c.nx = nx;
+      c.nz = nz;
+      c.nb = nb;
+    }
+    c.prev.upload(prev, (size_t)n, s);
+    c.cur.upload(cur, (size_t)n, s);
+    c.rv.upload(vel, (size_t)n, s);
+    launch_reciprocal(c.rv.get(), c.rv.get(), n, s);
+    c.u2.resize((size_t)n);
+    Grid g;
+    g.nx = nx;
+    g.nz = nz;
+    g.ld = nz;
+    g.ntz = (nz - 2 * kHalo + kTZ - 1) / kTZ;
+    g.ntx = (nx - 2 * kHalo + kTX - 1) / kTX;
+    g.nb = nb;
+    if (damped) {
+      g.delta_x = c.damp.dx.get();
+      g.delta_z = c.damp.dz.get();
+      g.zdamp = c.damp.zdamp;
+    }
+    FwdStepArgs a;
+    a.g = g;
+    a.cur = c.cur.get();
+    a.prev = c.prev.get();
+    a.rv = c.rv.get();
+    a.u2out = u2 ? c.u2.get() : nullptr;
+    launch_fwd_step(a, 1, damped, precision, s);
+    SWFWI_HIP(hipGetLastError());
+    c.prev.download(prev, (size_t)n, s);
+    std::vector<float> hu2;
+    if (u2) {
+      hu2.resize((size_t)n);
+      c.u2.download(hu2.data(), (size_t)n, s);
+    }
+    SWFWI_HIP(hipStreamSynchronize(s));
+    if (u2) {  // the reference writes u2 over [5, n-5) only
+      for (int ix = 5; ix < nx - 5; ix++)
+        for (int iz = 5; iz < nz - 5; iz++) u2[(size_t)ix * nz + iz] = hu2[(size_t)ix * nz + iz];
+    }
+  });
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* swfwi_last_error(void) { return g_last_error.c_str(); }
+const char* swfwi_version(void) { return "swfwi-mi355x 0.1 (gfx950)"; }
+
+int swfwi_device_count(int* count) {
+  return guarded([&] {
+    NEED(count);
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *count = n;
+  });
+}
+
+void fd4t10s_damp_zjh_2d_vtrans(float* prev_wave, const float* curr_wave, const float* vel, float* u2, int nx, int nz,
+                                int nb) {
+  if (run_step(prev_wave, curr_wave, vel, u2, nx, nz, nb, true, kExact) != SWFWI_OK) {
+    std::fprintf(stderr, "fd4t10s_damp_zjh_2d_vtrans: %s\n", g_last_error.c_str());
+    std::exit(1);
+  }
+}
+
+void fd4t10s_zjh_2d_vtrans(float* prev_wave, const float* curr_wave, const float* vel, float* u2, int nx, int nz) {
+  if (run_step(prev_wave, curr_wave, vel, u2, nx, nz, 0, false, kExact) != SWFWI_OK) {
+    std::fprintf(stderr, "fd4t10s_zjh_2d_vtrans: %s\n", g_last_error.c_str());
+    std::exit(1);
+  }
+}
+
+int swfwi_step_damped(float* prev_wave, const float* curr_wave, const float* vel, float* u2, int nx, int nz, int nb,
+                      int precision) {
+  return run_step(prev_wave, curr_wave, vel, u2, nx, nz, nb, true, precision);
+}
+
+int swfwi_step_undamped(float* prev_wave, const float* curr_wave, const float* vel, float* u2, int nx, int nz,
+                        int precision) {
+  return run_step(prev_wave, curr_wave, vel, u2, nx, nz, 0, false, precision);
+}
+
+int swfwi_expand_domain(const float* v, int nx, int nz, int nb, float dx, float dt, float* out) {
+  return guarded([&] {
+    NEED(v);
+    NEED(out);
+    if (nx <= 0 || nz <= 0 || nb < 0) fail(kErrArg, "bad model size");
+    expand_domain(v, nx, nz, nb, dx, dt, out);
+  });
+}
+
+int swfwi_ricker(float* wlt, int nt, float fm, float dt, float amp) {
+  return guarded([&] {
+    NEED(wlt);
+    ricker_wavelet(wlt, nt, fm, dt, amp);
+  });
+}
+
+int swfwi_random_codes(int seed, int skip, int n, int* codes) {
+  return guarded([&] {
+    NEED(codes);
+    CodeStream cs(seed);
+    if (skip > 0) cs.next(skip);
+    std::vector<int> c = cs.next(n);
+    for (int i = 0; i < n; i++) codes[i] = c[i];
+  });
+}
+
+int swfwi_solver_create(swfwi_solver* out, int nx, int nz, int nb, float dt, float dx, float fm, int nt, int ns,
+                        const int* sx, const int* sz, int ng, const int* gx, const int* gz, int precision) {
+  return guarded([&] {
+    NEED(out);
+    NEED(sx);
+    NEED(sz);
+    NEED(gx);
+    NEED(gz);
+    if (ns <= 0 || ng <= 0) fail(kErrArg, "need sources and receivers");
+    Geometry g;
+    g.nx = nx;
+    g.nz = nz;
+    g.nb = nb;
+    g.dt = dt;
+    g.dx = dx;
+    g.fm = fm;
+    g.nt = nt;
+    g.sx.assign(sx, sx + ns);
+    g.sz.assign(sz, sz + ns);
+    g.gx.assign(gx, gx + ng);
+    g.gz.assign(gz, gz + ng);
+    std::unique_ptr<swfwi_solver_s> h(new swfwi_solver_s);
+    h->s.reset(new Solver(g, precision));
+    *out = h.release();
+  });
+}
+
+int swfwi_solver_destroy(swfwi_solver h) {
+  return guarded([&] { delete h; });
+}
+
+int swfwi_solver_dims(swfwi_solver h, int* dims) {
+  return guarded([&] {
+    NEED(h);
+    NEED(dims);
+    const Geometry& g = h->s->geo();
+    dims[0] = g.nxpad;
+    dims[1] = g.nzpad;
+    dims[2] = g.bx0;
+    dims[3] = g.bz0;
+    dims[4] = g.bsize;
+    dims[5] = g.ntz * g.ntx;
+  });
+}
+
+int swfwi_solver_set_precision(swfwi_solver h, int precision) {
+  return guarded([&] {
+    NEED(h);
+    if (precision != kExact && precision != kFp32) fail(kErrArg, "unknown precision mode");
+    h->s->prop().set_precision(precision);
+  });
+}
+
+int swfwi_solver_bind_velocity(swfwi_solver h, const float* vel_t) {
+  return guarded([&] {
+    NEED(h);
+    NEED(vel_t);
+    h->s->bind_velocity(vel_t);
+  });
+}
+
+int swfwi_solver_get_velocity(swfwi_solver h, float* vel_t) {
+  return guarded([&] {
+    NEED(h);
+    NEED(vel_t);
+    h->s->download_velocity(vel_t);
+  });
+}
+
+int swfwi_forward_modeling(swfwi_solver h, const float* encsrc, int shot, float* dcal) {
+  return guarded([&] {
+    NEED(h);
+    NEED(encsrc);
+    NEED(dcal);
+    h->s->forward_modeling(encsrc, shot, dcal);
+  });
+}
+
+int swfwi_calgradient(swfwi_solver h, const float* encsrc, int shot, const float* vsrc, float* grad) {
+  return guarded([&] {
+    NEED(h);
+    NEED(encsrc);
+    NEED(vsrc);
+    NEED(grad);
+    h->s->calgradient(encsrc, shot, vsrc, grad);
+  });
+}
+
+int swfwi_forward_shots(swfwi_solver h, const float* wlt, float* dobs) {
+  return guarded([&] {
+    NEED(h);
+    NEED(wlt);
+    NEED(dobs);
+    h->s->forward_shots(wlt, dobs);
+  });
+}
+
+int swfwi_solver_counters(swfwi_solver h, double* point_updates, long long* launches) {
+  return guarded([&] {
+    NEED(h);
+    if (point_updates) *point_updates = h->s->point_updates();
+    if (launches) *launches = h->s->launches();
+  });
+}
+
+int swfwi_inversion_create(swfwi_inversion* out, swfwi_solver h, const float* wlt, const float* dobs, float vmin,
+                           float vmax, int nita, float maxdv) {
+  return guarded([&] {
+    NEED(out);
+    NEED(h);
+    NEED(wlt);
+    NEED(dobs);
+    UpdateParams p;
+    p.vmin = vmin;
+    p.vmax = vmax;
+    p.nita = nita;
+    p.maxdv = maxdv;
+    std::unique_ptr<swfwi_inversion_s> inv(new swfwi_inversion_s);
+    inv->inv.reset(new Inversion(*h->s, wlt, dobs, p));
+    *out = inv.release();
+  });
+}
+
+int swfwi_inversion_destroy(swfwi_inversion inv) {
+  return guarded([&] { delete inv; });
+}
+
+int swfwi_ess_epoch(swfwi_inversion inv, int iter) {
+  return guarded([&] {
+    NEED(inv);
+    inv->inv->ess_epoch(iter);
+  });
+}
+
+int swfwi_fwi_epoch(swfwi_inversion inv, int iter) {
+  return guarded([&] {
+    NEED(inv);
+    inv->inv->fwi_epoch(iter);
+  });
+}
+
+int swfwi_inversion_stats(swfwi_inversion inv, float* out, double* ms) {
+  return guarded([&] {
+    NEED(inv);
+    const EpochStats& s = inv->inv->stats();
+    if (out) {
+      out[0] = s.obj1;
+      out[1] = s.steplen;
+      out[2] = s.alpha2;
+      out[3] = s.alpha3;
+      out[4] = s.obj2;
+      out[5] = s.obj3;
+      out[6] = s.updateobj;
+      out[7] = s.initobj;
+      out[8] = (float)s.ntrial;
+      out[9] = (float)s.parabolic;
+    }
+    if (ms) {
+      ms[0] = s.grad_ms;
+      ms[1] = s.linesearch_ms;
+    }
+  });
+}
+
+int swfwi_inversion_gradient(swfwi_inversion inv, float* grad) {
+  return guarded([&] {
+    NEED(inv);
+    NEED(grad);
+    inv->inv->download_grad(grad);
+  });
+}
+
+int swfwi_inversion_direction(swfwi_inversion inv, float* dir) {
+  return guarded([&] {
+    NEED(inv);
+    NEED(dir);
+    inv->inv->download_direction(dir);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,117 @@
+// The device-resident acoustic propagator: the MI355X counterpart of
+// Damp4t10d's time loops (damp4t10d.cpp:272-307) and of calgradient
+// (fwiframework.cpp:114-177, essfwiframework.cpp:114-173), batched over
+// entries (shots, ensemble members, line-search trials).
+#pragma once
+#include <memory>
+#include <vector>
+
+#include "device.h"
+#include "kernels.h"
+#include "ops.h"
+
+namespace swfwi {
+
+// Acquisition + grid description, in the reference's terms: physical model
+// size nz x nx, sponge nb, positions as physical cell indices
+// (ShotPosition::getx/getz, shot-position.cpp:11-37).
+struct Geometry {
+  int nx = 0, nz = 0, nb = 0;
+  float dt = 0, dx = 0, fm = 0;
+  int nt = 0;
+  std::vector<int> sx, sz, gx, gz;
+
+  // derived (Damp4t10d ctor + expandDomain, damp4t10d.cpp:138-151,367-376)
+  int nxpad = 0, nzpad = 0, ld = 0;
+  int bx0 = 0, bxn = 0, bz0 = 0, bzn = 0;
+  int bsize = 0;      // floats of one step's boundary band (initBndryVector)
+  int ntz = 0, ntx = 0;
+  int ns() const { return (int)sx.size(); }
+  int ng() const { return (int)gx.size(); }
+  long long npts() const { return (long long)nxpad * ld; }
+  void derive();      // fill the derived members; throws kErrArg on bad input
+  GridGeom gridgeom() const { return GridGeom{nxpad, nzpad, ld, bx0, bxn, bz0, bzn}; }
+};
+
+// Per-tile CSR of a point set on the device.
+struct DevPoints {
+  DBuf<int> tile_ptr, items, idx;
+  int n = 0;
+  PointList view() const { return PointList{tile_ptr.get(), items.get(), idx.get()}; }
+};
+
+// Damping tables of the damped kernel for a padded grid and width nb.
+struct DampTables {
+  DBuf<float> dx, dz;
+  int zdamp = 0;
+};
+void build_damping(int nx, int nz, int nb, DampTables& t, hipStream_t s);
+void build_points(const std::vector<long long>& idx, int ld, int ntz, int ntx, DevPoints& out, hipStream_t s);
+
+// Wavefield workspace for E entries (two buffers per wavefield).
+struct Fields {
+  int E = 0;
+  DBuf<float> a, b, c, d;       // source field pair, receiver field pair
+  float* s_cur = nullptr;       // source field state after forward(): cur of the last step
+  float* s_nxt = nullptr;       //   ... and the field it produced
+  void ensure(int E, long long npts, bool receiver);
+};
+
+class Propagator {
+ public:
+  Propagator(const Geometry& g, int precision);
+
+  const Geometry& geo() const { return g_; }
+  int precision() const { return prec_; }
+  void set_precision(int p) { prec_ = p; }
+  hipStream_t stream() const { return stream_.get(); }
+  void sync() const { stream_.sync(); }
+  Grid grid() const;
+  const DevPoints& src_points() const { return src_; }
+  const DevPoints& rec_points() const { return rec_; }
+  const int* src_xp() const { return sxp_.get(); }  // padded coordinates (device)
+  const int* src_zp() const { return szp_.get(); }
+  const int* rec_xp() const { return gxp_.get(); }
+  const int* rec_zp() const { return gzp_.get(); }
+
+  // Source amplitudes for forward()/backward(): row t of entry e is
+  // amp + e*estride + t*ld; entry e injects source ids [lo[e], hi[e]) (device
+  // arrays; nullptr = every source).
+  struct Sources {
+    const float* amp = nullptr;
+    int ld = 0;
+    long long estride = 0;
+    const int* lo = nullptr;
+    const int* hi = nullptr;
+  };
+
+  // nt steps of the damped forward propagation for E entries with models
+  // rv + e*rvstride.  Records [E][nt][ng] into dcal and/or saves the boundary
+  // band [E][nt][bsize] when those are non-null.
+  void forward(int E, const float* rv, long long rvstride, const Sources& src, float* dcal, float* band, Fields& w,
+               int rvdiv = 1);
+
+  // The backward half of calgradient on the state forward() left in w:
+  // vsrc [E][nt][ng] adjoint sources (+e*vstride), grad accumulates the
+  // imaging condition (+e*gstride; caller zeroes it).  Returns the number of
+  // backward iterations run.
+  int backward(int E, const float* rv, long long rvstride, const Sources& src, const float* vsrc, long long vstride,
+               const float* band, float* grad, long long gstride, Fields& w, int rvdiv = 1);
+
+  // The backward iterations calgradient performs with an imaging condition
+  // (it = nt-1 down to the returned value), with their taper weights.
+  static int xcorr_range(float dt, int nt, std::vector<float>& weights);
+
+  // launch counters for throughput accounting
+  long long fwd_launches = 0, bwd_launches = 0;
+
+ private:
+  Geometry g_;
+  int prec_;
+  Stream stream_;
+  DampTables damp_;
+  DevPoints src_, rec_;
+  DBuf<int> sxp_, szp_, gxp_, gzp_;
+};
+
+}  // namespace swfwi

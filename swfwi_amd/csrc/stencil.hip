@@ -1,0 +1,361 @@
+// The damped/undamped 4t10s leapfrog step as fused, LDS-tiled CDNA4 kernels.
+//
+// One launch advances every entry of a batch (shots, ensemble members, line
+// search trials) by one time step.  Around the stencil it folds in the
+// per-step bookkeeping that the reference does in separate host loops:
+//   forward  (Damp4t10d::FwiForwardModeling/EssForwardModeling, calgradient's
+//            forward half, damp4t10d.cpp:272-307, fwiframework.cpp:137-144):
+//            record cur at receivers, save cur's boundary band, inject the next
+//            step's source amplitudes into next;
+//   backward (calgradient's backward half, fwiframework.cpp:146-176): undamped
+//            reconstruction of the source field + damped receiver field in the
+//            same tile, subSource, the zero-lag imaging condition, and the
+//            band restore / receiver injection that the next iteration needs.
+//
+// A tile is kTZ x kTX output points.  Its `cur` (with a 6-cell halo) is staged
+// in LDS, the unscaled 10th-order Laplacian u2 is formed in LDS for the tile
+// plus a 1-cell ring, and each thread then finishes kTX*kTZ/kThreads points of
+// one z-row.  HBM traffic per point and step: read cur, prev, rv; write next.
+//
+// Precision: kExact reproduces the reference's expression trees
+// (fd4t10s-damp-zjh.c:44-54,85-88; fd4t10s-zjh.c:28-38,49-52): float ring
+// sums, float coefficient products, double accumulation of -4.0*a0*p and the
+// (2-2d+d^2)p / 2p terms, one rounding to float on store, no contraction.  The
+// library is compiled with -ffp-contract=off; kFp32 uses explicit fmaf.
+#include "kernels.h"
+
+namespace swfwi {
+namespace {
+
+constexpr int LZ = kTZ + 2 * kHalo;  // LDS row length of the staged cur tile
+constexpr int LX = kTX + 2 * kHalo;
+constexpr int UZ = kTZ + 2;          // LDS row length of the u2 tile
+constexpr int UX = kTX + 2;
+constexpr int kRows = kThreads / kTZ;          // thread rows per block
+constexpr int kPer = kTX / kRows;              // outputs per thread
+
+// Zhang Jinhai coefficients as float, as the reference stores them.
+constexpr float A0 = 1.53400796f, A1 = 1.78858721f, A2 = -0.31660756f;
+constexpr float A3 = 0.07612173f, A4 = -0.01626042f, A5 = 0.00216736f;
+constexpr double M4A0 = -4.0 * (double)A0;
+constexpr float C12 = 1.0f / 12;
+
+template <int P>
+__device__ __forceinline__ float u2_at(const float* __restrict__ s) {
+  // s: LDS element; z neighbours at +-k, x neighbours at +-k*LZ
+  const float r1 = ((s[-1] + s[1]) + s[-LZ]) + s[LZ];
+  const float r2 = ((s[-2] + s[2]) + s[-2 * LZ]) + s[2 * LZ];
+  const float r3 = ((s[-3] + s[3]) + s[-3 * LZ]) + s[3 * LZ];
+  const float r4 = ((s[-4] + s[4]) + s[-4 * LZ]) + s[4 * LZ];
+  const float r5 = ((s[-5] + s[5]) + s[-5 * LZ]) + s[5 * LZ];
+  if constexpr (P == kExact) {
+    double acc = M4A0 * (double)s[0];
+    acc = acc + (double)(A1 * r1);
+    acc = acc + (double)(A2 * r2);
+    acc = acc + (double)(A3 * r3);
+    acc = acc + (double)(A4 * r4);
+    acc = acc + (double)(A5 * r5);
+    return (float)acc;
+  } else {
+    float acc = (-4.0f * A0) * s[0];
+    acc = __builtin_fmaf(A1, r1, acc);
+    acc = __builtin_fmaf(A2, r2, acc);
+    acc = __builtin_fmaf(A3, r3, acc);
+    acc = __builtin_fmaf(A4, r4, acc);
+    acc = __builtin_fmaf(A5, r5, acc);
+    return acc;
+  }
+}
+
+// next from cur c, prev p, reciprocal model rv, u2 tile pointer u (at the
+// point), damping delta (0 inside; ignored when !DAMPED).
+template <bool DAMPED, int P>
+__device__ __forceinline__ float update_at(float c, float p, float rv, const float* __restrict__ u, float delta) {
+  float lap = ((u[-1] + u[1]) + u[-UZ]) + u[UZ];
+  lap = lap - 4 * u[0];
+  if constexpr (P == kExact) {
+    const float cc = (C12 * rv) * rv;
+    double acc;
+    if constexpr (DAMPED) {
+      const double a = (2. - (double)(2 * delta)) + (double)(delta * delta);
+      const float b = 1 - 2 * delta;
+      acc = a * (double)c;
+      acc = acc - (double)(b * p);
+    } else {
+      acc = 2. * (double)c;
+      acc = acc - (double)p;
+    }
+    acc = acc + (double)(rv * u[0]);
+    acc = acc + (double)(cc * lap);
+    return (float)acc;
+  } else {
+    const float cc = (C12 * rv) * rv;
+    float a = 2.0f, b = 1.0f;
+    if constexpr (DAMPED) {
+      a = (2.0f - 2 * delta) + delta * delta;
+      b = 1 - 2 * delta;
+    }
+    float acc = __builtin_fmaf(a, c, -(b * p));
+    acc = __builtin_fmaf(rv, u[0], acc);
+    return __builtin_fmaf(cc, lap, acc);
+  }
+}
+
+// Offset of (ix, iz) in the reference's boundary band (writeBndry layout,
+// damp4t10d.cpp:478-514), or -1 when the point is not in the band.
+__device__ __forceinline__ int band_index(const Grid& g, int ix, int iz) {
+  const int w = 6;
+  const int xl = g.bx0 - w;                          // first band column
+  const int nxb = g.nx - (g.bx0 - w + g.bxn - w);
+  const int nzb = g.nz - g.bz0 - g.bzn;
+  const int zb = g.nz - g.bzn;                       // first bottom band row
+  if (iz >= zb && iz < zb + w && ix >= xl && ix < xl + nxb) return (iz - zb) + w * (ix - xl);
+  if (iz >= g.bz0 && iz < g.bz0 + nzb) {
+    if (ix >= xl && ix < g.bx0) return w * nxb + (iz - g.bz0) + nzb * (ix - xl);
+    const int xr = g.nx - g.bxn;
+    if (ix >= xr && ix < xr + w) return w * nxb + (iz - g.bz0) + nzb * (ix - xr + w);
+  }
+  return -1;
+}
+
+__device__ __forceinline__ float damping_delta(const Grid& g, int ix, int iz) {
+  return iz >= g.zdamp ? g.delta_z[iz] : g.delta_x[ix];
+}
+
+// Stage cur (tile + halo) into LDS; zero outside the grid.
+__device__ __forceinline__ void stage_tile(const Grid& g, const float* __restrict__ src, float* __restrict__ lds,
+                                           int x0, int z0) {
+  for (int i = threadIdx.x; i < LX * LZ; i += kThreads) {
+    const int lx = i / LZ, lz = i - lx * LZ;
+    const int gx = x0 - kHalo + lx, gz = z0 - kHalo + lz;
+    float v = 0.f;
+    if (gx >= 0 && gx < g.nx && gz >= 0 && gz < g.nz) v = src[(long long)gx * g.ld + gz];
+    lds[i] = v;
+  }
+}
+
+// u2 over the tile plus a one-cell ring, only where the reference defines it
+// ([5, n-5) in both directions); elsewhere 0 (never read by a valid output).
+template <int P>
+__device__ __forceinline__ void form_u2(const Grid& g, const float* __restrict__ cur, float* __restrict__ u2,
+                                        int x0, int z0) {
+  for (int i = threadIdx.x; i < UX * UZ; i += kThreads) {
+    const int ux = i / UZ, uz = i - ux * UZ;
+    const int gx = x0 - 1 + ux, gz = z0 - 1 + uz;
+    float v = 0.f;
+    if (gx >= 5 && gx < g.nx - 5 && gz >= 5 && gz < g.nz - 5) v = u2_at<P>(cur + (ux + kHalo - 1) * LZ + (uz + kHalo - 1));
+    u2[i] = v;
+  }
+}
+
+__device__ __forceinline__ bool entry_active(const Inject& in, int e, int pid) {
+  return in.lo == nullptr || (pid >= in.lo[e] && pid < in.hi[e]);
+}
+
+// Apply all active point amplitudes of this tile that land on grid index q.
+__device__ __forceinline__ float apply_points(const Inject& in, int tile, int e, long long q, float v) {
+  const int b = in.pts.tile_ptr[tile], eidx = in.pts.tile_ptr[tile + 1];
+  for (int j = b; j < eidx; j++) {
+    const int pid = in.pts.items[j];
+    if (in.pts.idx[pid] == q && entry_active(in, e, pid)) {
+      const float a = in.amp[e * in.estride + (long long)in.t * in.ld + pid];
+      v = in.sign > 0 ? v + a : v - a;
+    }
+  }
+  return v;
+}
+
+template <bool DAMPED, int P>
+__global__ __launch_bounds__(kThreads) void k_fwd_step(FwdStepArgs a) {
+  __shared__ float s_cur[LX * LZ];
+  __shared__ float s_u2[UX * UZ];
+  const Grid& g = a.g;
+  const int tile = blockIdx.x, e = blockIdx.y;
+  const int tz = tile % g.ntz, tx = tile / g.ntz;
+  const int z0 = kHalo + tz * kTZ, x0 = kHalo + tx * kTX;
+  const long long fo = (long long)e * a.fstride;
+  const float* __restrict__ cur = a.cur + fo;
+  float* __restrict__ prev = a.prev + fo;
+  const float* __restrict__ rv = a.rv + (long long)(e / a.rvdiv) * a.rvstride;
+
+  stage_tile(g, cur, s_cur, x0, z0);
+  __syncthreads();
+  form_u2<P>(g, s_cur, s_u2, x0, z0);
+  __syncthreads();
+
+  const int lz = threadIdx.x % kTZ, row = threadIdx.x / kTZ;
+  const int iz = z0 + lz;
+  const bool has_inj = a.inj.t >= 0 && a.inj.pts.tile_ptr[tile + 1] > a.inj.pts.tile_ptr[tile];
+  const bool has_rec = a.rec.out != nullptr && a.rec.pts.tile_ptr[tile + 1] > a.rec.pts.tile_ptr[tile];
+#pragma unroll
+  for (int k = 0; k < kPer; k++) {
+    const int lx = row + kRows * k;
+    const int ix = x0 + lx;
+    if (ix >= g.nx - kHalo || iz >= g.nz - kHalo) continue;
+    const long long q = (long long)ix * g.ld + iz;
+    const float c = s_cur[(lx + kHalo) * LZ + lz + kHalo];
+    const float* u = s_u2 + (lx + 1) * UZ + lz + 1;
+    if (a.u2out) a.u2out[fo + q] = u[0];
+    const float delta = DAMPED ? damping_delta(g, ix, iz) : 0.f;
+    float nxt = update_at<DAMPED, P>(c, prev[q], rv[q], u, delta);
+    if (has_inj) nxt = apply_points(a.inj, tile, e, q, nxt);
+    prev[q] = nxt;
+    if (a.band) {
+      const int bi = band_index(g, ix, iz);
+      if (bi >= 0) a.band[(long long)e * a.band_estride + bi] = c;
+    }
+    if (has_rec) {
+      const int b = a.rec.pts.tile_ptr[tile], eidx = a.rec.pts.tile_ptr[tile + 1];
+      for (int j = b; j < eidx; j++) {
+        const int pid = a.rec.pts.items[j];
+        if (a.rec.pts.idx[pid] == q) a.rec.out[(long long)e * a.rec.estride + pid] = c;
+      }
+    }
+  }
+  // u2 of the ring [5,6) and [n-6,n-5) that no output point owns: the
+  // reference writes it too, so the optional u2 export covers it.
+  if (a.u2out) {
+    for (int i = threadIdx.x; i < UX * UZ; i += kThreads) {
+      const int ux = i / UZ, uz = i - ux * UZ;
+      const int gx = x0 - 1 + ux, gz = z0 - 1 + uz;
+      // points of the ring may sit in two tiles' u2 halos; both write the
+      // same value
+      const bool ring = gx == 5 || gz == 5 || gx == g.nx - 6 || gz == g.nz - 6;
+      if (ring && gx >= 5 && gx < g.nx - 5 && gz >= 5 && gz < g.nz - 5)
+        a.u2out[fo + (long long)gx * g.ld + gz] = s_u2[i];
+    }
+  }
+}
+
+template <int P>
+__global__ __launch_bounds__(kThreads) void k_bwd_step(BwdStepArgs a) {
+  __shared__ float s_sc[LX * LZ];
+  __shared__ float s_gc[LX * LZ];
+  __shared__ float s_su[UX * UZ];
+  __shared__ float s_gu[UX * UZ];
+  const Grid& g = a.g;
+  const int tile = blockIdx.x, e = blockIdx.y;
+  const int tz = tile % g.ntz, tx = tile / g.ntz;
+  const int z0 = kHalo + tz * kTZ, x0 = kHalo + tx * kTX;
+  const long long fo = (long long)e * a.fstride;
+  const float* __restrict__ rv = a.rv + (long long)(e / a.rvdiv) * a.rvstride;
+  float* __restrict__ s_prev = a.s_prev + fo;
+  float* __restrict__ g_prev = a.g_prev + fo;
+  float* __restrict__ grad = a.grad + (long long)e * a.gstride;
+
+  stage_tile(g, a.s_cur + fo, s_sc, x0, z0);
+  stage_tile(g, a.g_cur + fo, s_gc, x0, z0);
+  __syncthreads();
+  form_u2<P>(g, s_sc, s_su, x0, z0);
+  form_u2<P>(g, s_gc, s_gu, x0, z0);
+  __syncthreads();
+
+  const int lz = threadIdx.x % kTZ, row = threadIdx.x / kTZ;
+  const int iz = z0 + lz;
+  const bool has_sub = a.s_sub.t >= 0 && a.s_sub.pts.tile_ptr[tile + 1] > a.s_sub.pts.tile_ptr[tile];
+  const bool has_inj = a.g_inj.t >= 0 && a.g_inj.pts.tile_ptr[tile + 1] > a.g_inj.pts.tile_ptr[tile];
+#pragma unroll
+  for (int k = 0; k < kPer; k++) {
+    const int lx = row + kRows * k;
+    const int ix = x0 + lx;
+    if (ix >= g.nx - kHalo || iz >= g.nz - kHalo) continue;
+    const long long q = (long long)ix * g.ld + iz;
+    const int l = (lx + kHalo) * LZ + lz + kHalo;
+    const int lu = (lx + 1) * UZ + lz + 1;
+    const float r = rv[q];
+    // source field: stepBackward (undamped) then subSource
+    float sn = update_at<false, P>(s_sc[l], s_prev[q], r, s_su + lu, 0.f);
+    if (has_sub) sn = apply_points(a.s_sub, tile, e, q, sn);
+    // receiver field: stepForward (damped) of the already injected cur
+    const float gc = s_gc[l];
+    float gn = update_at<true, P>(gc, g_prev[q], r, s_gu + lu, damping_delta(g, ix, iz));
+    // zero-lag imaging condition with the new source field and the injected
+    // receiver field (cross_correlation, fwiframework.cpp:107-112)
+    if (a.do_xcorr) grad[q] = grad[q] - sn * gc * a.weight;
+    // what the next iteration (it-1) expects to find: the band of the source
+    // field restored (readBndry) and the receivers injected (addSource)
+    if (a.band) {
+      const int bi = band_index(g, ix, iz);
+      if (bi >= 0) sn = a.band[(long long)e * a.band_estride + bi];
+    }
+    if (has_inj) gn = apply_points(a.g_inj, tile, e, q, gn);
+    s_prev[q] = sn;
+    g_prev[q] = gn;
+  }
+}
+
+__global__ void k_fill(float* p, long long n, float v) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+__global__ void k_fill_d(double* p, long long n, double v) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+// Sequential per entry (one lane each), so coincident points accumulate in
+// the reference's order (manipSource, damp4t10d.cpp:204-214).
+__global__ void k_point_inject(float* field, long long fstride, Inject in, int npoints, int nentries) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nentries) return;
+  float* f = field + (long long)e * fstride;
+  const int lo = in.lo ? in.lo[e] : 0, hi = in.hi ? in.hi[e] : npoints;
+  for (int pid = lo; pid < hi; pid++) {
+    const float a = in.amp[e * in.estride + (long long)in.t * in.ld + pid];
+    const long long q = in.pts.idx[pid];
+    f[q] = in.sign > 0 ? f[q] + a : f[q] - a;
+  }
+}
+
+__global__ void k_reciprocal(const float* vel, float* rv, long long n) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    rv[i] = 1.0f / vel[i];
+}
+
+inline int grid_for(long long n) {
+  long long b = (n + 255) / 256;
+  return (int)(b > 8192 ? 8192 : (b < 1 ? 1 : b));
+}
+
+}  // namespace
+
+void launch_fwd_step(const FwdStepArgs& a, int nentries, bool damped, int precision, hipStream_t stream) {
+  dim3 grid(a.g.ntz * a.g.ntx, nentries);
+  if (damped) {
+    if (precision == kExact)
+      hipLaunchKernelGGL((k_fwd_step<true, kExact>), grid, dim3(kThreads), 0, stream, a);
+    else
+      hipLaunchKernelGGL((k_fwd_step<true, kFp32>), grid, dim3(kThreads), 0, stream, a);
+  } else {
+    if (precision == kExact)
+      hipLaunchKernelGGL((k_fwd_step<false, kExact>), grid, dim3(kThreads), 0, stream, a);
+    else
+      hipLaunchKernelGGL((k_fwd_step<false, kFp32>), grid, dim3(kThreads), 0, stream, a);
+  }
+}
+
+void launch_bwd_step(const BwdStepArgs& a, int nentries, int precision, hipStream_t stream) {
+  dim3 grid(a.g.ntz * a.g.ntx, nentries);
+  if (precision == kExact)
+    hipLaunchKernelGGL((k_bwd_step<kExact>), grid, dim3(kThreads), 0, stream, a);
+  else
+    hipLaunchKernelGGL((k_bwd_step<kFp32>), grid, dim3(kThreads), 0, stream, a);
+}
+
+void launch_fill(float* p, long long n, float v, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(256), 0, s, p, n, v);
+}
+void launch_fill_d(double* p, long long n, double v, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_fill_d, dim3(grid_for(n)), dim3(256), 0, s, p, n, v);
+}
+void launch_point_inject(float* field, long long fstride, const Inject& inj, int npoints, int nentries,
+                         hipStream_t s) {
+  if (inj.t < 0 || npoints <= 0 || nentries <= 0) return;
+  hipLaunchKernelGGL(k_point_inject, dim3((nentries + 63) / 64), dim3(64), 0, s, field, fstride, inj, npoints,
+                     nentries);
+}
+void launch_reciprocal(const float* vel, float* rv, long long n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_reciprocal, dim3(grid_for(n)), dim3(256), 0, s, vel, rv, n);
+}
+
+}  // namespace swfwi

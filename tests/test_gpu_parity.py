@@ -1,0 +1,217 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle on the
+same seeded inputs.
+
+Tolerances: in exact mode the kernels reproduce the reference's expression
+trees, so seismograms and gradients must be bit-identical; in FP32 mode the
+north-star budgets apply (seismogram relative L2 <= 1e-5, gradient relative
+L2 <= 1e-4)."""
+import numpy as np
+import pytest
+
+from swfwi_amd import capi, workloads
+
+pytestmark = pytest.mark.gpu
+
+SEIS_TOL = 1e-5
+GRAD_TOL = 1e-4
+
+
+@pytest.fixture(scope="module", autouse=True)
+def need_gpu():
+    if capi.device_count() < 1:
+        pytest.skip("no HIP device")
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+def rel_l2(a, b):
+    a = a.astype(np.float64)
+    b = b.astype(np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def setup_case(oracle, case, precision=capi.PREC_EXACT):
+    vtrue, v0 = workloads.models(case)
+    vt = capi.expand_domain(vtrue, case.nb, case.dx, case.dt)
+    vi = capi.expand_domain(v0, case.nb, case.dx, case.dt)
+    s = capi.Solver(case.nx, case.nz, case.nb, case.dt, case.dx, case.fm, case.nt, case.sx, case.sz, case.gx,
+                    case.gz, precision)
+    g = oracle.Geom(case.nx, case.nz, case.nb, case.dt, case.dx, case.fm, case.nt, case.sx, case.sz, case.gx,
+                    case.gz)
+    wlt = capi.ricker(case.nt, case.fm, case.dt, case.amp)
+    return s, g, vt, vi, wlt
+
+
+# -- per-step kernels ----------------------------------------------------------
+
+@pytest.mark.parametrize("nx,nz,nb", [(48, 40, 14), (353, 233, 26), (101, 77, 36), (19, 13, 4)])
+def test_step_kernels_bitexact(oracle, nx, nz, nb):
+    print(f"step kernels nx={nx} nz={nz} nb={nb}", flush=True)
+    rng = np.random.default_rng(nx * 1000 + nz)
+    cur = rng.standard_normal((nx, nz)).astype(np.float32)
+    prev = rng.standard_normal((nx, nz)).astype(np.float32)
+    vel = rng.uniform(4.9, 44.0, (nx, nz)).astype(np.float32)
+    sl = (slice(5, nx - 5), slice(5, nz - 5))
+    a, ua = prev.copy(), np.zeros_like(cur)
+    capi.load().fd4t10s_damp_zjh_2d_vtrans(a, cur, vel, ua, nx, nz, nb)
+    b, ub = prev.copy(), np.zeros_like(cur)
+    oracle.fd4t10s_damp(b, cur, vel, ub, nb)
+    assert np.array_equal(bits(a), bits(b))
+    assert np.array_equal(bits(ua[sl]), bits(ub[sl]))
+    a, ua = prev.copy(), np.zeros_like(cur)
+    capi.load().fd4t10s_zjh_2d_vtrans(a, cur, vel, ua, nx, nz)
+    b, ub = prev.copy(), np.zeros_like(cur)
+    oracle.fd4t10s(b, cur, vel, ub)
+    assert np.array_equal(bits(a), bits(b))
+    assert np.array_equal(bits(ua[sl]), bits(ub[sl]))
+    # FP32 mode: same step within a few ulps
+    c, uc = prev.copy(), np.zeros_like(cur)
+    capi.step_damped(c, cur, vel, uc, nb, capi.PREC_FP32)
+    b, ub = prev.copy(), np.zeros_like(cur)
+    oracle.fd4t10s_damp(b, cur, vel, ub, nb)
+    assert rel_l2(c, b) < 1e-6
+
+
+def test_step_matches_golden_fixture():
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "fd4t10s_golden.npz"))
+    cur, prev, vel, nb = g["c0_cur"], g["c0_prev"], g["c0_vel"], int(g["c0_nb"])
+    p, u = prev.copy(), np.zeros_like(cur)
+    capi.step_damped(p, cur, vel, u, nb)
+    assert np.array_equal(bits(p), bits(g["c0_damp_next"]))
+
+
+# -- time loops ----------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def small(oracle):
+    case = workloads.small_case()
+    s, g, vt, vi, wlt = setup_case(oracle, case)
+    s.bind_velocity(vt)
+    return case, s, g, vt, vi, wlt
+
+
+def test_fwi_forward_modeling_bitexact(oracle, small):
+    case, s, g, vt, vi, wlt = small
+    for shot in (0, case.ns - 1):
+        print(f"FwiForwardModeling shot={shot}", flush=True)
+        d_gpu = s.forward_modeling(wlt, shot=shot)
+        d_cpu = g.forward_modeling(vt, wlt, shot=shot)
+        assert np.array_equal(bits(d_gpu), bits(d_cpu))
+
+
+def test_ess_forward_modeling_bitexact(oracle, small):
+    case, s, g, vt, vi, wlt = small
+    code = capi.random_codes(1, case.ns)
+    enc = oracle.encode_source(code, wlt)
+    d_gpu = s.forward_modeling(enc, shot=-1)
+    d_cpu = g.forward_modeling(vt, enc, shot=-1)
+    assert np.array_equal(bits(d_gpu), bits(d_cpu))
+
+
+def test_forward_shots_bitexact(oracle, small):
+    case, s, g, vt, vi, wlt = small
+    dobs = s.forward_shots(wlt)
+    for shot in range(case.ns):
+        assert np.array_equal(bits(dobs[shot]), bits(g.forward_modeling(vt, wlt, shot=shot)))
+
+
+@pytest.mark.parametrize("shot", [1, -1])
+def test_calgradient_bitexact(oracle, small, shot):
+    case, s, g, vt, vi, wlt = small
+    print(f"calgradient shot={shot}", flush=True)
+    rng = np.random.default_rng(11)
+    vsrc = rng.standard_normal((case.nt, case.ng)).astype(np.float32)
+    enc = wlt if shot >= 0 else oracle.encode_source(capi.random_codes(1, case.ns), wlt)
+    g_gpu = s.calgradient(enc, vsrc, shot=shot)
+    g_cpu = g.calgradient(vt, enc, vsrc, shot=shot)
+    assert np.count_nonzero(g_cpu) > 0
+    assert np.array_equal(bits(g_gpu), bits(g_cpu))
+
+
+def test_fp32_mode_within_budget(oracle, small):
+    case, s, g, vt, vi, wlt = small
+    s.set_precision(capi.PREC_FP32)
+    try:
+        d_gpu = s.forward_modeling(wlt, shot=1)
+        d_cpu = g.forward_modeling(vt, wlt, shot=1)
+        e1 = rel_l2(d_gpu, d_cpu)
+        rng = np.random.default_rng(12)
+        vsrc = rng.standard_normal((case.nt, case.ng)).astype(np.float32)
+        gg = s.calgradient(wlt, vsrc, shot=1)
+        gc = g.calgradient(vt, wlt, vsrc, shot=1)
+        sl = (slice(s.bx0, s.nxpad - s.bx0), slice(s.bz0, s.nzpad - s.bx0))
+        e2 = rel_l2(gg[sl], gc[sl])
+    finally:
+        s.set_precision(capi.PREC_EXACT)
+    print(f"fp32 seismogram relL2={e1:.3e} gradient relL2={e2:.3e}", flush=True)
+    assert e1 <= SEIS_TOL
+    assert e2 <= GRAD_TOL
+
+
+# -- drivers -------------------------------------------------------------------
+
+def test_ess_epochs_match_oracle(oracle, small):
+    case, s, g, vt, vi, wlt = small
+    dobs = s.forward_shots(wlt)
+    oracle.set_sum_mode(1)  # FP64 accumulation, as the GPU reductions
+    try:
+        s.bind_velocity(vi)
+        inv = capi.Inversion(s, wlt, dobs, 1400.0, 3600.0, 3, 200.0)
+        e = oracle.Ess(g, vi.copy(), wlt, dobs, 1400.0, 3600.0, 3, 200.0)
+        for it in range(2):
+            print(f"ess epoch {it}", flush=True)
+            inv.ess_epoch(it)
+            e.epoch(it)
+            a, b = inv.stats(), e.scalars()
+            for k in ("obj1", "steplen", "alpha2", "alpha3", "obj2", "obj3", "updateobj", "ntrial", "parabolic"):
+                assert abs(a[k] - b[k]) <= 1e-6 * max(abs(b[k]), 1e-30), (it, k, a[k], b[k])
+            assert rel_l2(inv.gradient(), e.grad()) < 1e-6
+            assert rel_l2(s.velocity(), e.vel) < 1e-6
+        inv.close()
+    finally:
+        oracle.set_sum_mode(0)
+        s.bind_velocity(vt)
+
+
+def test_fwi_epoch_matches_oracle(oracle, small):
+    case, s, g, vt, vi, wlt = small
+    dobs = s.forward_shots(wlt)
+    oracle.set_sum_mode(1)
+    try:
+        s.bind_velocity(vi)
+        inv = capi.Inversion(s, wlt, dobs, 1400.0, 3600.0, 3, 200.0)
+        e = oracle.Ess(g, vi.copy(), wlt, dobs, 1400.0, 3600.0, 3, 200.0)
+        for it in range(2):
+            print(f"fwi epoch {it}", flush=True)
+            inv.fwi_epoch(it)
+            e.fwi_epoch(it)
+            a, b = inv.stats(), e.scalars()
+            for k in ("obj1", "steplen", "obj2", "obj3", "updateobj"):
+                assert abs(a[k] - b[k]) <= 1e-6 * max(abs(b[k]), 1e-30), (it, k, a[k], b[k])
+            assert np.array_equal(bits(inv.gradient()), bits(e.grad()))
+            assert rel_l2(s.velocity(), e.vel) < 1e-6
+        inv.close()
+    finally:
+        oracle.set_sum_mode(0)
+        s.bind_velocity(vt)
+
+
+def test_config0_shot_full_size(oracle):
+    """One config-0 shot at full size (201x301, nb=20, nt=2000): seismogram and
+    raw gradient bit-identical to the oracle."""
+    case = workloads.config0()
+    s, g, vt, vi, wlt = setup_case(oracle, case)
+    s.bind_velocity(vi)
+    print("config0 forward", flush=True)
+    d_gpu = s.forward_modeling(wlt, shot=5)
+    d_cpu = g.forward_modeling(vi, wlt, shot=5)
+    assert np.array_equal(bits(d_gpu), bits(d_cpu))
+    rng = np.random.default_rng(5)
+    vsrc = (rng.standard_normal((case.nt, case.ng)) * 1e-3).astype(np.float32)
+    print("config0 gradient", flush=True)
+    g_gpu = s.calgradient(wlt, vsrc, shot=5)
+    g_cpu = g.calgradient(vi, wlt, vsrc, shot=5)
+    assert np.array_equal(bits(g_gpu), bits(g_cpu))

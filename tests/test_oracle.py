@@ -1,0 +1,357 @@
+"""Pins the CPU oracle: against the reference's own kernels (compiled unchanged
+when the reference tree is present), against the committed golden fixtures,
+and against closed forms / invariants for the pieces no fixture covers."""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD = os.path.join(HERE, "golden", "fd4t10s_golden.npz")
+
+
+def bits(a):
+    return np.ascontiguousarray(a, np.float32).view(np.uint32)
+
+
+# -- stencil kernels ---------------------------------------------------------
+
+def test_kernels_match_golden_fixtures(oracle):
+    g = np.load(GOLD)
+    k = 0
+    while f"c{k}_cur" in g:
+        cur, prev, vel, nb = g[f"c{k}_cur"], g[f"c{k}_prev"], g[f"c{k}_vel"], int(g[f"c{k}_nb"])
+        for par in (0, 1):
+            p, u = prev.copy(), np.zeros_like(cur)
+            oracle.fd4t10s_damp(p, cur, vel, u, nb, par)
+            assert np.array_equal(bits(p), bits(g[f"c{k}_damp_next"])), (k, par)
+            nx, nz = cur.shape
+            sl = (slice(5, nx - 5), slice(5, nz - 5))
+            assert np.array_equal(bits(u[sl]), bits(g[f"c{k}_damp_u2"][sl]))
+        p, u = prev.copy(), np.zeros_like(cur)
+        oracle.fd4t10s(p, cur, vel, u)
+        assert np.array_equal(bits(p), bits(g[f"c{k}_undamp_next"]))
+        k += 1
+    assert k >= 3
+
+
+def test_kernel_config0_grid_golden(oracle):
+    g = np.load(GOLD)
+    nx, nz, nb = 353, 233, 26
+    rng = np.random.default_rng(int(g["c0grid_vel_seed"]))
+    vel = rng.uniform(4.9, 44.0, (nx, nz)).astype(np.float32)
+    p0 = np.zeros((nx, nz), np.float32)
+    p1 = np.zeros((nx, nz), np.float32)
+    u2 = np.zeros((nx, nz), np.float32)
+    p1[nx // 2, 7] = 1.0
+    for _ in range(int(g["c0grid_steps"])):
+        oracle.fd4t10s_damp(p0, p1, vel, u2, nb, 0)
+        p0, p1 = p1, p0
+    assert hashlib.sha256(p1.tobytes()).digest() == g["c0grid_sha256"].tobytes()
+    assert np.array_equal(bits(p1[nx // 2]), bits(g["c0grid_column"]))
+
+
+def test_kernels_match_reference_build(oracle):
+    ref = oracle.ref_lib()
+    if ref is None:
+        pytest.skip("reference kernels not built here (oracle/_ref absent)")
+    rng = np.random.default_rng(7)
+    for nx, nz, nb in [(31, 27, 9), (64, 50, 26), (17, 13, 3)]:
+        cur = rng.standard_normal((nx, nz)).astype(np.float32) * 1e-3
+        prev = rng.standard_normal((nx, nz)).astype(np.float32) * 1e-3
+        vel = rng.uniform(3.0, 60.0, (nx, nz)).astype(np.float32)
+        a, b = prev.copy(), prev.copy()
+        ua, ub = np.zeros_like(cur), np.zeros_like(cur)
+        ref.fd4t10s_damp_zjh_2d_vtrans(a, cur, vel, ua, nx, nz, nb)
+        oracle.fd4t10s_damp(b, cur, vel, ub, nb, 0)
+        assert np.array_equal(bits(a), bits(b))
+        a, b = prev.copy(), prev.copy()
+        ref.fd4t10s_zjh_2d_vtrans(a, cur, vel, ua, nx, nz)
+        oracle.fd4t10s(b, cur, vel, ub)
+        assert np.array_equal(bits(a), bits(b))
+
+
+def test_constant_field_closed_form(oracle):
+    # u2(c) = -4 a0 c + sum_k a_k * 4c (the coefficients do not sum to zero in
+    # float: 4*(sum a_1..5 - a_0) ~ 1.65e-6), and next = 2c - c + r2*u2 + 0
+    a = np.array([1.53400796, 1.78858721, -0.31660756, 0.07612173, -0.01626042, 0.00216736], np.float32)
+    nx, nz, c = 30, 24, np.float32(0.75)
+    cur = np.full((nx, nz), c, np.float32)
+    vel = np.full((nx, nz), 8.0, np.float32)
+    p, u = cur.copy(), np.zeros_like(cur)
+    oracle.fd4t10s(p, cur, vel, u)
+    expect_u2 = np.float64(-4.0) * np.float64(a[0]) * np.float64(c) + sum(
+        np.float64(np.float32(a[k]) * np.float32(4 * c)) for k in range(1, 6))
+    assert u[12, 12] == np.float32(expect_u2)
+    assert abs(float(u[12, 12]) / float(c) - 4 * (float(a[1:].astype(np.float64).sum()) - float(a[0]))) < 1e-6
+    # interior: Lap5 of a constant u2 is 0 -> next = c + u2/8
+    assert p[12, 12] == np.float32(np.float64(2.0) * c - c + np.float64(np.float32(np.float32(1 / 8.0) * u[12, 12])))
+
+
+def test_undamped_step_is_time_reversible(oracle):
+    # p2 = 2 p1 - p0 + L p1  =>  p0 = 2 p1 - p2 + L p1 (exact in exact arithmetic)
+    rng = np.random.default_rng(3)
+    nx, nz = 40, 36
+    vel = rng.uniform(10, 40, (nx, nz)).astype(np.float32)
+    p0 = np.zeros((nx, nz), np.float32)
+    p0[8:-8, 8:-8] = rng.standard_normal((nx - 16, nz - 16))
+    p1 = p0 * 0.9
+    u = np.zeros_like(p0)
+    p2 = p0.copy()
+    oracle.fd4t10s(p2, p1, vel, u)
+    back = p2.copy()
+    oracle.fd4t10s(back, p1, vel, u)
+    sl = (slice(6, -6), slice(6, -6))
+    assert np.max(np.abs(back[sl] - p0[sl])) < 1e-5 * max(1.0, np.abs(p0).max())
+
+
+# -- index maps, band, mute --------------------------------------------------
+
+def _geom(oracle, nx=30, nz=20, nb=7, nt=50, ns=3, fm=20.0):
+    sx = np.array([1, 14, 28], np.int32)[:ns]
+    return oracle.Geom(nx, nz, nb, 0.001, 10.0, fm, nt, sx, np.ones(ns, np.int32) * 2,
+                       np.arange(nx, dtype=np.int32), np.zeros(nx, np.int32))
+
+
+def test_injection_and_recording_index_maps(oracle):
+    g = _geom(oracle)
+    p = g.zeros()
+    g.add_sources(p, np.array([1.0, 2.0, 3.0], np.float32))
+    nzp = g.nzpad
+    for s, amp in zip([1, 14, 28], [1.0, 2.0, 3.0]):
+        assert p[s + g.bx0, 2 + g.bz0] == amp  # pos = sz + nz*sx, offsets bx0/bz0
+    assert np.count_nonzero(p) == 3
+    q = np.arange(p.size, dtype=np.float32).reshape(p.shape)
+    rec = g.record(q)
+    assert np.array_equal(rec, q[np.arange(30) + g.bx0, g.bz0])
+    assert g.shape == (30 + 2 * 7 + 12, 20 + 7 + 12) and nzp == 39
+
+
+def test_boundary_band_layout_brute_force(oracle):
+    g = _geom(oracle)
+    nxp, nzp, bx0, bz0 = g.nxpad, g.nzpad, g.bx0, g.bz0
+    q = np.arange(nxp * nzp, dtype=np.float32).reshape(nxp, nzp)
+    band = g.write_bndr(q)
+    w = 6
+    nxb = nxp - 2 * (bx0 - w)
+    nzb = nzp - bz0 - bx0
+    exp = []
+    exp_b = np.zeros(w * nxb, np.float32)
+    for ix in range(nxb):
+        for iz in range(w):
+            exp_b[iz + w * ix] = q[ix + bx0 - w, nzp - bx0 + iz]
+    exp_lr = np.zeros(2 * w * nzb, np.float32)
+    for iz in range(nzb):
+        for ix in range(w):
+            exp_lr[iz + nzb * ix] = q[bx0 - w + ix, bz0 + iz]
+            exp_lr[iz + nzb * (ix + w)] = q[nxp - bx0 + ix, bz0 + iz]
+    exp = np.concatenate([exp_b, exp_lr])
+    assert band.size == g.bndr_size == w * (nxb + 2 * nzb)
+    assert np.array_equal(band, exp)
+    r = np.zeros_like(q)
+    g.read_bndr(band, r)
+    mask = r != 0
+    assert np.array_equal(r[mask], q[mask]) and np.count_nonzero(mask) == band.size
+
+
+def test_mute_window_closed_form(oracle):
+    g = _geom(oracle, fm=20.0, nt=400)
+    rng = np.random.default_rng(1)
+    vel = rng.uniform(5, 40, g.shape).astype(np.float32)
+    st, en = g.mute_windows(vel, src=1)
+    sx, sz, gz = 14 + g.bx0, 2 + g.bz0, g.bz0
+    rows = vel[:, min(sz, gz):max(sz, gz) + 1].astype(np.float32)
+    acc = np.float32(0)
+    for v in rows.ravel():  # serial FP32 mean, x outer, z inner
+        acc = np.float32(acc + v)
+    avg = np.float32(acc / np.float32(g.nxpad * (max(sz, gz) - min(sz, gz) + 1)))
+    half = int(np.float32(1.5 / 20.0) / np.float32(0.001))
+    for r in range(30):
+        gx = r + g.bx0
+        dist = np.float32((gx - sx) ** 2 + (gz - sz) ** 2)
+        t = int(np.sqrt(np.float32(dist * avg), dtype=np.float32))
+        assert st[r] == t and en[r] == min(t + 2 * half, 400)
+
+
+# -- data-side pieces --------------------------------------------------------
+
+def test_trans_vsrc_exact_on_quadratics(oracle):
+    # the 4th-order 5-point 2nd difference is exact for polynomials of degree
+    # <= 5; for t^2 it is 2 everywhere in the interior
+    nt, ng = 50, 3
+    t = np.arange(nt, dtype=np.float64)
+    v = np.stack([t ** 2, 3 * t ** 2 + t, t ** 3 / 100.0], axis=1).astype(np.float32)
+    w = v.copy()
+    oracle.trans_vsrc(w)
+    assert np.all(w[:2] == 0) and np.all(w[-2:] == 0)
+    assert np.allclose(w[2:-2, 0], 2.0, rtol=0, atol=1e-3)
+    assert np.allclose(w[2:-2, 1], 6.0, rtol=0, atol=1e-3)
+    assert np.allclose(w[2:-2, 2], 6 * t[2:-2] / 100.0, rtol=1e-4, atol=1e-3)
+
+
+def test_scale_and_mask_gradient_brute_force(oracle):
+    g = _geom(oracle)
+    grad = np.ones(g.shape, np.float32)
+    g.scale_gradient(grad)
+    g.mask_gradient(grad)
+    bx0, bz0 = g.bx0, g.bz0
+    for ix in range(g.nxpad):
+        for iz in range(g.nzpad):
+            inside = bx0 <= ix < g.nxpad - bx0 and bz0 <= iz < g.nzpad - bx0
+            want = 0.0 if not inside else (1.0 if iz == bz0 else np.sqrt(np.float32(iz - bz0), dtype=np.float32))
+            assert grad[ix, iz] == np.float32(want)
+
+
+def test_refill_boundary_is_edge_replication(oracle):
+    g = _geom(oracle)
+    rng = np.random.default_rng(2)
+    a = rng.standard_normal(g.shape).astype(np.float32)
+    b = a.copy()
+    g.refill_boundary(b)
+    ix = np.clip(np.arange(g.nxpad), g.bx0, g.nxpad - g.bx0 - 1)
+    iz = np.clip(np.arange(g.nzpad), g.bz0, g.nzpad - g.bx0 - 1)
+    assert np.array_equal(b, a[np.ix_(ix, iz)])
+
+
+def test_expand_domain_matches_transformed_replication(oracle):
+    rng = np.random.default_rng(4)
+    v = rng.uniform(1500, 4000, (12, 9)).astype(np.float32)
+    nb, dx, dt = 5, 10.0, 0.001
+    e = oracle.expand_domain(v, nb, dx, dt)
+    assert e.shape == (12 + 2 * nb + 12, 9 + nb + 12)
+    tr = (np.float32(dx) * np.float32(dx)) / (v * v * np.float32(dt) * np.float32(dt))
+    ix = np.clip(np.arange(e.shape[0]) - 6 - nb, 0, 11)
+    iz = np.clip(np.arange(e.shape[1]) - 6, 0, 8)
+    assert np.array_equal(e, tr[np.ix_(ix, iz)])
+
+
+def test_ricker_closed_form(oracle):
+    nt, fm, dt, amp = 300, 15.0, 0.001, 1.0
+    w = oracle.ricker(nt, fm, dt, amp)
+    t = np.arange(nt) * dt - 1.0 / fm
+    x = (np.pi * fm * t) ** 2
+    ref = amp * (1 - 2 * x) * np.exp(-x)
+    assert np.max(np.abs(w - ref)) < 2e-6
+    assert np.argmax(w) == int(round(1 / fm / dt))
+
+
+def test_minstd_known_answer(oracle):
+    # [rand.predef]: the 10000th output of minstd_rand seeded with 1 is 399268537
+    x = 1
+    for _ in range(10000):
+        x = (x * 48271) % 2147483647
+    assert x == 399268537
+    c = oracle.gen_codes(1, 10000)
+    assert c[-1] == (2 * ((399268537 - 1) // 1073741823) - 1)
+    assert set(np.unique(c)) == {-1, 1} and abs(c.mean()) < 0.05
+
+
+def test_encoder_linearity(oracle):
+    rng = np.random.default_rng(9)
+    dobs = rng.standard_normal((5, 7, 4)).astype(np.float32)
+    code = np.array([1, -1, 1, 1, -1], np.int32)
+    enc = oracle.encode_obs(code, dobs)
+    assert np.allclose(enc, (dobs * code[:, None, None]).sum(0), atol=1e-6)
+    same = oracle.encode_obs(np.array([1, -1, 1, -1], np.int32), dobs[:1].repeat(4, 0))
+    assert np.count_nonzero(same) == 0  # opposite codes cancel exactly
+    w = rng.standard_normal(11).astype(np.float32)
+    es = oracle.encode_source(code, w)
+    assert np.array_equal(es, w[:, None] * code[None, :].astype(np.float32))
+
+
+def test_parabola_vertex(oracle):
+    # exact parabola y = 3 (x - 2)^2 + 1 sampled at 0, 1, 4
+    xv, yv = oracle.parabola_vertex(0.0, 13.0, 1.0, 4.0, 4.0, 13.0)
+    assert abs(xv - 2.0) < 1e-6 and abs(yv - 1.0) < 1e-5
+    # collinear points: degenerate fallback returns (x3, y3)
+    assert oracle.parabola_vertex(0.0, 1.0, 1.0, 2.0, 2.0, 3.0) == (2.0, 3.0)
+    # main-test.cpp:37-60 input (the reference's recorded line-search state)
+    xv, yv = oracle.parabola_vertex(0.0, 6.84734e6, 6.17151e-05, 5.49986e6, 1.2343e-04, 5.26678e6)
+    x = np.array([0.0, 6.17151e-05, 1.2343e-04], np.float32).astype(np.float64)
+    y = np.array([6.84734e6, 5.49986e6, 5.26678e6], np.float32).astype(np.float64)
+    c = np.polyfit(x, y, 2)
+    assert abs(xv - (-c[1] / (2 * c[0]))) / abs(xv) < 1e-4
+
+
+def test_update_vel_clamp_and_bounds(oracle):
+    tmin, tmax = oracle.vel_bounds(1500.0, 4500.0, 10.0, 0.001)
+    assert np.isclose(tmin, (10 / 0.001 / 4500) ** 2, rtol=1e-6) and np.isclose(tmax, (10 / 0.001 / 1500) ** 2,
+                                                                                   rtol=1e-6)
+    v = np.array([10.0, 20.0, 40.0], np.float32)
+    d = np.array([1.0, -100.0, 100.0], np.float32)
+    nv = oracle.update_vel(v, d, 0.5, tmin, tmax)
+    assert nv[0] == np.float32(10.5) and nv[1] == np.float32(tmin) and nv[2] == np.float32(tmax)
+
+
+def test_update_grad_pr_plus(oracle):
+    rng = np.random.default_rng(5)
+    n = 1000
+    g0 = rng.standard_normal(n).astype(np.float32)
+    g1 = rng.standard_normal(n).astype(np.float32)
+    d = rng.standard_normal(n).astype(np.float32)
+    for mode in (0, 1):
+        oracle.set_sum_mode(mode)
+        pre, dd = g0.copy(), d.copy()
+        oracle.update_grad(pre, g1, dd, 1)
+        beta = max(0.0, (g1 @ g1 - g1 @ g0) / (g0 @ g0))
+        assert np.allclose(dd, g1 + np.float32(beta) * d, rtol=1e-4, atol=1e-5)
+        assert np.array_equal(pre, g1)
+    oracle.set_sum_mode(0)
+
+
+def test_max_alpha_brute_force(oracle):
+    rng = np.random.default_rng(6)
+    vel = rng.uniform(5, 40, 500).astype(np.float32)
+    grad = rng.standard_normal(500).astype(np.float32)
+    grad[::7] = 0
+    a2, a3 = oracle.cal_max_alpha23(vel, grad, 0.001, 10.0, 200.0)
+    v = 10.0 / (0.001 * np.sqrt(vel.astype(np.float64))) - 200.0
+    tv = (10.0 / (0.001 * v)) ** 2
+    ok = np.abs(grad) >= 1e-10
+    ref = np.min((tv[ok] - vel[ok]) / np.abs(grad[ok]))
+    assert abs(a2 - ref) / abs(ref) < 1e-4 and a3 == 2 * a2
+
+
+def test_clip_position(oracle):
+    s = np.array([10.0, 5.0, 1.0, 1e-5, 1e-7])
+    # brute force restatement of the stopping rule
+    assert oracle.clip_position(s) == 3
+    assert oracle.clip_position(np.ones(6)) == 6
+
+
+# -- sequences ---------------------------------------------------------------
+
+def test_gradient_is_linear_in_adjoint_source(oracle):
+    g = _geom(oracle, nx=24, nz=18, nb=7, nt=330)
+    rng = np.random.default_rng(8)
+    vel = rng.uniform(8, 30, g.shape).astype(np.float32)
+    wlt = oracle.ricker(g.nt, 20.0, 0.001, 1.0)
+    v1 = rng.standard_normal((g.nt, g.ng)).astype(np.float32)
+    z = np.zeros_like(v1)
+    g0 = g.calgradient(vel, wlt, z, shot=1)
+    assert np.count_nonzero(g0) == 0
+    ga = g.calgradient(vel, wlt, v1, shot=1)
+    gb = g.calgradient(vel, wlt, 2 * v1, shot=1)
+    assert np.allclose(gb, 2 * ga, rtol=1e-5, atol=1e-6 * np.abs(ga).max())
+
+
+def test_gradient_is_a_descent_direction(oracle):
+    # Sign conventions (residual obs-cal, subtracted imaging condition,
+    # update +alpha*d): a small step along the masked gradient lowers the misfit.
+    nx, nz, nb, nt = 40, 30, 8, 600
+    sx = np.array([5, 20, 34], np.int32)
+    g = oracle.Geom(nx, nz, nb, 0.001, 10.0, 25.0, nt, sx, np.ones(3, np.int32),
+                    np.arange(nx, dtype=np.int32), np.zeros(nx, np.int32))
+    vtrue = np.full((nx, nz), 2000.0, np.float32)
+    vtrue[:, 15:] = 2600.0
+    v0 = np.full((nx, nz), 2000.0, np.float32)
+    vt = oracle.expand_domain(vtrue, nb, 10.0, 0.001)
+    vi = oracle.expand_domain(v0, nb, 10.0, 0.001)
+    wlt = oracle.ricker(nt, 25.0, 0.001, 1.0)
+    dobs = np.stack([g.forward_modeling(vt, wlt, shot=s) for s in range(3)])
+    tmin, tmax = oracle.vel_bounds(1500.0, 4000.0, 10.0, 0.001)
+    e = oracle.Ess(g, vi.copy(), wlt, dobs, 1500.0, 4000.0, 2, 200.0)
+    e.fwi_epoch(0)
+    s = e.scalars()
+    assert s["updateobj"] < s["obj1"] or s["obj2"] < s["obj1"]
