@@ -1,0 +1,179 @@
+"""Benchmark: encoded-source (ESS) FWI on the 301x921 grid (nb=30, nt=4000,
+60 Ricker 10 Hz shots, 921 receivers), one GPU.
+
+A step is one EssFwiFramework epoch on the GPU: new +-1 codes, encoded
+forward modelling + boundary save (one fused pass), mute, residual, adjoint
+gradient, PR+ direction, bracketing/parabolic line search (batched trial
+forward runs), model update.  value = stencil grid-point updates per second
+over the timed epochs.
+
+    python bench.py [--steps K] [--warmup W] [--precision exact|fp32]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FWD_BYTES_PER_PT = 16.0   # read cur, prev, model; write next
+BWD_BYTES_PER_PT = 40.0   # two steps (source + receiver field) + gradient read-modify-write
+
+
+def cpu_baseline(case, seconds_cap=30.0):
+    """The oracle (CPU restatement of the reference path, bit-identical to it)
+    timed on this host on a bounded sample of the same workload: the ESS
+    gradient of the 301x921 grid truncated to nt=500 (500 forward steps, 200
+    backward iterations of two steps), reference threading (serial u2 pass of
+    the damped kernel) and all-parallel."""
+    from oracle import pyoracle as po
+    from swfwi_amd import workloads
+
+    nt = 500
+    vtrue, v0 = workloads.models(case)
+    out = {}
+    for label, par in (("reference_threading", 0), ("all_parallel", 1)):
+        g = po.Geom(case.nx, case.nz, case.nb, case.dt, case.dx, case.fm, nt, case.sx, case.sz, case.gx, case.gz,
+                    parallel_u2=par)
+        vi = po.expand_domain(v0, case.nb, case.dx, case.dt)
+        wlt = po.ricker(nt, case.fm, case.dt, case.amp)
+        enc = po.encode_source(po.gen_codes(1, case.ns), wlt)
+        rng = np.random.default_rng(0)
+        vsrc = (rng.standard_normal((nt, case.ng)) * 1e-3).astype(np.float32)
+        t0 = time.perf_counter()
+        g.calgradient(vi, enc, vsrc, shot=-1)
+        dt = time.perf_counter() - t0
+        iters = sum(1 for it in range(nt - 1, -1, -1) if np.float32(case.dt) * np.float32(it) > 0.3) + 1
+        pts = case.update_points * (nt + 2 * iters)
+        out[label] = (pts / dt / 1e9, dt, nt, iters)
+        if dt > seconds_cap:
+            break
+    return out, po.lib().orc_max_threads()
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precision", choices=["exact", "fp32"], default="exact")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--nt", type=int, default=None, help="override nt (testing only)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (shots/members shard across GPUs "
+              "with one gradient all-reduce per iteration; that path is not part of this build)", file=sys.stderr)
+        sys.exit(2)
+
+    from swfwi_amd import capi, workloads
+
+    if capi.device_count() < 1:
+        print("no HIP device", file=sys.stderr)
+        sys.exit(1)
+    case = workloads.config1(nt=args.nt or 4000)
+    prec = capi.PREC_EXACT if args.precision == "exact" else capi.PREC_FP32
+    vtrue, v0 = workloads.models(case)
+    vt = capi.expand_domain(vtrue, case.nb, case.dx, case.dt)
+    vi = capi.expand_domain(v0, case.nb, case.dx, case.dt)
+    wlt = capi.ricker(case.nt, case.fm, case.dt, case.amp)
+    s = capi.Solver(case.nx, case.nz, case.nb, case.dt, case.dx, case.fm, case.nt, case.sx, case.sz, case.gx,
+                    case.gz, prec)
+    # observed data: every shot on the true model, batched (fm-damp)
+    s.bind_velocity(vt)
+    t0 = time.perf_counter()
+    dobs = s.forward_shots(wlt)
+    t_obs = time.perf_counter() - t0
+    print(f"[bench] observed data {case.ns} shots in {t_obs:.2f} s", file=sys.stderr, flush=True)
+    s.bind_velocity(vi)
+    inv = capi.Inversion(s, wlt, dobs, case.vmin, case.vmax, 5, 200.0)
+    objs = []
+    for it in range(args.warmup):
+        inv.ess_epoch(it)  # also captures the time loops into HIP graphs
+        objs.append(inv.stats()["obj1"])
+        print(f"[bench] warmup epoch {it} obj {objs[-1]:.4e}", file=sys.stderr, flush=True)
+    pu0 = s.counters()[0]
+    stats = []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        inv.ess_epoch(args.warmup + k)
+        stats.append(inv.stats())
+    t1 = time.perf_counter()
+    pu1 = s.counters()[0]
+    for st in stats:
+        objs.append(st["obj1"])
+    wall = t1 - t0
+    gpts = (sum(pu1) - sum(pu0)) / wall / 1e9
+    # kernel durations: the same epochs continued with a HIP event pair around
+    # every stencil launch on the library's stream (HIP cannot time events
+    # inside a replayed graph, so these loops launch directly)
+    s.set_timing(True)
+    pu0 = s.counters()[0]
+    for k in range(args.steps):
+        inv.ess_epoch(args.warmup + args.steps + k)
+        objs.append(inv.stats()["obj1"])
+    pu1 = s.counters()[0]
+    kt = s.kernel_times()
+    s.set_timing(False)
+    # dominant kernel: the one with the larger total time in the timed region
+    fwd_ms, fwd_n = kt["fwd"]
+    bwd_ms, bwd_n = kt["bwd"]
+    grad_ms = float(np.mean([st["grad_ms"] for st in stats]))
+    ls_ms = float(np.mean([st["linesearch_ms"] for st in stats]))
+    ntrial = float(np.mean([st["ntrial"] for st in stats]))
+    # algorithmic bytes from the E-weighted point updates of each kernel type
+    # (the backward kernel advances two fields per point: 40 B per point and
+    # backward iteration)
+    fwd_pu = pu1[0] - pu0[0]
+    bwd_pu = (pu1[1] - pu0[1]) / 2.0
+    if bwd_ms >= fwd_ms:
+        name, ms, n, byts = "k_bwd_step", bwd_ms, bwd_n, bwd_pu * BWD_BYTES_PER_PT
+    else:
+        name, ms, n, byts = "k_fwd_step", fwd_ms, fwd_n, fwd_pu * FWD_BYTES_PER_PT
+    achieved = byts / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": name,
+            "avg_launch_us": round(1e3 * ms / max(n, 1), 3), "launches": n}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        res, threads = cpu_baseline(case)
+        ref = res.get("reference_threading")
+        par = res.get("all_parallel")
+        cpu = {"value": round(ref[0], 4), "unit": "Gpts/s", "cores": threads, "kind": "port",
+               "sample": f"oracle ESS calgradient on the 301x921 grid truncated to nt={ref[2]} "
+                         f"({ref[2]} forward steps + {ref[3]} backward iterations), {ref[1]:.1f} s, "
+                         "serial u2 pass as the reference build",
+               "all_parallel_value": round(par[0], 4) if par else None}
+
+    line = {
+        "metric": "FWI gradient time/shot & Gpts/s, learned vs plain FP32 at equal data misfit",
+        "value": round(gpts, 3),
+        "unit": "Gpts/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * wall / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded layered model; observed data from the same propagator on the true model)",
+        "config": {"workload": "ESS-FWI 301x921 dx=12.5 nb=30 nt=4000, 60 shots x 921 receivers, one supershot "
+                               "per epoch (configs[1])", "precision": args.precision, "parallelism": "single GPU"},
+        "gradient_ms_per_supershot": round(grad_ms, 3),
+        "linesearch_ms": round(ls_ms, 3),
+        "trial_forward_runs_per_epoch": ntrial,
+        "misfit": [float(f"{o:.6e}") for o in objs],
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

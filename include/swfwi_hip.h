@@ -39,6 +39,10 @@ const char *swfwi_last_error(void);
 const char *swfwi_version(void);
 /* Number of visible HIP devices (0 when none); never initialises a context. */
 int swfwi_device_count(int *count);
+/* Practical bandwidth ceiling: a plain float4 device copy of n floats,
+ * `iters` launches; *gbs = (read + write bytes) / time.  No counterpart in
+ * the reference: it calibrates the roofline and the HBM byte counters. */
+int swfwi_copy_bandwidth(long long n, int iters, double *gbs);
 
 /* ---- drop-in per-step kernels (host pointers, in place prev <- next) ----- */
 /* Replaces fd4t10s_damp_zjh_2d_vtrans (fd4t10s-damp-zjh.h:11,
@@ -90,8 +94,24 @@ int swfwi_calgradient(swfwi_solver h, const float *encsrc, int shot, const float
 /* Observed data of every shot, one batched launch per time step (fm-damp's
  * shot loop, main-fm-damp.cpp:241-256): wlt [nt] -> dobs [ns][nt][ng]. */
 int swfwi_forward_shots(swfwi_solver h, const float *wlt, float *dobs);
-/* Grid-point updates and stencil launches issued by this handle so far. */
+/* Grid-point updates (entry-weighted; the backward kernel counts its two
+ * fields) and stencil launches issued by this handle so far, [0] forward
+ * step kernel, [1] backward step kernel. */
 int swfwi_solver_counters(swfwi_solver h, double *point_updates, long long *launches);
+/* Capture time loops into HIP graphs and replay them when the same buffers
+ * recur (on by default inside the inversion drivers' epochs). */
+int swfwi_solver_set_graphs(swfwi_solver h, int on);
+/* Stencil tile width along x: 8, 16 or 32, or 0 to let the tile tuner time
+ * all three on the first run of each batch size (the default). */
+int swfwi_solver_set_tile_x(swfwi_solver h, int tile_x);
+/* Tile width in use for a kernel kind (0 forward, 1 backward) and batch
+ * size, 0 when that batch size has not run yet. */
+int swfwi_solver_get_tile_x(swfwi_solver h, int kind, int entries, int *tile_x);
+/* Record a HIP event pair around every stencil launch on the handle's stream;
+ * swfwi_solver_kernel_times returns the accumulated milliseconds and launch
+ * counts of the forward [0] and backward [1] step kernels. */
+int swfwi_solver_set_timing(swfwi_solver h, int on);
+int swfwi_solver_kernel_times(swfwi_solver h, double *ms, long long *launches);
 
 /* ---- inversion drivers (FwiFramework / EssFwiFramework) ------------------ */
 /* wlt [nt], dobs [ns][nt][ng]; UpdateVelOp(vmin, vmax, dx, dt) and

@@ -24,6 +24,7 @@ SIGNATURES = {
     "swfwi_last_error": (C.c_char_p, []),
     "swfwi_version": (C.c_char_p, []),
     "swfwi_device_count": (_I, [C.POINTER(_I)]),
+    "swfwi_copy_bandwidth": (_I, [C.c_longlong, _I, C.POINTER(C.c_double)]),
     "fd4t10s_damp_zjh_2d_vtrans": (None, [_f32, _f32, _f32, _f32, _I, _I, _I]),
     "fd4t10s_zjh_2d_vtrans": (None, [_f32, _f32, _f32, _f32, _I, _I]),
     "swfwi_step_damped": (_I, [_f32, _f32, _f32, _f32, _I, _I, _I, _I]),
@@ -40,7 +41,12 @@ SIGNATURES = {
     "swfwi_forward_modeling": (_I, [_VP, _f32, _I, _f32]),
     "swfwi_calgradient": (_I, [_VP, _f32, _I, _f32, _f32]),
     "swfwi_forward_shots": (_I, [_VP, _f32, _f32]),
-    "swfwi_solver_counters": (_I, [_VP, C.POINTER(C.c_double), C.POINTER(C.c_longlong)]),
+    "swfwi_solver_counters": (_I, [_VP, _f64, np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")]),
+    "swfwi_solver_set_graphs": (_I, [_VP, _I]),
+    "swfwi_solver_set_tile_x": (_I, [_VP, _I]),
+    "swfwi_solver_get_tile_x": (_I, [_VP, _I, _I, C.POINTER(_I)]),
+    "swfwi_solver_set_timing": (_I, [_VP, _I]),
+    "swfwi_solver_kernel_times": (_I, [_VP, _f64, np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")]),
     "swfwi_inversion_create": (_I, [C.POINTER(_VP), _VP, _f32, _f32, _F, _F, _I, _F]),
     "swfwi_inversion_destroy": (_I, [_VP]),
     "swfwi_ess_epoch": (_I, [_VP, _I]),
@@ -84,6 +90,13 @@ def device_count():
     n = _I(0)
     check(load().swfwi_device_count(C.byref(n)))
     return n.value
+
+
+def copy_bandwidth(n=1 << 28, iters=20):
+    """GB/s (read + write) of a plain device copy of n floats."""
+    g = C.c_double(0)
+    check(load().swfwi_copy_bandwidth(n, iters, C.byref(g)))
+    return g.value
 
 
 def _f(a):
@@ -179,9 +192,32 @@ class Solver:
         return dobs
 
     def counters(self):
-        pu, la = C.c_double(0), C.c_longlong(0)
-        check(load().swfwi_solver_counters(self.h, C.byref(pu), C.byref(la)))
-        return pu.value, la.value
+        """([forward, backward] point updates, [forward, backward] launches)."""
+        pu = np.zeros(2, np.float64)
+        la = np.zeros(2, np.int64)
+        check(load().swfwi_solver_counters(self.h, pu, la))
+        return pu.tolist(), la.tolist()
+
+    def set_graphs(self, on):
+        check(load().swfwi_solver_set_graphs(self.h, int(bool(on))))
+
+    def set_tile_x(self, tx):
+        check(load().swfwi_solver_set_tile_x(self.h, int(tx)))
+
+    def tile_x(self, kind, entries):
+        t = _I(0)
+        check(load().swfwi_solver_get_tile_x(self.h, kind, entries, C.byref(t)))
+        return t.value
+
+    def set_timing(self, on):
+        check(load().swfwi_solver_set_timing(self.h, int(bool(on))))
+
+    def kernel_times(self):
+        """{'fwd': (ms_total, launches), 'bwd': (...)} of the timed stencil launches."""
+        ms = np.zeros(2, np.float64)
+        n = np.zeros(2, np.int64)
+        check(load().swfwi_solver_kernel_times(self.h, ms, n))
+        return {"fwd": (float(ms[0]), int(n[0])), "bwd": (float(ms[1]), int(n[1]))}
 
 
 class Inversion:

@@ -92,7 +92,8 @@ int run_step(float* prev, const float* cur, const float* vel, float* u2, int nx,
     g.nz = nz;
     g.ld = nz;
     g.ntz = (nz - 2 * kHalo + kTZ - 1) / kTZ;
-    g.ntx = (nx - 2 * kHalo + kTX - 1) / kTX;
+    g.ntx = tile_count_x(nx, 16);
+    g.tile_x = 16;
     g.nb = nb;
     if (damped) {
       g.delta_x = c.damp.dx.get();
@@ -134,6 +135,15 @@ int swfwi_device_count(int* count) {
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
     *count = n;
+  });
+}
+
+int swfwi_copy_bandwidth(long long n, int iters, double* gbs) {
+  return guarded([&] {
+    NEED(gbs);
+    if (n < 1024 || iters < 1) fail(kErrArg, "copy probe needs n >= 1024 and iters >= 1");
+    Stream st;
+    *gbs = op_copy_bandwidth(n, iters, st.get());
   });
 }
 
@@ -288,8 +298,56 @@ int swfwi_forward_shots(swfwi_solver h, const float* wlt, float* dobs) {
 int swfwi_solver_counters(swfwi_solver h, double* point_updates, long long* launches) {
   return guarded([&] {
     NEED(h);
-    if (point_updates) *point_updates = h->s->point_updates();
-    if (launches) *launches = h->s->launches();
+    Propagator& p = h->s->prop();
+    if (point_updates) {
+      point_updates[0] = p.fwd_point_updates;
+      point_updates[1] = p.bwd_point_updates;
+    }
+    if (launches) {
+      launches[0] = p.fwd_launches;
+      launches[1] = p.bwd_launches;
+    }
+  });
+}
+
+int swfwi_solver_set_graphs(swfwi_solver h, int on) {
+  return guarded([&] {
+    NEED(h);
+    h->s->prop().set_graphs(on != 0);
+  });
+}
+
+int swfwi_solver_set_tile_x(swfwi_solver h, int tile_x) {
+  return guarded([&] {
+    NEED(h);
+    h->s->prop().set_tile_x(tile_x);
+  });
+}
+
+int swfwi_solver_get_tile_x(swfwi_solver h, int kind, int entries, int* tile_x) {
+  return guarded([&] {
+    NEED(h);
+    NEED(tile_x);
+    *tile_x = h->s->prop().tile_choice_cached(kind, entries);
+  });
+}
+
+int swfwi_solver_set_timing(swfwi_solver h, int on) {
+  return guarded([&] {
+    NEED(h);
+    h->s->prop().set_timing(on != 0);
+  });
+}
+
+int swfwi_solver_kernel_times(swfwi_solver h, double* ms, long long* launches) {
+  return guarded([&] {
+    NEED(h);
+    Propagator& p = h->s->prop();
+    p.collect_timing();
+    for (int k = 0; k < 2; k++) {
+      if (ms) ms[k] = p.kernel_ms[k];
+      if (launches) launches[k] = p.kernel_n[k];
+    }
   });
 }
 

@@ -35,6 +35,7 @@ struct PointList {
 struct Grid {
   int nx = 0, nz = 0, ld = 0;     // padded dims and leading dimension
   int ntz = 0, ntx = 0;           // tiles along z and x
+  int tile_x = 16;                // tile width along x: 8, 16 or 32
   int nb = 0;                     // damping width handed to the damped kernel (bx0)
   int bx0 = 0, bxn = 0, bz0 = 0, bzn = 0;
   const float* delta_x = nullptr; // damping delta per ix (x sponge), 0 inside
@@ -99,6 +100,10 @@ struct BwdStepArgs {
 };
 
 // ---- launchers (all asynchronous on `stream`) -------------------------------
+// Tile widths the step kernels are instantiated for, and the tile count along
+// x of a padded grid for one of them.
+constexpr int kTileWidths[3] = {8, 16, 32};
+int tile_count_x(int nx, int tile_x);
 void launch_fwd_step(const FwdStepArgs& a, int nentries, bool damped, int precision, hipStream_t stream);
 void launch_bwd_step(const BwdStepArgs& a, int nentries, int precision, hipStream_t stream);
 

@@ -67,6 +67,10 @@ void op_refill(float* a, long long stride, int E, const GridGeom& g, hipStream_t
 // rv = 1/vel for E models.
 void op_reciprocal(const float* vel, float* rv, long long n, hipStream_t s);
 
+// Device-to-device copy bandwidth (GB/s, read + write bytes) of a plain
+// float4 grid-stride copy of n floats, `iters` back-to-back launches.
+double op_copy_bandwidth(long long n, int iters, hipStream_t s);
+
 // Add `src` (+e*stride) into dst (+e*stride) elementwise (FP32).
 void op_axpy1(const float* src, float* dst, long long n, hipStream_t s);
 

@@ -282,6 +282,11 @@ __global__ void k_reciprocal2(const float* vel, float* rv, long long n) {
     rv[i] = 1.0f / vel[i];
 }
 
+__global__ void k_fill_any(float* p, long long n) {
+  for (long long i = blockIdx.x * (long long)kB + threadIdx.x; i < n; i += (long long)gridDim.x * kB)
+    p[i] = (float)(i & 1023);
+}
+
 __global__ void k_axpy1(const float* src, float* dst, long long n) {
   for (long long i = blockIdx.x * (long long)kB + threadIdx.x; i < n; i += (long long)gridDim.x * kB)
     dst[i] = dst[i] + src[i];
@@ -413,6 +418,33 @@ void op_reciprocal(const float* vel, float* rv, long long n, hipStream_t s) {
 }
 void op_axpy1(const float* src, float* dst, long long n, hipStream_t s) {
   hipLaunchKernelGGL(k_axpy1, dim3(nblocks(n)), dim3(kB), 0, s, src, dst, n);
+}
+
+namespace {
+__global__ void k_copy4(const float4* __restrict__ src, float4* __restrict__ dst, long long n4) {
+  for (long long i = blockIdx.x * (long long)kB + threadIdx.x; i < n4; i += (long long)gridDim.x * kB) dst[i] = src[i];
+}
+}  // namespace
+
+double op_copy_bandwidth(long long n, int iters, hipStream_t s) {
+  const long long n4 = n / 4;
+  DBuf<float> a((size_t)n4 * 4), b((size_t)n4 * 4);
+  hipLaunchKernelGGL(k_fill_any, dim3(1024), dim3(kB), 0, s, a.get(), n4 * 4);
+  hipEvent_t e0, e1;
+  SWFWI_HIP(hipEventCreate(&e0));
+  SWFWI_HIP(hipEventCreate(&e1));
+  const int grid = 256 * 8 * 4;
+  hipLaunchKernelGGL(k_copy4, dim3(grid), dim3(kB), 0, s, (const float4*)a.get(), (float4*)b.get(), n4);
+  SWFWI_HIP(hipEventRecord(e0, s));
+  for (int i = 0; i < iters; i++)
+    hipLaunchKernelGGL(k_copy4, dim3(grid), dim3(kB), 0, s, (const float4*)a.get(), (float4*)b.get(), n4);
+  SWFWI_HIP(hipEventRecord(e1, s));
+  SWFWI_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  SWFWI_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return 2.0 * 16.0 * (double)n4 * iters / (ms * 1e-3) / 1e9;
 }
 
 void op_gram_f64(const double* A, const double* B, int N, long long m, double* C, hipStream_t s) {

@@ -27,7 +27,7 @@ void Geometry::derive() {
   const int nzb = nzpad - bz0 - bzn;
   bsize = w * (nxb + 2 * nzb);
   ntz = (nzpad - 2 * kHalo + kTZ - 1) / kTZ;
-  ntx = (nxpad - 2 * kHalo + kTX - 1) / kTX;
+  ntx = tile_count_x(nxpad, 16);
 }
 
 void build_damping(int nx, int nz, int nb, DampTables& t, hipStream_t s) {
@@ -51,13 +51,14 @@ void build_damping(int nx, int nz, int nb, DampTables& t, hipStream_t s) {
   t.zdamp = nz - nb;
 }
 
-void build_points(const std::vector<long long>& idx, int ld, int ntz, int ntx, DevPoints& out, hipStream_t s) {
+void build_points(const std::vector<long long>& idx, int ld, int ntz, int ntx, int tile_x, DevPoints& out,
+                  hipStream_t s) {
   const int ntiles = ntz * ntx;
   const int n = (int)idx.size();
   std::vector<int> tile(n);
   for (int p = 0; p < n; p++) {
     const int ix = (int)(idx[p] / ld), iz = (int)(idx[p] % ld);
-    const int tz = (iz - kHalo) / kTZ, tx = (ix - kHalo) / kTX;
+    const int tz = (iz - kHalo) / kTZ, tx = (ix - kHalo) / tile_x;
     if (iz < kHalo || ix < kHalo || tz >= ntz || tx >= ntx) fail(kErrArg, "point outside the stencil update region");
     tile[p] = tx * ntz + tz;
   }
@@ -103,8 +104,11 @@ Propagator::Propagator(const Geometry& g, int precision) : g_(g), prec_(precisio
     gzp[i] = g_.gz[i] + g_.bz0;
     gi[i] = (long long)gxp[i] * g_.ld + gzp[i];
   }
-  build_points(si, g_.ld, g_.ntz, g_.ntx, src_, s);
-  build_points(gi, g_.ld, g_.ntz, g_.ntx, rec_, s);
+  for (int w = 0; w < 3; w++) {
+    const int ntx = tile_count_x(g_.nxpad, kTileWidths[w]);
+    build_points(si, g_.ld, g_.ntz, ntx, kTileWidths[w], src_[w], s);
+    build_points(gi, g_.ld, g_.ntz, ntx, kTileWidths[w], rec_[w], s);
+  }
   sxp_.upload(sxp, s);
   szp_.upload(szp, s);
   gxp_.upload(gxp, s);
@@ -112,13 +116,14 @@ Propagator::Propagator(const Geometry& g, int precision) : g_(g), prec_(precisio
   stream_.sync();
 }
 
-Grid Propagator::grid() const {
+Grid Propagator::grid(int w) const {
   Grid gr;
   gr.nx = g_.nxpad;
   gr.nz = g_.nzpad;
   gr.ld = g_.ld;
   gr.ntz = g_.ntz;
-  gr.ntx = g_.ntx;
+  gr.tile_x = kTileWidths[w];
+  gr.ntx = tile_count_x(g_.nxpad, gr.tile_x);
   gr.nb = g_.bx0;
   gr.bx0 = g_.bx0;
   gr.bxn = g_.bxn;
@@ -128,6 +133,99 @@ Grid Propagator::grid() const {
   gr.delta_z = damp_.dz.get();
   gr.zdamp = damp_.zdamp;
   return gr;
+}
+
+void Propagator::set_tile_x(int tx) {
+  if (tx != 0 && tx != 8 && tx != 16 && tx != 32) fail(kErrArg, "tile width must be 0 (tuned), 8, 16 or 32");
+  tile_override_ = tx;
+}
+
+int Propagator::tile_choice_cached(int kind, int E) const {
+  if (tile_override_) return tile_override_;
+  const long long key = ((long long)kind << 32) + E;
+  for (const auto& kv : tile_cache_)
+    if (kv.first == key) return kTileWidths[kv.second];
+  return 0;
+}
+
+int Propagator::tile_choice(int kind, int E, const float* rv, long long rvstride, int rvdiv, Fields& w,
+                            bool may_time) {
+  if (tile_override_) {
+    for (int i = 0; i < 3; i++)
+      if (kTileWidths[i] == tile_override_) return i;
+  }
+  const long long key = ((long long)kind << 32) + E;
+  for (const auto& kv : tile_cache_)
+    if (kv.first == key) return kv.second;
+  int best = 1;
+  if (!tune_ || !may_time) {
+    // heuristic: enough blocks to fill 256 CUs with the widest tile that does
+    const long long tiles16 = (long long)g_.ntz * tile_count_x(g_.nxpad, 16) * E;
+    best = tiles16 >= 4 * 256 ? 2 : (tiles16 < 256 ? 0 : 1);
+  } else {
+    // timed trial on scratch wavefields (the caller's workspace is free at
+    // this point: forward() tunes before it zeroes and runs)
+    hipStream_t s = stream();
+    const long long np = g_.npts();
+    w.ensure(E, np, true);
+    launch_fill(w.a.get(), (long long)E * np, 0.f, s);
+    launch_fill(w.b.get(), (long long)E * np, 0.f, s);
+    launch_fill(w.c.get(), (long long)E * np, 0.f, s);
+    launch_fill(w.d.get(), (long long)E * np, 0.f, s);
+    hipEvent_t e0, e1;
+    SWFWI_HIP(hipEventCreate(&e0));
+    SWFWI_HIP(hipEventCreate(&e1));
+    // three interleaved rounds (the first warms clocks and caches), the best
+    // of the last two per width
+    float best_of[3] = {0, 0, 0};
+    const int reps = 6;
+    for (int round = 0; round < 3; round++) {
+      for (int i = 0; i < 3; i++) {
+        float ms = 0;
+        SWFWI_HIP(hipEventRecord(e0, s));
+        for (int r = 0; r < reps; r++) {
+          if (kind == 0) {
+            FwdStepArgs a;
+            a.g = grid(i);
+            a.fstride = np;
+            a.cur = (r & 1) ? w.a.get() : w.b.get();
+            a.prev = (r & 1) ? w.b.get() : w.a.get();
+            a.rv = rv;
+            a.rvstride = rvstride;
+            a.rvdiv = rvdiv;
+            launch_fwd_step(a, E, true, prec_, s);
+          } else {
+            BwdStepArgs a;
+            a.g = grid(i);
+            a.fstride = np;
+            a.s_cur = (r & 1) ? w.a.get() : w.b.get();
+            a.s_prev = (r & 1) ? w.b.get() : w.a.get();
+            a.g_cur = (r & 1) ? w.c.get() : w.d.get();
+            a.g_prev = (r & 1) ? w.d.get() : w.c.get();
+            a.rv = rv;
+            a.rvstride = rvstride;
+            a.rvdiv = rvdiv;
+            a.grad = w.c.get();  // scratch only: the timing run's values are discarded
+            a.gstride = np;
+            a.do_xcorr = 0;
+            launch_bwd_step(a, E, prec_, s);
+          }
+        }
+        SWFWI_HIP(hipEventRecord(e1, s));
+        SWFWI_HIP(hipEventSynchronize(e1));
+        SWFWI_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (round == 1 || (round == 2 && ms < best_of[i])) best_of[i] = ms;
+      }
+    }
+    best = 0;
+    for (int i = 1; i < 3; i++)
+      if (best_of[i] < best_of[best]) best = i;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    SWFWI_HIP(hipGetLastError());
+  }
+  tile_cache_.push_back({key, best});
+  return best;
 }
 
 int Propagator::xcorr_range(float dt, int nt, std::vector<float>& weights) {
@@ -148,17 +246,16 @@ int Propagator::xcorr_range(float dt, int nt, std::vector<float>& weights) {
   return it_min;
 }
 
-void Propagator::forward(int E, const float* rv, long long rvstride, const Sources& src, float* dcal, float* band,
-                         Fields& w, int rvdiv) {
+void Propagator::forward_body(int E, const float* rv, long long rvstride, const Sources& src, float* dcal,
+                              float* band, Fields& fw, int rvdiv, int w) {
   hipStream_t s = stream();
   const long long np = g_.npts();
   const int nt = g_.nt, ng = g_.ng();
-  w.ensure(E, np, false);
-  launch_fill(w.a.get(), (long long)E * np, 0.f, s);
-  launch_fill(w.b.get(), (long long)E * np, 0.f, s);
+  launch_fill(fw.a.get(), (long long)E * np, 0.f, s);
+  launch_fill(fw.b.get(), (long long)E * np, 0.f, s);
 
   Inject inj;
-  inj.pts = src_.view();
+  inj.pts = src_[w].view();
   inj.amp = src.amp;
   inj.ld = src.ld;
   inj.estride = src.estride;
@@ -168,55 +265,222 @@ void Propagator::forward(int E, const float* rv, long long rvstride, const Sourc
 
   // step 0 injects into the initially zero cur (addSource before stepForward)
   inj.t = 0;
-  launch_point_inject(w.b.get(), np, inj, g_.ns(), E, s);
+  launch_point_inject(fw.b.get(), np, inj, g_.ns(), E, s);
 
   FwdStepArgs a;
-  a.g = grid();
+  a.g = grid(w);
   a.fstride = np;
   a.rv = rv;
   a.rvstride = rvstride;
   a.rvdiv = rvdiv;
   a.inj = inj;
-  a.rec.pts = rec_.view();
+  a.rec.pts = rec_[w].view();
   a.rec.estride = (long long)nt * ng;
   a.band_estride = (long long)nt * g_.bsize;
   for (int it = 0; it < nt; it++) {
-    float* cur = (it % 2 == 0) ? w.b.get() : w.a.get();
-    float* prv = (it % 2 == 0) ? w.a.get() : w.b.get();
+    float* cur = (it % 2 == 0) ? fw.b.get() : fw.a.get();
+    float* prv = (it % 2 == 0) ? fw.a.get() : fw.b.get();
     a.cur = cur;
     a.prev = prv;
     a.inj.t = it + 1 < nt ? it + 1 : -1;
     a.rec.out = dcal ? dcal + (long long)it * ng : nullptr;
     a.band = band ? band + (long long)it * g_.bsize : nullptr;
-    launch_fwd_step(a, E, true, prec_, s);
-    fwd_launches++;
+    timed_launch(0, [&] { launch_fwd_step(a, E, true, prec_, s); });
   }
-  w.s_cur = ((nt - 1) % 2 == 0) ? w.b.get() : w.a.get();
-  w.s_nxt = ((nt - 1) % 2 == 0) ? w.a.get() : w.b.get();
+  fw.s_cur = ((nt - 1) % 2 == 0) ? fw.b.get() : fw.a.get();
+  fw.s_nxt = ((nt - 1) % 2 == 0) ? fw.a.get() : fw.b.get();
   SWFWI_HIP(hipGetLastError());
+}
+
+void Propagator::forward(int E, const float* rv, long long rvstride, const Sources& src, float* dcal, float* band,
+                         Fields& w, int rvdiv) {
+  const long long np = g_.npts();
+  w.ensure(E, np, false);
+  // tune both step kernels for this batch now, while the workspace is free
+  // (backward() runs on the state this forward leaves behind)
+  const int tw = tile_choice(0, E, rv, rvstride, rvdiv, w, true);
+  tile_choice(1, E, rv, rvstride, rvdiv, w, true);
+  const uintptr_t key[] = {1,
+                           (uintptr_t)tw,
+                           (uintptr_t)E,
+                           (uintptr_t)rv,
+                           (uintptr_t)rvstride,
+                           (uintptr_t)rvdiv,
+                           (uintptr_t)src.amp,
+                           (uintptr_t)src.ld,
+                           (uintptr_t)src.estride,
+                           (uintptr_t)src.lo,
+                           (uintptr_t)src.hi,
+                           (uintptr_t)dcal,
+                           (uintptr_t)band,
+                           (uintptr_t)w.a.get(),
+                           (uintptr_t)w.b.get(),
+                           (uintptr_t)prec_,
+                           (uintptr_t)timing_};
+  run_maybe_graph(std::vector<uintptr_t>(std::begin(key), std::end(key)),
+                  [&] { forward_body(E, rv, rvstride, src, dcal, band, w, rvdiv, tw); });
+  w.s_cur = ((g_.nt - 1) % 2 == 0) ? w.b.get() : w.a.get();
+  w.s_nxt = ((g_.nt - 1) % 2 == 0) ? w.a.get() : w.b.get();
+  fwd_launches += g_.nt;
+  fwd_point_updates += (double)E * (g_.nxpad - 2 * kHalo) * (g_.nzpad - 2 * kHalo) * g_.nt;
+}
+
+template <typename F>
+void Propagator::timed_launch(int kind, F&& launch) {
+  if (!timing_) {
+    launch();
+    return;
+  }
+  hipEvent_t ev[2];
+  for (hipEvent_t& e : ev) {
+    if (!event_pool_.empty()) {
+      e = event_pool_.back();
+      event_pool_.pop_back();
+    } else {
+      SWFWI_HIP(hipEventCreate(&e));
+    }
+  }
+  SWFWI_HIP(hipEventRecord(ev[0], stream()));
+  launch();
+  SWFWI_HIP(hipEventRecord(ev[1], stream()));
+  active_events_[kind].push_back(ev[0]);
+  active_events_[kind].push_back(ev[1]);
+}
+
+void Propagator::set_timing(bool on) {
+  timing_ = on;
+  kernel_ms[0] = kernel_ms[1] = 0;
+  kernel_n[0] = kernel_n[1] = 0;
+}
+
+void Propagator::collect_timing() {}
+
+void Propagator::read_events(const std::vector<hipEvent_t>* ev) {
+  SWFWI_HIP(hipStreamSynchronize(stream()));
+  for (int k = 0; k < 2; k++)
+    for (size_t i = 0; i + 1 < ev[k].size(); i += 2) {
+      float ms = 0;
+      SWFWI_HIP(hipEventElapsedTime(&ms, ev[k][i], ev[k][i + 1]));
+      kernel_ms[k] += ms;
+      kernel_n[k] += 1;
+    }
+}
+
+void Propagator::run_maybe_graph(const std::vector<uintptr_t>& key, const std::function<void()>& body) {
+  hipStream_t s = stream();
+  active_events_[0].clear();
+  active_events_[1].clear();
+  // HIP cannot time events recorded inside a captured graph, so the timed
+  // mode launches the loop directly (same kernels, same stream).
+  if (!graphs_ || timing_) {
+    body();
+    if (timing_) {
+      read_events(active_events_);
+      for (int k = 0; k < 2; k++) event_pool_.insert(event_pool_.end(), active_events_[k].begin(), active_events_[k].end());
+    }
+    return;
+  }
+  for (auto& ge : graph_cache_) {
+    if (ge.key == key) {
+      SWFWI_HIP(hipGraphLaunch(ge.exec, s));
+      if (timing_) read_events(ge.events);
+      return;
+    }
+  }
+  hipGraph_t graph = nullptr;
+  SWFWI_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  try {
+    body();
+  } catch (...) {
+    (void)hipStreamEndCapture(s, &graph);
+    if (graph) (void)hipGraphDestroy(graph);
+    throw;
+  }
+  SWFWI_HIP(hipStreamEndCapture(s, &graph));
+  hipGraphExec_t exec = nullptr;
+  SWFWI_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+  (void)hipGraphDestroy(graph);
+  if (graph_cache_.size() >= 16) {
+    drop_graph(graph_cache_.front());
+    graph_cache_.erase(graph_cache_.begin());
+  }
+  GraphEntry ge;
+  ge.key = key;
+  ge.exec = exec;
+  ge.events[0] = active_events_[0];
+  ge.events[1] = active_events_[1];
+  graph_cache_.push_back(ge);
+  SWFWI_HIP(hipGraphLaunch(exec, s));
+  if (timing_) read_events(ge.events);
+}
+
+void Propagator::drop_graph(GraphEntry& ge) {
+  (void)hipGraphExecDestroy(ge.exec);
+  for (int k = 0; k < 2; k++)
+    for (hipEvent_t e : ge.events[k]) (void)hipEventDestroy(e);
+}
+
+Propagator::~Propagator() {
+  for (auto& ge : graph_cache_) drop_graph(ge);
+  for (hipEvent_t e : event_pool_) (void)hipEventDestroy(e);
 }
 
 int Propagator::backward(int E, const float* rv, long long rvstride, const Sources& src, const float* vsrc,
                          long long vstride, const float* band, float* grad, long long gstride, Fields& w,
                          int rvdiv) {
-  hipStream_t s = stream();
   const long long np = g_.npts();
-  const int nt = g_.nt, ng = g_.ng();
   if (!w.s_cur) fail(kErrState, "backward() needs the state of a forward() run");
   w.ensure(E, np, true);
   std::vector<float> wts;
-  const int it_min = xcorr_range(g_.dt, nt, wts);
-  if (it_min >= nt) return 0;
+  const int it_min = xcorr_range(g_.dt, g_.nt, wts);
+  if (it_min >= g_.nt) return 0;
+  const int tw = tile_choice(1, E, rv, rvstride, rvdiv, w, false);
+  const uintptr_t key[] = {2,
+                           (uintptr_t)tw,
+                           (uintptr_t)E,
+                           (uintptr_t)rv,
+                           (uintptr_t)rvstride,
+                           (uintptr_t)rvdiv,
+                           (uintptr_t)src.amp,
+                           (uintptr_t)src.ld,
+                           (uintptr_t)src.estride,
+                           (uintptr_t)src.lo,
+                           (uintptr_t)src.hi,
+                           (uintptr_t)vsrc,
+                           (uintptr_t)vstride,
+                           (uintptr_t)band,
+                           (uintptr_t)grad,
+                           (uintptr_t)gstride,
+                           (uintptr_t)w.s_cur,
+                           (uintptr_t)w.s_nxt,
+                           (uintptr_t)w.c.get(),
+                           (uintptr_t)w.d.get(),
+                           (uintptr_t)prec_,
+                           (uintptr_t)timing_};
+  run_maybe_graph(std::vector<uintptr_t>(std::begin(key), std::end(key)), [&] {
+    backward_body(E, rv, rvstride, src, vsrc, vstride, band, grad, gstride, w, rvdiv, it_min, wts, tw);
+  });
+  const int iters = g_.nt - it_min;
+  bwd_launches += iters;
+  bwd_point_updates += 2.0 * E * (g_.nxpad - 2 * kHalo) * (g_.nzpad - 2 * kHalo) * iters;
+  return iters;
+}
 
-  float* s_cur = w.s_cur;
-  float* s_prv = w.s_nxt;
-  float* g_cur = w.c.get();
-  float* g_prv = w.d.get();
+void Propagator::backward_body(int E, const float* rv, long long rvstride, const Sources& src, const float* vsrc,
+                               long long vstride, const float* band, float* grad, long long gstride, Fields& fw,
+                               int rvdiv, int it_min, const std::vector<float>& wts, int w) {
+  hipStream_t s = stream();
+  const long long np = g_.npts();
+  const int nt = g_.nt, ng = g_.ng();
+  float* s_cur = fw.s_cur;
+  float* s_prv = fw.s_nxt;
+  float* g_cur = fw.c.get();
+  float* g_prv = fw.d.get();
   launch_fill(g_cur, (long long)E * np, 0.f, s);
   launch_fill(g_prv, (long long)E * np, 0.f, s);
 
   Inject ginj;
-  ginj.pts = rec_.view();
+  ginj.pts = rec_[w].view();
   ginj.amp = vsrc;
   ginj.ld = ng;
   ginj.estride = vstride;
@@ -225,12 +489,12 @@ int Propagator::backward(int E, const float* rv, long long rvstride, const Sourc
   launch_point_inject(g_cur, np, ginj, ng, E, s);
 
   BwdStepArgs a;
-  a.g = grid();
+  a.g = grid(w);
   a.fstride = np;
   a.rv = rv;
   a.rvstride = rvstride;
   a.rvdiv = rvdiv;
-  a.s_sub.pts = src_.view();
+  a.s_sub.pts = src_[w].view();
   a.s_sub.amp = src.amp;
   a.s_sub.ld = src.ld;
   a.s_sub.estride = src.estride;
@@ -242,7 +506,6 @@ int Propagator::backward(int E, const float* rv, long long rvstride, const Sourc
   a.grad = grad;
   a.gstride = gstride;
   a.do_xcorr = 1;
-  int iters = 0;
   for (int it = nt - 1; it >= it_min; it--) {
     a.s_cur = s_cur;
     a.s_prev = s_prv;
@@ -253,14 +516,11 @@ int Propagator::backward(int E, const float* rv, long long rvstride, const Sourc
     a.g_inj.t = more ? it - 1 : -1;
     a.band = more ? band + (long long)(it - 1) * g_.bsize : nullptr;
     a.weight = wts[it];
-    launch_bwd_step(a, E, prec_, s);
-    bwd_launches++;
+    timed_launch(1, [&] { launch_bwd_step(a, E, prec_, s); });
     std::swap(s_cur, s_prv);
     std::swap(g_cur, g_prv);
-    iters++;
   }
   SWFWI_HIP(hipGetLastError());
-  return iters;
 }
 
 }  // namespace swfwi

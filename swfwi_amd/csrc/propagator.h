@@ -3,6 +3,8 @@
 // (fwiframework.cpp:114-177, essfwiframework.cpp:114-173), batched over
 // entries (shots, ensemble members, line-search trials).
 #pragma once
+#include <cstdint>
+#include <functional>
 #include <memory>
 #include <vector>
 
@@ -46,7 +48,8 @@ struct DampTables {
   int zdamp = 0;
 };
 void build_damping(int nx, int nz, int nb, DampTables& t, hipStream_t s);
-void build_points(const std::vector<long long>& idx, int ld, int ntz, int ntx, DevPoints& out, hipStream_t s);
+void build_points(const std::vector<long long>& idx, int ld, int ntz, int ntx, int tile_x, DevPoints& out,
+                  hipStream_t s);
 
 // Wavefield workspace for E entries (two buffers per wavefield).
 struct Fields {
@@ -66,9 +69,7 @@ class Propagator {
   void set_precision(int p) { prec_ = p; }
   hipStream_t stream() const { return stream_.get(); }
   void sync() const { stream_.sync(); }
-  Grid grid() const;
-  const DevPoints& src_points() const { return src_; }
-  const DevPoints& rec_points() const { return rec_; }
+  Grid grid(int w = 1) const;  // w indexes kTileWidths
   const int* src_xp() const { return sxp_.get(); }  // padded coordinates (device)
   const int* src_zp() const { return szp_.get(); }
   const int* rec_xp() const { return gxp_.get(); }
@@ -104,13 +105,62 @@ class Propagator {
 
   // launch counters for throughput accounting
   long long fwd_launches = 0, bwd_launches = 0;
+  double fwd_point_updates = 0, bwd_point_updates = 0;
+
+  // Time loops are captured into HIP graphs and replayed when the same
+  // buffers come back (the drivers' epochs); off by default for one-shot calls.
+  void set_graphs(bool on) { graphs_ = on; }
+  // Tile width override (8, 16, 32) or 0 for the tuner; tuning off falls back
+  // to a size heuristic.
+  void set_tile_x(int tx);
+  void set_tuning(bool on) { tune_ = on; }
+  int tile_choice_cached(int kind, int E) const;
+  bool graphs() const { return graphs_; }
+  // Per-launch HIP events around every stencil kernel (forward = 0,
+  // backward = 1), read back after each time loop into kernel_ms / kernel_n.
+  // Timed loops are launched directly, not from a graph.
+  void set_timing(bool on);
+  void collect_timing();
+  double kernel_ms[2] = {0, 0};
+  long long kernel_n[2] = {0, 0};
+
+  ~Propagator();
+  Propagator(const Propagator&) = delete;
+  Propagator& operator=(const Propagator&) = delete;
 
  private:
+  void forward_body(int E, const float* rv, long long rvstride, const Sources& src, float* dcal, float* band,
+                    Fields& fw, int rvdiv, int w);
+  void backward_body(int E, const float* rv, long long rvstride, const Sources& src, const float* vsrc,
+                     long long vstride, const float* band, float* grad, long long gstride, Fields& fw, int rvdiv,
+                     int it_min, const std::vector<float>& wts, int w);
+  // Tile width (index into kTileWidths) for a kernel kind (0 forward,
+  // 1 backward) and batch size: fixed override, cached choice, or a timed
+  // trial of every width on scratch wavefields (the tile tuner).
+  int tile_choice(int kind, int E, const float* rv, long long rvstride, int rvdiv, Fields& w, bool may_time);
+  void run_maybe_graph(const std::vector<uintptr_t>& key, const std::function<void()>& body);
+  template <typename F>
+  void timed_launch(int kind, F&& launch);
+
+  struct GraphEntry {
+    std::vector<uintptr_t> key;
+    hipGraphExec_t exec = nullptr;
+    std::vector<hipEvent_t> events[2];  // timing events captured into the graph
+  };
+  void drop_graph(GraphEntry& ge);
+  void read_events(const std::vector<hipEvent_t>* ev);
+  std::vector<GraphEntry> graph_cache_;
+  bool graphs_ = false, timing_ = false;
+  std::vector<hipEvent_t> active_events_[2], event_pool_;
+
   Geometry g_;
   int prec_;
   Stream stream_;
   DampTables damp_;
-  DevPoints src_, rec_;
+  DevPoints src_[3], rec_[3];
+  int tile_override_ = 0;                 // 0 = tune, else a width from kTileWidths
+  bool tune_ = true;
+  std::vector<std::pair<long long, int>> tile_cache_;  // (kind*2^32 + E) -> width index
   DBuf<int> sxp_, szp_, gxp_, gzp_;
 };
 

@@ -58,11 +58,7 @@ void Solver::set_wavelet(const float* wlt_host) {
   prop_.sync();
 }
 
-double Solver::point_updates() const {
-  const Geometry& g = prop_.geo();
-  const double per = (double)(g.nxpad - 12) * (g.nzpad - 12);
-  return per * (double)prop_.fwd_launches + 2.0 * per * (double)prop_.bwd_launches;
-}
+double Solver::point_updates() const { return prop_.fwd_point_updates + prop_.bwd_point_updates; }
 
 void Solver::forward_modeling(const float* encsrc, int shot, float* dcal_host) {
   if (!bound_) fail(kErrState, "bind the velocity first");
@@ -180,6 +176,8 @@ Inversion::Inversion(Solver& s, const float* wlt_host, const float* dobs_host, c
   tmin_ = (g.dx / g.dt / p.vmax) * (g.dx / g.dt / p.vmax);
   tmax_ = (g.dx / g.dt / p.vmin) * (g.dx / g.dt / p.vmin);
   if (tmax_ <= tmin_) fail(kErrArg, "vmax must exceed vmin");
+  // epochs re-run the same time loops on the same buffers: replay graphs
+  s.prop().set_graphs(true);
   s.set_wavelet(wlt_host);
   const size_t m = (size_t)g.ns() * g.nt * g.ng();
   dobs_.upload(dobs_host, m, s.stream());

@@ -12,10 +12,18 @@
 //            same tile, subSource, the zero-lag imaging condition, and the
 //            band restore / receiver injection that the next iteration needs.
 //
-// A tile is kTZ x kTX output points.  Its `cur` (with a 6-cell halo) is staged
-// in LDS, the unscaled 10th-order Laplacian u2 is formed in LDS for the tile
-// plus a 1-cell ring, and each thread then finishes kTX*kTZ/kThreads points of
-// one z-row.  HBM traffic per point and step: read cur, prev, rv; write next.
+// A tile is kTZ x TX output points (TX = 8, 16 or 32, picked per workload by
+// the propagator's tile tuner).  Its `cur` plus a 6-cell halo is staged in
+// LDS with every global load of the block in flight at once, the unscaled
+// 10th-order Laplacian u2 is formed in LDS for the tile plus a 1-cell ring,
+// and each thread finishes TX/4 points of one z-row.  prev, the model and
+// (backward) the gradient of the thread's points are prefetched into
+// registers before the staging so their latency overlaps it.
+//
+// Block order: blocks are spread round-robin over the 8 XCDs, so the block
+// index is remapped to give each XCD one contiguous run of work, and inside a
+// run all entries of a tile come back to back: the tile's model and the
+// halos of neighbouring tiles are then re-read from that XCD's L2.
 //
 // Precision: kExact reproduces the reference's expression trees
 // (fd4t10s-damp-zjh.c:44-54,85-88; fd4t10s-zjh.c:28-38,49-52): float ring
@@ -27,12 +35,18 @@
 namespace swfwi {
 namespace {
 
-constexpr int LZ = kTZ + 2 * kHalo;  // LDS row length of the staged cur tile
-constexpr int LX = kTX + 2 * kHalo;
-constexpr int UZ = kTZ + 2;          // LDS row length of the u2 tile
-constexpr int UX = kTX + 2;
-constexpr int kRows = kThreads / kTZ;          // thread rows per block
-constexpr int kPer = kTX / kRows;              // outputs per thread
+constexpr int kRows = kThreads / kTZ;  // thread rows per block (4)
+
+template <int TX>
+struct Tile {
+  static constexpr int LZ = kTZ + 2 * kHalo;  // LDS row length of the staged cur tile
+  static constexpr int LX = TX + 2 * kHalo;
+  static constexpr int UZ = kTZ + 2;          // LDS row length of the u2 tile
+  static constexpr int UX = TX + 2;
+  static constexpr int PER = TX / kRows;      // outputs per thread
+  static constexpr int NSTAGE = (LX * LZ + kThreads - 1) / kThreads;
+  static constexpr int NU2 = (UX * UZ + kThreads - 1) / kThreads;
+};
 
 // Zhang Jinhai coefficients as float, as the reference stores them.
 constexpr float A0 = 1.53400796f, A1 = 1.78858721f, A2 = -0.31660756f;
@@ -40,7 +54,7 @@ constexpr float A3 = 0.07612173f, A4 = -0.01626042f, A5 = 0.00216736f;
 constexpr double M4A0 = -4.0 * (double)A0;
 constexpr float C12 = 1.0f / 12;
 
-template <int P>
+template <int P, int LZ>
 __device__ __forceinline__ float u2_at(const float* __restrict__ s) {
   // s: LDS element; z neighbours at +-k, x neighbours at +-k*LZ
   const float r1 = ((s[-1] + s[1]) + s[-LZ]) + s[LZ];
@@ -69,12 +83,12 @@ __device__ __forceinline__ float u2_at(const float* __restrict__ s) {
 
 // next from cur c, prev p, reciprocal model rv, u2 tile pointer u (at the
 // point), damping delta (0 inside; ignored when !DAMPED).
-template <bool DAMPED, int P>
+template <bool DAMPED, int P, int UZ>
 __device__ __forceinline__ float update_at(float c, float p, float rv, const float* __restrict__ u, float delta) {
   float lap = ((u[-1] + u[1]) + u[-UZ]) + u[UZ];
   lap = lap - 4 * u[0];
+  const float cc = (C12 * rv) * rv;
   if constexpr (P == kExact) {
-    const float cc = (C12 * rv) * rv;
     double acc;
     if constexpr (DAMPED) {
       const double a = (2. - (double)(2 * delta)) + (double)(delta * delta);
@@ -89,7 +103,6 @@ __device__ __forceinline__ float update_at(float c, float p, float rv, const flo
     acc = acc + (double)(cc * lap);
     return (float)acc;
   } else {
-    const float cc = (C12 * rv) * rv;
     float a = 2.0f, b = 1.0f;
     if constexpr (DAMPED) {
       a = (2.0f - 2 * delta) + delta * delta;
@@ -122,28 +135,54 @@ __device__ __forceinline__ float damping_delta(const Grid& g, int ix, int iz) {
   return iz >= g.zdamp ? g.delta_z[iz] : g.delta_x[ix];
 }
 
-// Stage cur (tile + halo) into LDS; zero outside the grid.
+// Block -> (tile, entry): XCD-contiguous runs, entries fastest.
+__device__ __forceinline__ void block_work(int ntiles, int nentries, int& tile, int& e) {
+  const int n = ntiles * nentries;
+  const int b = blockIdx.x;
+  const int per = n / 8, rem = n % 8;
+  const int xcd = b % 8, local = b / 8;
+  const int l = xcd * per + (xcd < rem ? xcd : rem) + local;
+  tile = l / nentries;
+  e = l - tile * nentries;
+}
+
+// Stage cur (tile + halo) into LDS, zero outside the grid; all loads of the
+// thread are issued before the first LDS store.
+template <int TX>
 __device__ __forceinline__ void stage_tile(const Grid& g, const float* __restrict__ src, float* __restrict__ lds,
                                            int x0, int z0) {
-  for (int i = threadIdx.x; i < LX * LZ; i += kThreads) {
-    const int lx = i / LZ, lz = i - lx * LZ;
+  using T = Tile<TX>;
+  float v[T::NSTAGE];
+#pragma unroll
+  for (int k = 0; k < T::NSTAGE; k++) {
+    const int i = threadIdx.x + k * kThreads;
+    const int lx = i / T::LZ, lz = i - lx * T::LZ;
     const int gx = x0 - kHalo + lx, gz = z0 - kHalo + lz;
-    float v = 0.f;
-    if (gx >= 0 && gx < g.nx && gz >= 0 && gz < g.nz) v = src[(long long)gx * g.ld + gz];
-    lds[i] = v;
+    const bool ok = i < T::LX * T::LZ && gx >= 0 && gx < g.nx && gz >= 0 && gz < g.nz;
+    v[k] = ok ? src[(long long)gx * g.ld + gz] : 0.f;
+  }
+#pragma unroll
+  for (int k = 0; k < T::NSTAGE; k++) {
+    const int i = threadIdx.x + k * kThreads;
+    if (i < T::LX * T::LZ) lds[i] = v[k];
   }
 }
 
 // u2 over the tile plus a one-cell ring, only where the reference defines it
 // ([5, n-5) in both directions); elsewhere 0 (never read by a valid output).
-template <int P>
+template <int P, int TX>
 __device__ __forceinline__ void form_u2(const Grid& g, const float* __restrict__ cur, float* __restrict__ u2,
                                         int x0, int z0) {
-  for (int i = threadIdx.x; i < UX * UZ; i += kThreads) {
-    const int ux = i / UZ, uz = i - ux * UZ;
+  using T = Tile<TX>;
+#pragma unroll
+  for (int k = 0; k < T::NU2; k++) {
+    const int i = threadIdx.x + k * kThreads;
+    if (i >= T::UX * T::UZ) break;
+    const int ux = i / T::UZ, uz = i - ux * T::UZ;
     const int gx = x0 - 1 + ux, gz = z0 - 1 + uz;
     float v = 0.f;
-    if (gx >= 5 && gx < g.nx - 5 && gz >= 5 && gz < g.nz - 5) v = u2_at<P>(cur + (ux + kHalo - 1) * LZ + (uz + kHalo - 1));
+    if (gx >= 5 && gx < g.nx - 5 && gz >= 5 && gz < g.nz - 5)
+      v = u2_at<P, T::LZ>(cur + (ux + kHalo - 1) * T::LZ + (uz + kHalo - 1));
     u2[i] = v;
   }
 }
@@ -152,7 +191,8 @@ __device__ __forceinline__ bool entry_active(const Inject& in, int e, int pid) {
   return in.lo == nullptr || (pid >= in.lo[e] && pid < in.hi[e]);
 }
 
-// Apply all active point amplitudes of this tile that land on grid index q.
+// Apply all active point amplitudes of this tile that land on grid index q,
+// in point order (manipSource, damp4t10d.cpp:204-214).
 __device__ __forceinline__ float apply_points(const Inject& in, int tile, int e, long long q, float v) {
   const int b = in.pts.tile_ptr[tile], eidx = in.pts.tile_ptr[tile + 1];
   for (int j = b; j < eidx; j++) {
@@ -165,39 +205,49 @@ __device__ __forceinline__ float apply_points(const Inject& in, int tile, int e,
   return v;
 }
 
-template <bool DAMPED, int P>
-__global__ __launch_bounds__(kThreads) void k_fwd_step(FwdStepArgs a) {
-  __shared__ float s_cur[LX * LZ];
-  __shared__ float s_u2[UX * UZ];
+template <bool DAMPED, int P, int TX>
+__global__ __launch_bounds__(kThreads) void k_fwd_step(FwdStepArgs a, int nentries) {
+  using T = Tile<TX>;
+  __shared__ float s_cur[T::LX * T::LZ];
+  __shared__ float s_u2[T::UX * T::UZ];
   const Grid& g = a.g;
-  const int tile = blockIdx.x, e = blockIdx.y;
+  int tile, e;
+  block_work(g.ntz * g.ntx, nentries, tile, e);
   const int tz = tile % g.ntz, tx = tile / g.ntz;
-  const int z0 = kHalo + tz * kTZ, x0 = kHalo + tx * kTX;
+  const int z0 = kHalo + tz * kTZ, x0 = kHalo + tx * TX;
   const long long fo = (long long)e * a.fstride;
-  const float* __restrict__ cur = a.cur + fo;
   float* __restrict__ prev = a.prev + fo;
   const float* __restrict__ rv = a.rv + (long long)(e / a.rvdiv) * a.rvstride;
 
-  stage_tile(g, cur, s_cur, x0, z0);
-  __syncthreads();
-  form_u2<P>(g, s_cur, s_u2, x0, z0);
-  __syncthreads();
-
   const int lz = threadIdx.x % kTZ, row = threadIdx.x / kTZ;
   const int iz = z0 + lz;
+  float pv[T::PER], rr[T::PER];
+#pragma unroll
+  for (int k = 0; k < T::PER; k++) {
+    const int ix = x0 + row + kRows * k;
+    const bool ok = ix < g.nx - kHalo && iz < g.nz - kHalo;
+    const long long q = (long long)ix * g.ld + iz;
+    pv[k] = ok ? prev[q] : 0.f;
+    rr[k] = ok ? rv[q] : 1.f;
+  }
+  stage_tile<TX>(g, a.cur + fo, s_cur, x0, z0);
+  __syncthreads();
+  form_u2<P, TX>(g, s_cur, s_u2, x0, z0);
+  __syncthreads();
+
   const bool has_inj = a.inj.t >= 0 && a.inj.pts.tile_ptr[tile + 1] > a.inj.pts.tile_ptr[tile];
   const bool has_rec = a.rec.out != nullptr && a.rec.pts.tile_ptr[tile + 1] > a.rec.pts.tile_ptr[tile];
 #pragma unroll
-  for (int k = 0; k < kPer; k++) {
+  for (int k = 0; k < T::PER; k++) {
     const int lx = row + kRows * k;
     const int ix = x0 + lx;
     if (ix >= g.nx - kHalo || iz >= g.nz - kHalo) continue;
     const long long q = (long long)ix * g.ld + iz;
-    const float c = s_cur[(lx + kHalo) * LZ + lz + kHalo];
-    const float* u = s_u2 + (lx + 1) * UZ + lz + 1;
+    const float c = s_cur[(lx + kHalo) * T::LZ + lz + kHalo];
+    const float* u = s_u2 + (lx + 1) * T::UZ + lz + 1;
     if (a.u2out) a.u2out[fo + q] = u[0];
     const float delta = DAMPED ? damping_delta(g, ix, iz) : 0.f;
-    float nxt = update_at<DAMPED, P>(c, prev[q], rv[q], u, delta);
+    float nxt = update_at<DAMPED, P, T::UZ>(c, pv[k], rr[k], u, delta);
     if (has_inj) nxt = apply_points(a.inj, tile, e, q, nxt);
     prev[q] = nxt;
     if (a.band) {
@@ -215,11 +265,10 @@ __global__ __launch_bounds__(kThreads) void k_fwd_step(FwdStepArgs a) {
   // u2 of the ring [5,6) and [n-6,n-5) that no output point owns: the
   // reference writes it too, so the optional u2 export covers it.
   if (a.u2out) {
-    for (int i = threadIdx.x; i < UX * UZ; i += kThreads) {
-      const int ux = i / UZ, uz = i - ux * UZ;
+    for (int i = threadIdx.x; i < T::UX * T::UZ; i += kThreads) {
+      const int ux = i / T::UZ, uz = i - ux * T::UZ;
       const int gx = x0 - 1 + ux, gz = z0 - 1 + uz;
-      // points of the ring may sit in two tiles' u2 halos; both write the
-      // same value
+      // ring points may sit in two tiles' u2 halos; both write the same value
       const bool ring = gx == 5 || gz == 5 || gx == g.nx - 6 || gz == g.nz - 6;
       if (ring && gx >= 5 && gx < g.nx - 5 && gz >= 5 && gz < g.nz - 5)
         a.u2out[fo + (long long)gx * g.ld + gz] = s_u2[i];
@@ -227,51 +276,63 @@ __global__ __launch_bounds__(kThreads) void k_fwd_step(FwdStepArgs a) {
   }
 }
 
-template <int P>
-__global__ __launch_bounds__(kThreads) void k_bwd_step(BwdStepArgs a) {
-  __shared__ float s_sc[LX * LZ];
-  __shared__ float s_gc[LX * LZ];
-  __shared__ float s_su[UX * UZ];
-  __shared__ float s_gu[UX * UZ];
+template <int P, int TX>
+__global__ __launch_bounds__(kThreads) void k_bwd_step(BwdStepArgs a, int nentries) {
+  using T = Tile<TX>;
+  __shared__ float s_sc[T::LX * T::LZ];
+  __shared__ float s_gc[T::LX * T::LZ];
+  __shared__ float s_su[T::UX * T::UZ];
+  __shared__ float s_gu[T::UX * T::UZ];
   const Grid& g = a.g;
-  const int tile = blockIdx.x, e = blockIdx.y;
+  int tile, e;
+  block_work(g.ntz * g.ntx, nentries, tile, e);
   const int tz = tile % g.ntz, tx = tile / g.ntz;
-  const int z0 = kHalo + tz * kTZ, x0 = kHalo + tx * kTX;
+  const int z0 = kHalo + tz * kTZ, x0 = kHalo + tx * TX;
   const long long fo = (long long)e * a.fstride;
   const float* __restrict__ rv = a.rv + (long long)(e / a.rvdiv) * a.rvstride;
   float* __restrict__ s_prev = a.s_prev + fo;
   float* __restrict__ g_prev = a.g_prev + fo;
   float* __restrict__ grad = a.grad + (long long)e * a.gstride;
 
-  stage_tile(g, a.s_cur + fo, s_sc, x0, z0);
-  stage_tile(g, a.g_cur + fo, s_gc, x0, z0);
-  __syncthreads();
-  form_u2<P>(g, s_sc, s_su, x0, z0);
-  form_u2<P>(g, s_gc, s_gu, x0, z0);
-  __syncthreads();
-
   const int lz = threadIdx.x % kTZ, row = threadIdx.x / kTZ;
   const int iz = z0 + lz;
+  float sp[T::PER], gp[T::PER], rr[T::PER], gr[T::PER];
+#pragma unroll
+  for (int k = 0; k < T::PER; k++) {
+    const int ix = x0 + row + kRows * k;
+    const bool ok = ix < g.nx - kHalo && iz < g.nz - kHalo;
+    const long long q = (long long)ix * g.ld + iz;
+    sp[k] = ok ? s_prev[q] : 0.f;
+    gp[k] = ok ? g_prev[q] : 0.f;
+    rr[k] = ok ? rv[q] : 1.f;
+    gr[k] = (ok && a.do_xcorr) ? grad[q] : 0.f;
+  }
+  stage_tile<TX>(g, a.s_cur + fo, s_sc, x0, z0);
+  stage_tile<TX>(g, a.g_cur + fo, s_gc, x0, z0);
+  __syncthreads();
+  form_u2<P, TX>(g, s_sc, s_su, x0, z0);
+  form_u2<P, TX>(g, s_gc, s_gu, x0, z0);
+  __syncthreads();
+
   const bool has_sub = a.s_sub.t >= 0 && a.s_sub.pts.tile_ptr[tile + 1] > a.s_sub.pts.tile_ptr[tile];
   const bool has_inj = a.g_inj.t >= 0 && a.g_inj.pts.tile_ptr[tile + 1] > a.g_inj.pts.tile_ptr[tile];
 #pragma unroll
-  for (int k = 0; k < kPer; k++) {
+  for (int k = 0; k < T::PER; k++) {
     const int lx = row + kRows * k;
     const int ix = x0 + lx;
     if (ix >= g.nx - kHalo || iz >= g.nz - kHalo) continue;
     const long long q = (long long)ix * g.ld + iz;
-    const int l = (lx + kHalo) * LZ + lz + kHalo;
-    const int lu = (lx + 1) * UZ + lz + 1;
-    const float r = rv[q];
+    const int l = (lx + kHalo) * T::LZ + lz + kHalo;
+    const int lu = (lx + 1) * T::UZ + lz + 1;
     // source field: stepBackward (undamped) then subSource
-    float sn = update_at<false, P>(s_sc[l], s_prev[q], r, s_su + lu, 0.f);
+    float sn = update_at<false, P, T::UZ>(s_sc[l], sp[k], rr[k], s_su + lu, 0.f);
     if (has_sub) sn = apply_points(a.s_sub, tile, e, q, sn);
     // receiver field: stepForward (damped) of the already injected cur
     const float gc = s_gc[l];
-    float gn = update_at<true, P>(gc, g_prev[q], r, s_gu + lu, damping_delta(g, ix, iz));
+    float gn = update_at<true, P, T::UZ>(gc, gp[k], rr[k], s_gu + lu, damping_delta(g, ix, iz));
     // zero-lag imaging condition with the new source field and the injected
     // receiver field (cross_correlation, fwiframework.cpp:107-112)
-    if (a.do_xcorr) grad[q] = grad[q] - sn * gc * a.weight;
+    if (a.do_xcorr) grad[q] = gr[k] - sn * gc * a.weight;
     // what the next iteration (it-1) expects to find: the band of the source
     // field restored (readBndry) and the receivers injected (addSource)
     if (a.band) {
@@ -317,29 +378,49 @@ inline int grid_for(long long n) {
   return (int)(b > 8192 ? 8192 : (b < 1 ? 1 : b));
 }
 
-}  // namespace
-
-void launch_fwd_step(const FwdStepArgs& a, int nentries, bool damped, int precision, hipStream_t stream) {
-  dim3 grid(a.g.ntz * a.g.ntx, nentries);
+template <int TX>
+void fwd_dispatch(const FwdStepArgs& a, int E, bool damped, int precision, hipStream_t s) {
+  const dim3 grid(a.g.ntz * a.g.ntx * E);
   if (damped) {
     if (precision == kExact)
-      hipLaunchKernelGGL((k_fwd_step<true, kExact>), grid, dim3(kThreads), 0, stream, a);
+      hipLaunchKernelGGL((k_fwd_step<true, kExact, TX>), grid, dim3(kThreads), 0, s, a, E);
     else
-      hipLaunchKernelGGL((k_fwd_step<true, kFp32>), grid, dim3(kThreads), 0, stream, a);
+      hipLaunchKernelGGL((k_fwd_step<true, kFp32, TX>), grid, dim3(kThreads), 0, s, a, E);
   } else {
     if (precision == kExact)
-      hipLaunchKernelGGL((k_fwd_step<false, kExact>), grid, dim3(kThreads), 0, stream, a);
+      hipLaunchKernelGGL((k_fwd_step<false, kExact, TX>), grid, dim3(kThreads), 0, s, a, E);
     else
-      hipLaunchKernelGGL((k_fwd_step<false, kFp32>), grid, dim3(kThreads), 0, stream, a);
+      hipLaunchKernelGGL((k_fwd_step<false, kFp32, TX>), grid, dim3(kThreads), 0, s, a, E);
+  }
+}
+
+template <int TX>
+void bwd_dispatch(const BwdStepArgs& a, int E, int precision, hipStream_t s) {
+  const dim3 grid(a.g.ntz * a.g.ntx * E);
+  if (precision == kExact)
+    hipLaunchKernelGGL((k_bwd_step<kExact, TX>), grid, dim3(kThreads), 0, s, a, E);
+  else
+    hipLaunchKernelGGL((k_bwd_step<kFp32, TX>), grid, dim3(kThreads), 0, s, a, E);
+}
+
+}  // namespace
+
+int tile_count_x(int nx, int tile_x) { return (nx - 2 * kHalo + tile_x - 1) / tile_x; }
+
+void launch_fwd_step(const FwdStepArgs& a, int nentries, bool damped, int precision, hipStream_t stream) {
+  switch (a.g.tile_x) {
+    case 8: fwd_dispatch<8>(a, nentries, damped, precision, stream); break;
+    case 32: fwd_dispatch<32>(a, nentries, damped, precision, stream); break;
+    default: fwd_dispatch<16>(a, nentries, damped, precision, stream); break;
   }
 }
 
 void launch_bwd_step(const BwdStepArgs& a, int nentries, int precision, hipStream_t stream) {
-  dim3 grid(a.g.ntz * a.g.ntx, nentries);
-  if (precision == kExact)
-    hipLaunchKernelGGL((k_bwd_step<kExact>), grid, dim3(kThreads), 0, stream, a);
-  else
-    hipLaunchKernelGGL((k_bwd_step<kFp32>), grid, dim3(kThreads), 0, stream, a);
+  switch (a.g.tile_x) {
+    case 8: bwd_dispatch<8>(a, nentries, precision, stream); break;
+    case 32: bwd_dispatch<32>(a, nentries, precision, stream); break;
+    default: bwd_dispatch<16>(a, nentries, precision, stream); break;
+  }
 }
 
 void launch_fill(float* p, long long n, float v, hipStream_t s) {

@@ -131,6 +131,26 @@ def test_calgradient_bitexact(oracle, small, shot):
     assert np.array_equal(bits(g_gpu), bits(g_cpu))
 
 
+@pytest.mark.parametrize("tile_x", [8, 16, 32])
+def test_every_tile_width_is_bitexact(oracle, small, tile_x):
+    """Each instantiated tile width (the tuner may pick any) reproduces the
+    oracle's seismogram and gradient bit for bit."""
+    case, s, g, vt, vi, wlt = small
+    print(f"tile_x={tile_x}", flush=True)
+    s.set_tile_x(tile_x)
+    try:
+        d_gpu = s.forward_modeling(wlt, shot=0)
+        assert s.tile_x(0, 1) == tile_x
+        assert np.array_equal(bits(d_gpu), bits(g.forward_modeling(vt, wlt, shot=0)))
+        rng = np.random.default_rng(13)
+        vsrc = rng.standard_normal((case.nt, case.ng)).astype(np.float32)
+        assert np.array_equal(bits(s.calgradient(wlt, vsrc, shot=0)), bits(g.calgradient(vt, wlt, vsrc, shot=0)))
+        dobs = s.forward_shots(wlt)
+        assert np.array_equal(bits(dobs[case.ns - 1]), bits(g.forward_modeling(vt, wlt, shot=case.ns - 1)))
+    finally:
+        s.set_tile_x(0)
+
+
 def test_fp32_mode_within_budget(oracle, small):
     case, s, g, vt, vi, wlt = small
     s.set_precision(capi.PREC_FP32)
