@@ -131,6 +131,26 @@ int swfwi_inversion_stats(swfwi_inversion inv, float *out, double *ms);
 int swfwi_inversion_gradient(swfwi_inversion inv, float *grad);   /* scaled + masked */
 int swfwi_inversion_direction(swfwi_inversion inv, float *dir);   /* CG direction */
 
+/* ---- ensemble Kalman analysis (EnkfAnalyze) ------------------------------- */
+/* EnkfAnalyze(fm, wlt, dobs, sigfac) (enkfanalyze.h:21-29) for `nmembers`
+ * velocity members on the solver's geometry (members live on the device). */
+int swfwi_enkf_create(swfwi_enkf *out, swfwi_solver h, const float *wlt, const float *dobs, float sigfac,
+                      int nmembers);
+int swfwi_enkf_destroy(swfwi_enkf e);
+/* members [N][nxpad][nzpad] in transformed units (velSet) */
+int swfwi_enkf_set_members(swfwi_enkf e, const float *vels);
+int swfwi_enkf_get_members(swfwi_enkf e, float *vels);
+/* EnkfAnalyze::pAnalyze (enkfanalyze.cpp:113-183, gain 343-495): new ENKF
+ * codes, batched encoded forward runs of all members, FP64 analysis, member
+ * update.  perturbation [N][nt][ng] FP64 may be given; when null it is drawn
+ * N(0, sigma) from a counter-based generator seeded with `seed`. */
+int swfwi_enkf_analyze(swfwi_enkf e, const double *perturbation, unsigned long long seed);
+/* EnkfAnalyze::pCreateAMean (enkfanalyze.cpp:578-597): mean [nxpad][nzpad] */
+int swfwi_enkf_mean(swfwi_enkf e, float *mean);
+/* sigma of the perturbation, clip position, singular values [N] and the
+ * N x N gain t4 (column major) of the last analysis; any pointer may be null */
+int swfwi_enkf_info(swfwi_enkf e, double *sigma, int *clip, double *svals, double *gain);
+
 #ifdef __cplusplus
 }
 #endif

@@ -47,6 +47,13 @@ SIGNATURES = {
     "swfwi_solver_get_tile_x": (_I, [_VP, _I, _I, C.POINTER(_I)]),
     "swfwi_solver_set_timing": (_I, [_VP, _I]),
     "swfwi_solver_kernel_times": (_I, [_VP, _f64, np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")]),
+    "swfwi_enkf_create": (_I, [C.POINTER(_VP), _VP, _f32, _f32, _F, _I]),
+    "swfwi_enkf_destroy": (_I, [_VP]),
+    "swfwi_enkf_set_members": (_I, [_VP, _f32]),
+    "swfwi_enkf_get_members": (_I, [_VP, _f32]),
+    "swfwi_enkf_analyze": (_I, [_VP, C.c_void_p, C.c_ulonglong]),
+    "swfwi_enkf_mean": (_I, [_VP, _f32]),
+    "swfwi_enkf_info": (_I, [_VP, C.POINTER(C.c_double), C.POINTER(_I), C.c_void_p, C.c_void_p]),
     "swfwi_inversion_create": (_I, [C.POINTER(_VP), _VP, _f32, _f32, _F, _F, _I, _F]),
     "swfwi_inversion_destroy": (_I, [_VP]),
     "swfwi_ess_epoch": (_I, [_VP, _I]),

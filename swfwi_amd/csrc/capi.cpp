@@ -427,3 +427,75 @@ int swfwi_inversion_direction(swfwi_inversion inv, float* dir) {
 }
 
 }  // extern "C"
+
+// ---- ensemble Kalman analysis ------------------------------------------------
+#include "enkf.h"
+
+struct swfwi_enkf_s {
+  std::unique_ptr<swfwi::Enkf> e;
+};
+
+extern "C" {
+
+int swfwi_enkf_create(swfwi_enkf* out, swfwi_solver h, const float* wlt, const float* dobs, float sigfac,
+                      int nmembers) {
+  return guarded([&] {
+    NEED(out);
+    NEED(h);
+    NEED(wlt);
+    NEED(dobs);
+    std::unique_ptr<swfwi_enkf_s> e(new swfwi_enkf_s);
+    e->e.reset(new Enkf(*h->s, wlt, dobs, sigfac, nmembers));
+    *out = e.release();
+  });
+}
+
+int swfwi_enkf_destroy(swfwi_enkf e) {
+  return guarded([&] { delete e; });
+}
+
+int swfwi_enkf_set_members(swfwi_enkf e, const float* vels) {
+  return guarded([&] {
+    NEED(e);
+    NEED(vels);
+    e->e->set_members(vels);
+  });
+}
+
+int swfwi_enkf_get_members(swfwi_enkf e, float* vels) {
+  return guarded([&] {
+    NEED(e);
+    NEED(vels);
+    e->e->get_members(vels);
+  });
+}
+
+int swfwi_enkf_analyze(swfwi_enkf e, const double* perturbation, unsigned long long seed) {
+  return guarded([&] {
+    NEED(e);
+    e->e->analyze(perturbation, seed);
+  });
+}
+
+int swfwi_enkf_mean(swfwi_enkf e, float* mean) {
+  return guarded([&] {
+    NEED(e);
+    NEED(mean);
+    e->e->mean(mean);
+  });
+}
+
+int swfwi_enkf_info(swfwi_enkf e, double* sigma, int* clip, double* svals, double* gain) {
+  return guarded([&] {
+    NEED(e);
+    if (sigma) *sigma = e->e->sigma();
+    if (clip) *clip = e->e->clip();
+    const int N = e->e->members();
+    if (svals)
+      for (int k = 0; k < N && k < (int)e->e->svals().size(); k++) svals[k] = e->e->svals()[k];
+    if (gain)
+      for (size_t k = 0; k < e->e->gain().size(); k++) gain[k] = e->e->gain()[k];
+  });
+}
+
+}  // extern "C"
