@@ -61,7 +61,36 @@ def build(verbose=False, jobs=None):
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, OUT)
+    build_apps(jobs)
     return OUT
+
+
+APPS = {  # binary -> (source, extra defines)
+    "fm-damp": ("fm_damp.cpp", []),
+    "essfwi-damp": ("fwi_damp.cpp", []),
+    "fwi-damp": ("fwi_damp.cpp", ["-DFWI_PER_SHOT"]),
+}
+
+
+def _build_app(item):
+    name, (src, defs) = item
+    bindir = os.path.join(HERE, "bin")
+    out = os.path.join(bindir, name)
+    cmd = [hipcc(), "-O2", "-std=c++17", "-ffp-contract=off", f"--offload-arch={ARCH}"] + defs + [
+        os.path.join(HERE, "apps", src), "-o", out + ".tmp", "-L", HERE, "-lswfwi_hip",
+        "-Wl,-rpath,$ORIGIN/.."]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"app build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_apps(jobs=None):
+    """The reference's command-line tools over the library (swfwi_amd/bin)."""
+    os.makedirs(os.path.join(HERE, "bin"), exist_ok=True)
+    with cf.ThreadPoolExecutor(jobs or 3) as ex:
+        return list(ex.map(_build_app, APPS.items()))
 
 
 if __name__ == "__main__":

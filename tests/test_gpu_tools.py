@@ -1,0 +1,100 @@
+"""The command-line tools (fm-damp, essfwi-damp, fwi-damp) on RSF files, run
+through the drop-in C++ classes, against the oracle."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from swfwi_amd import capi, workloads
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "swfwi_amd", "bin")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def need_gpu():
+    if capi.device_count() < 1:
+        pytest.skip("no HIP device")
+    for b in ("fm-damp", "essfwi-damp", "fwi-damp"):
+        if not os.path.exists(os.path.join(BIN, b)):
+            pytest.fail(f"tool {b} not built (python -m swfwi_amd.build)")
+
+
+def write_rsf(path, data, **keys):
+    data = np.ascontiguousarray(data, np.float32)
+    data.tofile(path + "@")
+    with open(path, "w") as f:
+        for k, v in keys.items():
+            f.write(f"\t{k}={v}\n")
+        f.write(f'\tin="{path}@"\n\tesize=4\n\tdata_format="native_float"\n')
+
+
+def read_rsf(path):
+    keys = {}
+    for tok in open(path).read().split():
+        if "=" in tok:
+            k, v = tok.split("=", 1)
+            keys[k] = v.strip('"')
+    return keys, np.fromfile(keys["in"], np.float32)
+
+
+def run(cmd):
+    print(" ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+    return r.stdout
+
+
+def test_fm_then_essfwi_and_fwi_match_oracle(tmp_path, oracle):
+    case = workloads.small_case(nx=48, nz=32, nb=8, nt=360, ns=3)
+    vtrue, v0 = workloads.models(case)
+    d = str(tmp_path)
+    # RSF stores n1 = nz (fastest), n2 = nx: our [nx][nz] arrays as they are
+    write_rsf(f"{d}/vtrue.rsf", vtrue, n1=case.nz, n2=case.nx, d1=case.dx, d2=case.dx)
+    write_rsf(f"{d}/vinit.rsf", v0, n1=case.nz, n2=case.nx, d1=case.dx, d2=case.dx)
+    sx0, jsx = int(case.sx[0]), int(case.sx[1] - case.sx[0])
+    sx = np.array([sx0 + k * jsx for k in range(case.ns)], np.int32)
+    run([f"{BIN}/fm-damp", f"vinit={d}/vtrue.rsf", f"shots={d}/shots.rsf", "amp=1", f"fm={case.fm}", f"nb={case.nb}",
+         f"dt={case.dt}", f"nt={case.nt}", f"ns={case.ns}", f"ng={case.ng}", f"jsx={jsx}", "jsz=0", "jgx=1", "jgz=0",
+         f"sxbeg={sx0}", "szbeg=1", "gxbeg=0", "gzbeg=0"])
+    keys, shots = read_rsf(f"{d}/shots.rsf")
+    assert int(keys["n1"]) == case.nt and int(keys["n2"]) == case.ng and int(keys["n3"]) == case.ns
+    shots = shots.reshape(case.ns, case.ng, case.nt)
+    g = oracle.Geom(case.nx, case.nz, case.nb, case.dt, case.dx, case.fm, case.nt, sx, np.ones_like(sx), case.gx,
+                    case.gz)
+    vt = oracle.expand_domain(vtrue, case.nb, case.dx, case.dt)
+    wlt = oracle.ricker(case.nt, case.fm, case.dt, 1.0)
+    for s in range(case.ns):
+        ref = g.forward_modeling(vt, wlt, shot=s)
+        assert np.array_equal(shots[s].T.view(np.uint32), ref.view(np.uint32))
+    dobs = np.ascontiguousarray(shots.transpose(0, 2, 1))
+    # the inversion tools, two iterations each, against the oracle epochs
+    oracle.set_sum_mode(1)
+    try:
+        for tool, epoch in (("essfwi-damp", "epoch"), ("fwi-damp", "fwi_epoch")):
+            out = run([f"{BIN}/{tool}", f"vin={d}/vinit.rsf", f"shots={d}/shots.rsf", f"vout={d}/{tool}_v.rsf",
+                       f"absobjs={d}/{tool}_a.rsf", f"norobjs={d}/{tool}_n.rsf", "niter=2", "maxdv=200", "nita=3"])
+            print(out)
+            _, vout = read_rsf(f"{d}/{tool}_v.rsf")
+            _, absobj = read_rsf(f"{d}/{tool}_a.rsf")
+            vmin, vmax = float(keys["vmin"]), float(keys["vmax"])
+            vi = oracle.expand_domain(v0, case.nb, case.dx, case.dt)
+            e = oracle.Ess(g, vi, wlt, dobs, vmin, vmax, 3, 200.0)
+            objs = []
+            for it in range(2):
+                getattr(e, epoch)(it)
+                sc = e.scalars()
+                if it == 0:
+                    objs.append(sc["initobj"])
+                objs.append(sc["updateobj"])
+            assert np.allclose(absobj, objs, rtol=1e-6), (tool, absobj, objs)
+            # last written model (m/s, physical part) vs the oracle's
+            vphys = vout.reshape(2, case.nx, case.nz)[-1]
+            vv = e.vel.copy()
+            oracle.lib().orc_recover_vel(vv.ravel(), vv.size, case.dx, case.dt)
+            ref = vv[case.nb + 6:case.nb + 6 + case.nx, 6:6 + case.nz]
+            assert np.allclose(vphys, ref, rtol=1e-5), tool
+    finally:
+        oracle.set_sum_mode(0)
