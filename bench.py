@@ -64,6 +64,8 @@ def main():
     ap.add_argument("--precision", choices=["exact", "fp32"], default="exact")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--nt", type=int, default=None, help="override nt (testing only)")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch the time loops directly instead of replaying HIP graphs (for kernel tracers)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (shots/members shard across GPUs "
@@ -75,7 +77,7 @@ def main():
     if capi.device_count() < 1:
         print("no HIP device", file=sys.stderr)
         sys.exit(1)
-    case = workloads.config1(nt=args.nt or 4000)
+    case = workloads.ess_301x921(nt=args.nt or 4000)
     prec = capi.PREC_EXACT if args.precision == "exact" else capi.PREC_FP32
     vtrue, v0 = workloads.models(case)
     vt = capi.expand_domain(vtrue, case.nb, case.dx, case.dt)
@@ -91,6 +93,8 @@ def main():
     print(f"[bench] observed data {case.ns} shots in {t_obs:.2f} s", file=sys.stderr, flush=True)
     s.bind_velocity(vi)
     inv = capi.Inversion(s, wlt, dobs, case.vmin, case.vmax, 5, 200.0)
+    if args.no_graphs:
+        s.set_graphs(False)
     objs = []
     for it in range(args.warmup):
         inv.ess_epoch(it)  # also captures the time loops into HIP graphs
@@ -164,7 +168,8 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded layered model; observed data from the same propagator on the true model)",
         "config": {"workload": "ESS-FWI 301x921 dx=12.5 nb=30 nt=4000, 60 shots x 921 receivers, one supershot "
-                               "per epoch (configs[1])", "precision": args.precision, "parallelism": "single GPU"},
+                               "per epoch", "precision_mode": args.precision, "graphs": not args.no_graphs,
+                   "parallelism": "single GPU"},
         "gradient_ms_per_supershot": round(grad_ms, 3),
         "linesearch_ms": round(ls_ms, 3),
         "trial_forward_runs_per_epoch": ntrial,

@@ -15,6 +15,7 @@ class UpdateSteplenOp {
   int maxIterSelectAlpha3() const { return max_iter_select_alpha3; }
   float maxDv() const { return maxdv; }
   const UpdateVelOp &velOp() const { return *updateVelOp; }
+  const Damp4t10d &modeling() const { return *fmMethod; }
 
  private:
   const Damp4t10d *fmMethod;

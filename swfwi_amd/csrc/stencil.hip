@@ -191,6 +191,58 @@ __device__ __forceinline__ bool entry_active(const Inject& in, int e, int pid) {
   return in.lo == nullptr || (pid >= in.lo[e] && pid < in.hi[e]);
 }
 
+// A tile's points of interest staged in LDS at kernel start, so no global
+// load depends on the stencil result: grid index (-1 = inactive for this
+// entry) and, for injections, the amplitude of this step.  Tiles with more
+// than kTilePts points fall back to scanning the global list.
+constexpr int kTilePts = 64;
+struct TilePts {
+  int q[kTilePts];
+  int pid[kTilePts];
+  float amp[kTilePts];
+};
+
+// One thread's share of a tile's point list: threads 0..n-1 load one item
+// each before the staging, and store it into LDS with the staged tile.  n is
+// the tile's item count (uniform over the block).
+struct PtsLoad {
+  int n, q, pid;
+  float amp;
+};
+__device__ __forceinline__ PtsLoad load_points(const PointList& pts, const Inject* in, int tile, int e) {
+  PtsLoad r;
+  const int b = pts.tile_ptr[tile];
+  r.n = pts.tile_ptr[tile + 1] - b;
+  r.q = -1;
+  r.pid = 0;
+  r.amp = 0.f;
+  if (r.n <= kTilePts && (int)threadIdx.x < r.n) {
+    const int pid = pts.items[b + threadIdx.x];
+    r.pid = pid;
+    r.q = pts.idx[pid];
+    if (in) {
+      if (entry_active(*in, e, pid))
+        r.amp = in->amp[e * in->estride + (long long)in->t * in->ld + pid];
+      else
+        r.q = -1;
+    }
+  }
+  return r;
+}
+__device__ __forceinline__ void store_points(const PtsLoad& r, TilePts& tp) {
+  if (r.n <= kTilePts && (int)threadIdx.x < r.n) {
+    tp.q[threadIdx.x] = r.q;
+    tp.pid[threadIdx.x] = r.pid;
+    tp.amp[threadIdx.x] = r.amp;
+  }
+}
+// Add or subtract, in point order, every staged amplitude that lands on q.
+__device__ __forceinline__ float apply_staged(const TilePts& tp, int n, int sign, long long q, float v) {
+  for (int j = 0; j < n; j++)
+    if (tp.q[j] == q) v = sign > 0 ? v + tp.amp[j] : v - tp.amp[j];
+  return v;
+}
+
 // Apply all active point amplitudes of this tile that land on grid index q,
 // in point order (manipSource, damp4t10d.cpp:204-214).
 __device__ __forceinline__ float apply_points(const Inject& in, int tile, int e, long long q, float v) {
@@ -210,6 +262,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd_step(FwdStepArgs a, int nentri
   using T = Tile<TX>;
   __shared__ float s_cur[T::LX * T::LZ];
   __shared__ float s_u2[T::UX * T::UZ];
+  __shared__ TilePts s_inj, s_rec;
   const Grid& g = a.g;
   int tile, e;
   block_work(g.ntz * g.ntx, nentries, tile, e);
@@ -221,7 +274,11 @@ __global__ __launch_bounds__(kThreads) void k_fwd_step(FwdStepArgs a, int nentri
 
   const int lz = threadIdx.x % kTZ, row = threadIdx.x / kTZ;
   const int iz = z0 + lz;
-  float pv[T::PER], rr[T::PER];
+  // every global load of the block is issued here, before the first barrier
+  PtsLoad li{0, -1, 0, 0.f}, lr{0, -1, 0, 0.f};
+  if (a.inj.t >= 0) li = load_points(a.inj.pts, &a.inj, tile, e);
+  if (a.rec.out != nullptr) lr = load_points(a.rec.pts, nullptr, tile, e);
+  float pv[T::PER], rr[T::PER], dl[T::PER];
 #pragma unroll
   for (int k = 0; k < T::PER; k++) {
     const int ix = x0 + row + kRows * k;
@@ -229,14 +286,15 @@ __global__ __launch_bounds__(kThreads) void k_fwd_step(FwdStepArgs a, int nentri
     const long long q = (long long)ix * g.ld + iz;
     pv[k] = ok ? prev[q] : 0.f;
     rr[k] = ok ? rv[q] : 1.f;
+    dl[k] = (DAMPED && ok) ? damping_delta(g, ix, iz) : 0.f;
   }
   stage_tile<TX>(g, a.cur + fo, s_cur, x0, z0);
+  store_points(li, s_inj);
+  store_points(lr, s_rec);
   __syncthreads();
   form_u2<P, TX>(g, s_cur, s_u2, x0, z0);
   __syncthreads();
 
-  const bool has_inj = a.inj.t >= 0 && a.inj.pts.tile_ptr[tile + 1] > a.inj.pts.tile_ptr[tile];
-  const bool has_rec = a.rec.out != nullptr && a.rec.pts.tile_ptr[tile + 1] > a.rec.pts.tile_ptr[tile];
 #pragma unroll
   for (int k = 0; k < T::PER; k++) {
     const int lx = row + kRows * k;
@@ -246,15 +304,18 @@ __global__ __launch_bounds__(kThreads) void k_fwd_step(FwdStepArgs a, int nentri
     const float c = s_cur[(lx + kHalo) * T::LZ + lz + kHalo];
     const float* u = s_u2 + (lx + 1) * T::UZ + lz + 1;
     if (a.u2out) a.u2out[fo + q] = u[0];
-    const float delta = DAMPED ? damping_delta(g, ix, iz) : 0.f;
-    float nxt = update_at<DAMPED, P, T::UZ>(c, pv[k], rr[k], u, delta);
-    if (has_inj) nxt = apply_points(a.inj, tile, e, q, nxt);
+    float nxt = update_at<DAMPED, P, T::UZ>(c, pv[k], rr[k], u, dl[k]);
+    if (li.n > 0)
+      nxt = li.n <= kTilePts ? apply_staged(s_inj, li.n, a.inj.sign, q, nxt) : apply_points(a.inj, tile, e, q, nxt);
     prev[q] = nxt;
     if (a.band) {
       const int bi = band_index(g, ix, iz);
       if (bi >= 0) a.band[(long long)e * a.band_estride + bi] = c;
     }
-    if (has_rec) {
+    if (lr.n > 0 && lr.n <= kTilePts) {
+      for (int j = 0; j < lr.n; j++)
+        if (s_rec.q[j] == q) a.rec.out[(long long)e * a.rec.estride + s_rec.pid[j]] = c;
+    } else if (lr.n > 0) {
       const int b = a.rec.pts.tile_ptr[tile], eidx = a.rec.pts.tile_ptr[tile + 1];
       for (int j = b; j < eidx; j++) {
         const int pid = a.rec.pts.items[j];
@@ -283,6 +344,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd_step(BwdStepArgs a, int nentri
   __shared__ float s_gc[T::LX * T::LZ];
   __shared__ float s_su[T::UX * T::UZ];
   __shared__ float s_gu[T::UX * T::UZ];
+  __shared__ TilePts s_sub, s_inj;
   const Grid& g = a.g;
   int tile, e;
   block_work(g.ntz * g.ntx, nentries, tile, e);
@@ -296,7 +358,12 @@ __global__ __launch_bounds__(kThreads) void k_bwd_step(BwdStepArgs a, int nentri
 
   const int lz = threadIdx.x % kTZ, row = threadIdx.x / kTZ;
   const int iz = z0 + lz;
-  float sp[T::PER], gp[T::PER], rr[T::PER], gr[T::PER];
+  // every global load of the block is issued here, before the first barrier
+  PtsLoad ls{0, -1, 0, 0.f}, li{0, -1, 0, 0.f};
+  if (a.s_sub.t >= 0) ls = load_points(a.s_sub.pts, &a.s_sub, tile, e);
+  if (a.g_inj.t >= 0) li = load_points(a.g_inj.pts, &a.g_inj, tile, e);
+  float sp[T::PER], gp[T::PER], rr[T::PER], gr[T::PER], dl[T::PER], bv[T::PER];
+  unsigned inband = 0;
 #pragma unroll
   for (int k = 0; k < T::PER; k++) {
     const int ix = x0 + row + kRows * k;
@@ -306,16 +373,20 @@ __global__ __launch_bounds__(kThreads) void k_bwd_step(BwdStepArgs a, int nentri
     gp[k] = ok ? g_prev[q] : 0.f;
     rr[k] = ok ? rv[q] : 1.f;
     gr[k] = (ok && a.do_xcorr) ? grad[q] : 0.f;
+    dl[k] = ok ? damping_delta(g, ix, iz) : 0.f;
+    const int bi = (ok && a.band) ? band_index(g, ix, iz) : -1;
+    bv[k] = bi >= 0 ? a.band[(long long)e * a.band_estride + bi] : 0.f;
+    if (bi >= 0) inband |= 1u << k;
   }
   stage_tile<TX>(g, a.s_cur + fo, s_sc, x0, z0);
   stage_tile<TX>(g, a.g_cur + fo, s_gc, x0, z0);
+  store_points(ls, s_sub);
+  store_points(li, s_inj);
   __syncthreads();
   form_u2<P, TX>(g, s_sc, s_su, x0, z0);
   form_u2<P, TX>(g, s_gc, s_gu, x0, z0);
   __syncthreads();
 
-  const bool has_sub = a.s_sub.t >= 0 && a.s_sub.pts.tile_ptr[tile + 1] > a.s_sub.pts.tile_ptr[tile];
-  const bool has_inj = a.g_inj.t >= 0 && a.g_inj.pts.tile_ptr[tile + 1] > a.g_inj.pts.tile_ptr[tile];
 #pragma unroll
   for (int k = 0; k < T::PER; k++) {
     const int lx = row + kRows * k;
@@ -326,20 +397,19 @@ __global__ __launch_bounds__(kThreads) void k_bwd_step(BwdStepArgs a, int nentri
     const int lu = (lx + 1) * T::UZ + lz + 1;
     // source field: stepBackward (undamped) then subSource
     float sn = update_at<false, P, T::UZ>(s_sc[l], sp[k], rr[k], s_su + lu, 0.f);
-    if (has_sub) sn = apply_points(a.s_sub, tile, e, q, sn);
+    if (ls.n > 0)
+      sn = ls.n <= kTilePts ? apply_staged(s_sub, ls.n, a.s_sub.sign, q, sn) : apply_points(a.s_sub, tile, e, q, sn);
     // receiver field: stepForward (damped) of the already injected cur
     const float gc = s_gc[l];
-    float gn = update_at<true, P, T::UZ>(gc, gp[k], rr[k], s_gu + lu, damping_delta(g, ix, iz));
+    float gn = update_at<true, P, T::UZ>(gc, gp[k], rr[k], s_gu + lu, dl[k]);
     // zero-lag imaging condition with the new source field and the injected
     // receiver field (cross_correlation, fwiframework.cpp:107-112)
     if (a.do_xcorr) grad[q] = gr[k] - sn * gc * a.weight;
     // what the next iteration (it-1) expects to find: the band of the source
     // field restored (readBndry) and the receivers injected (addSource)
-    if (a.band) {
-      const int bi = band_index(g, ix, iz);
-      if (bi >= 0) sn = a.band[(long long)e * a.band_estride + bi];
-    }
-    if (has_inj) gn = apply_points(a.g_inj, tile, e, q, gn);
+    if (inband & (1u << k)) sn = bv[k];
+    if (li.n > 0)
+      gn = li.n <= kTilePts ? apply_staged(s_inj, li.n, a.g_inj.sign, q, gn) : apply_points(a.g_inj, tile, e, q, gn);
     s_prev[q] = sn;
     g_prev[q] = gn;
   }

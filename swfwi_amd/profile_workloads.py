@@ -2,7 +2,8 @@
 
     python -m swfwi_amd.profile_workloads copy      # 256 Mi-float device copy (counter calibration)
     python -m swfwi_amd.profile_workloads shots     # 60-shot batched forward, 301x921, nt=400
-    python -m swfwi_amd.profile_workloads ess       # one ESS epoch, 301x921, nt=1000
+    python -m swfwi_amd.profile_workloads ess       # ESS epochs, 301x921, nt=1000 (--no-graphs for counters)
+    python -m swfwi_amd.profile_workloads enkf      # EnKF analyses, 301x921, N members
 """
 import argparse
 import sys
@@ -19,7 +20,7 @@ def run_copy(n):
 
 
 def run_shots(nt):
-    case = workloads.config1(nt=nt)
+    case = workloads.ess_301x921(nt=nt)
     vtrue, _ = workloads.models(case)
     vt = capi.expand_domain(vtrue, case.nb, case.dx, case.dt)
     wlt = capi.ricker(case.nt, case.fm, case.dt, case.amp)
@@ -33,8 +34,8 @@ def run_shots(nt):
           f"({case.ns * nt * case.update_points / dt / 1e9:.1f} Gpts/s incl. transfer)", flush=True)
 
 
-def run_ess(nt, epochs):
-    case = workloads.config1(nt=nt)
+def run_ess(nt, epochs, graphs=True):
+    case = workloads.ess_301x921(nt=nt)
     vtrue, v0 = workloads.models(case)
     vt = capi.expand_domain(vtrue, case.nb, case.dx, case.dt)
     vi = capi.expand_domain(v0, case.nb, case.dx, case.dt)
@@ -45,6 +46,8 @@ def run_ess(nt, epochs):
     dobs = s.forward_shots(wlt)
     s.bind_velocity(vi)
     inv = capi.Inversion(s, wlt, dobs, case.vmin, case.vmax, 5, 200.0)
+    if not graphs:
+        s.set_graphs(False)  # one dispatch per launch, for per-dispatch counters
     for it in range(epochs):
         inv.ess_epoch(it)
         st = inv.stats()
@@ -60,7 +63,7 @@ def run_enkf(nt, members):
 
     from .enkf import Enkf
 
-    case = workloads.config1(nt=nt)
+    case = workloads.ess_301x921(nt=nt)
     vtrue, v0 = workloads.models(case)
     rng = np.random.default_rng(2)
     mem = []
@@ -93,6 +96,7 @@ def main(argv=None):
     ap.add_argument("--n", type=int, default=1 << 28)
     ap.add_argument("--nt", type=int, default=None)
     ap.add_argument("--epochs", type=int, default=2)
+    ap.add_argument("--no-graphs", action="store_true", help="launch directly instead of replaying HIP graphs")
     a = ap.parse_args(argv)
     if capi.device_count() < 1:
         print("no HIP device", file=sys.stderr)
@@ -104,7 +108,7 @@ def main(argv=None):
     elif a.mode == "enkf":
         run_enkf(a.nt or 4000, a.members)
     else:
-        run_ess(a.nt or 1000, a.epochs)
+        run_ess(a.nt or 1000, a.epochs, graphs=not a.no_graphs)
     return 0
 
 

@@ -77,21 +77,21 @@ def smoothed(v, sigma):
     return np.ascontiguousarray(gaussian_filter(v.astype(np.float64), sigma), np.float32)
 
 
-def config0(nt=2000):
+def fwi_201x301(nt=2000):
     """Plain 2-D acoustic FWI: 201x301 at 10 m, nb=20, dt=1 ms, 11 Ricker
     15 Hz sources at depth 1 cell, a receiver at every surface point."""
     nx, nz = 301, 201
     sx = np.arange(0, 301, 30, dtype=np.int32)
-    return Case("config0", nz, nx, 10.0, 20, 0.001, nt, 15.0, sx, np.ones_like(sx),
+    return Case("fwi_201x301", nz, nx, 10.0, 20, 0.001, nt, 15.0, sx, np.ones_like(sx),
                 np.arange(nx, dtype=np.int32), np.zeros(nx, np.int32), 1500.0, 4500.0, seed=0)
 
 
-def config1(nt=4000):
+def ess_301x921(nt=4000):
     """ESS-FWI: 301x921 at 12.5 m, nb=30, dt=1 ms, 60 Ricker 10 Hz shots,
     921 surface receivers."""
     nx, nz = 921, 301
     sx = np.arange(0, 900, 15, dtype=np.int32)
-    return Case("config1", nz, nx, 12.5, 30, 0.001, nt, 10.0, sx, np.ones_like(sx),
+    return Case("ess_301x921", nz, nx, 12.5, 30, 0.001, nt, 10.0, sx, np.ones_like(sx),
                 np.arange(nx, dtype=np.int32), np.zeros(nx, np.int32), 1500.0, 5500.0, seed=1)
 
 

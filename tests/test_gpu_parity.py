@@ -219,19 +219,19 @@ def test_fwi_epoch_matches_oracle(oracle, small):
         s.bind_velocity(vt)
 
 
-def test_config0_shot_full_size(oracle):
+def test_fwi_201x301_shot_full_size(oracle):
     """One config-0 shot at full size (201x301, nb=20, nt=2000): seismogram and
     raw gradient bit-identical to the oracle."""
-    case = workloads.config0()
+    case = workloads.fwi_201x301()
     s, g, vt, vi, wlt = setup_case(oracle, case)
     s.bind_velocity(vi)
-    print("config0 forward", flush=True)
+    print("fwi_201x301 forward", flush=True)
     d_gpu = s.forward_modeling(wlt, shot=5)
     d_cpu = g.forward_modeling(vi, wlt, shot=5)
     assert np.array_equal(bits(d_gpu), bits(d_cpu))
     rng = np.random.default_rng(5)
     vsrc = (rng.standard_normal((case.nt, case.ng)) * 1e-3).astype(np.float32)
-    print("config0 gradient", flush=True)
+    print("fwi_201x301 gradient", flush=True)
     g_gpu = s.calgradient(wlt, vsrc, shot=5)
     g_cpu = g.calgradient(vi, wlt, vsrc, shot=5)
     assert np.array_equal(bits(g_gpu), bits(g_cpu))

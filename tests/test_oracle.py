@@ -36,7 +36,7 @@ def test_kernels_match_golden_fixtures(oracle):
     assert k >= 3
 
 
-def test_kernel_config0_grid_golden(oracle):
+def test_kernel_fwi_201x301_grid_golden(oracle):
     g = np.load(GOLD)
     nx, nz, nb = 353, 233, 26
     rng = np.random.default_rng(int(g["c0grid_vel_seed"]))
