@@ -34,6 +34,7 @@ extern "C" {
 typedef struct swfwi_solver_s *swfwi_solver;       /* Damp4t10d: geometry + bound model */
 typedef struct swfwi_inversion_s *swfwi_inversion; /* FwiFramework/EssFwiFramework state */
 typedef struct swfwi_enkf_s *swfwi_enkf;           /* EnkfAnalyze state */
+typedef struct swfwi_enfwi_s *swfwi_enfwi;         /* ensemble of EssFwiFramework drivers */
 
 const char *swfwi_last_error(void);
 const char *swfwi_version(void);
@@ -112,6 +113,11 @@ int swfwi_solver_get_tile_x(swfwi_solver h, int kind, int entries, int *tile_x);
  * counts of the forward [0] and backward [1] step kernels. */
 int swfwi_solver_set_timing(swfwi_solver h, int on);
 int swfwi_solver_kernel_times(swfwi_solver h, double *ms, long long *launches);
+/* Microseconds per launch of `reps` back-to-back step kernels (kind 0 forward,
+ * 1 backward) of width tile_x (8, 16, 32) over `entries` scratch wavefields of
+ * the bound model: the tile tuner's measurement, for sweeps.  Uses the
+ * handle's workspace, so a following calgradient needs a new forward run. */
+int swfwi_solver_time_steps(swfwi_solver h, int kind, int entries, int tile_x, int reps, float *us);
 
 /* ---- inversion drivers (FwiFramework / EssFwiFramework) ------------------ */
 /* wlt [nt], dobs [ns][nt][ng]; UpdateVelOp(vmin, vmax, dx, dt) and
@@ -150,6 +156,26 @@ int swfwi_enkf_mean(swfwi_enkf e, float *mean);
 /* sigma of the perturbation, clip position, singular values [N] and the
  * N x N gain t4 (column major) of the last analysis; any pointer may be null */
 int swfwi_enkf_info(swfwi_enkf e, double *sigma, int *clip, double *svals, double *gain);
+
+/* ---- ensemble-Kalman FWI (the enfwi-damp driver) --------------------------- */
+/* One EssFwiFramework + UpdateSteplenOp(nita, maxdv) + UpdateVelOp(vmin, vmax)
+ * per member of `e` (main-enfwi-damp.cpp:509-516); the members' velocities are
+ * the ones held by `e`, updated in place. */
+int swfwi_enfwi_create(swfwi_enfwi *out, swfwi_solver h, swfwi_enkf e, float vmin, float vmax, int nita,
+                       float maxdv);
+int swfwi_enfwi_destroy(swfwi_enfwi f);
+/* essfwis[i]->epoch(iter) for every member (main-enfwi-damp.cpp:563-567), all
+ * members' forward runs, gradients and line-search trials batched */
+int swfwi_enfwi_epoch(swfwi_enfwi f, int iter);
+/* calobj of the ensemble-mean model (main-enfwi-damp.cpp:262-283,586-603);
+ * mean [nxpad][nzpad] (may be null) receives the mean model */
+int swfwi_enfwi_mean_objective(swfwi_enfwi f, float *mean, float *obj);
+/* out[N][10] per member as swfwi_inversion_stats; ms[2] = gradient and
+ * line-search time of the whole ensemble; counts[2] = batched trial rounds,
+ * trial forward runs.  Any pointer may be null. */
+int swfwi_enfwi_stats(swfwi_enfwi f, float *out, double *ms, int *counts);
+/* scaled + masked gradients of the last epoch, [N][nxpad][nzpad] */
+int swfwi_enfwi_gradients(swfwi_enfwi f, float *grads);
 
 #ifdef __cplusplus
 }

@@ -69,6 +69,7 @@ APPS = {  # binary -> (source, extra defines)
     "fm-damp": ("fm_damp.cpp", []),
     "essfwi-damp": ("fwi_damp.cpp", []),
     "fwi-damp": ("fwi_damp.cpp", ["-DFWI_PER_SHOT"]),
+    "enfwi-damp": ("enfwi_damp.cpp", []),
 }
 
 

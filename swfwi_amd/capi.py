@@ -47,6 +47,7 @@ SIGNATURES = {
     "swfwi_solver_get_tile_x": (_I, [_VP, _I, _I, C.POINTER(_I)]),
     "swfwi_solver_set_timing": (_I, [_VP, _I]),
     "swfwi_solver_kernel_times": (_I, [_VP, _f64, np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")]),
+    "swfwi_solver_time_steps": (_I, [_VP, _I, _I, _I, _I, C.POINTER(_F)]),
     "swfwi_enkf_create": (_I, [C.POINTER(_VP), _VP, _f32, _f32, _F, _I]),
     "swfwi_enkf_destroy": (_I, [_VP]),
     "swfwi_enkf_set_members": (_I, [_VP, _f32]),
@@ -54,6 +55,12 @@ SIGNATURES = {
     "swfwi_enkf_analyze": (_I, [_VP, C.c_void_p, C.c_ulonglong]),
     "swfwi_enkf_mean": (_I, [_VP, _f32]),
     "swfwi_enkf_info": (_I, [_VP, C.POINTER(C.c_double), C.POINTER(_I), C.c_void_p, C.c_void_p]),
+    "swfwi_enfwi_create": (_I, [C.POINTER(_VP), _VP, _VP, _F, _F, _I, _F]),
+    "swfwi_enfwi_destroy": (_I, [_VP]),
+    "swfwi_enfwi_epoch": (_I, [_VP, _I]),
+    "swfwi_enfwi_mean_objective": (_I, [_VP, C.c_void_p, C.POINTER(_F)]),
+    "swfwi_enfwi_stats": (_I, [_VP, _f32, _f64, _i32]),
+    "swfwi_enfwi_gradients": (_I, [_VP, _f32]),
     "swfwi_inversion_create": (_I, [C.POINTER(_VP), _VP, _f32, _f32, _F, _F, _I, _F]),
     "swfwi_inversion_destroy": (_I, [_VP]),
     "swfwi_ess_epoch": (_I, [_VP, _I]),
@@ -215,6 +222,12 @@ class Solver:
         t = _I(0)
         check(load().swfwi_solver_get_tile_x(self.h, kind, entries, C.byref(t)))
         return t.value
+
+    def time_steps(self, kind, entries, tile_x, reps=100):
+        """Microseconds per launch of back-to-back step kernels on scratch fields."""
+        us = _F(0)
+        check(load().swfwi_solver_time_steps(self.h, kind, entries, tile_x, reps, C.byref(us)))
+        return us.value
 
     def set_timing(self, on):
         check(load().swfwi_solver_set_timing(self.h, int(bool(on))))

@@ -332,6 +332,23 @@ int swfwi_solver_get_tile_x(swfwi_solver h, int kind, int entries, int* tile_x) 
   });
 }
 
+int swfwi_solver_time_steps(swfwi_solver h, int kind, int entries, int tile_x, int reps, float* us) {
+  return guarded([&] {
+    NEED(h);
+    NEED(us);
+    if (kind != 0 && kind != 1) fail(kErrArg, "kind must be 0 (forward) or 1 (backward)");
+    if (entries < 1 || reps < 1) fail(kErrArg, "entries and reps must be >= 1");
+    int width = -1;
+    for (int i = 0; i < 3; i++)
+      if (kTileWidths[i] == tile_x) width = i;
+    if (width < 0) fail(kErrArg, "tile width must be 8, 16 or 32");
+    Solver& s = *h->s;
+    if (!s.rv()) fail(kErrState, "bind the velocity first");
+    *us = s.prop().time_steps(kind, entries, s.rv(), 0, 1, s.w, width, reps, true);
+    s.w.s_cur = s.w.s_nxt = nullptr;  // the workspace no longer holds a forward state
+  });
+}
+
 int swfwi_solver_set_timing(swfwi_solver h, int on) {
   return guarded([&] {
     NEED(h);
@@ -495,6 +512,84 @@ int swfwi_enkf_info(swfwi_enkf e, double* sigma, int* clip, double* svals, doubl
       for (int k = 0; k < N && k < (int)e->e->svals().size(); k++) svals[k] = e->e->svals()[k];
     if (gain)
       for (size_t k = 0; k < e->e->gain().size(); k++) gain[k] = e->e->gain()[k];
+  });
+}
+
+}  // extern "C"
+
+// ---- ensemble-Kalman FWI -------------------------------------------------------
+#include "ensemble.h"
+
+struct swfwi_enfwi_s {
+  std::unique_ptr<swfwi::EnsembleFwi> f;
+};
+
+extern "C" {
+
+int swfwi_enfwi_create(swfwi_enfwi* out, swfwi_solver h, swfwi_enkf e, float vmin, float vmax, int nita,
+                       float maxdv) {
+  return guarded([&] {
+    NEED(out);
+    NEED(h);
+    NEED(e);
+    UpdateParams p;
+    p.vmin = vmin;
+    p.vmax = vmax;
+    p.nita = nita;
+    p.maxdv = maxdv;
+    std::unique_ptr<swfwi_enfwi_s> f(new swfwi_enfwi_s);
+    f->f.reset(new EnsembleFwi(*h->s, *e->e, p));
+    *out = f.release();
+  });
+}
+
+int swfwi_enfwi_destroy(swfwi_enfwi f) {
+  return guarded([&] { delete f; });
+}
+
+int swfwi_enfwi_epoch(swfwi_enfwi f, int iter) {
+  return guarded([&] {
+    NEED(f);
+    if (iter < 0) fail(kErrArg, "iteration must be >= 0");
+    f->f->epoch(iter);
+  });
+}
+
+int swfwi_enfwi_mean_objective(swfwi_enfwi f, float* mean, float* obj) {
+  return guarded([&] {
+    NEED(f);
+    NEED(obj);
+    *obj = f->f->mean_objective(mean);
+  });
+}
+
+int swfwi_enfwi_stats(swfwi_enfwi f, float* out, double* ms, int* counts) {
+  return guarded([&] {
+    NEED(f);
+    const std::vector<EpochStats>& st = f->f->stats();
+    if (out)
+      for (size_t i = 0; i < st.size(); i++) {
+        const EpochStats& s = st[i];
+        const float v[10] = {s.obj1, s.steplen, s.alpha2, s.alpha3, s.obj2, s.obj3, s.updateobj, s.initobj,
+                             (float)s.ntrial, (float)s.parabolic};
+        for (int k = 0; k < 10; k++) out[i * 10 + k] = v[k];
+      }
+    if (ms) {
+      ms[0] = f->f->grad_ms();
+      ms[1] = f->f->linesearch_ms();
+    }
+    if (counts) {
+      counts[0] = f->f->trial_rounds();
+      counts[1] = f->f->trials();
+    }
+  });
+}
+
+int swfwi_enfwi_gradients(swfwi_enfwi f, float* grads) {
+  return guarded([&] {
+    NEED(f);
+    NEED(grads);
+    f->f->download_grads(grads);
   });
 }
 

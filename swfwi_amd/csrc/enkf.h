@@ -34,6 +34,7 @@ class Enkf {
   void set_members(const float* host);  // [N][nxpad][nzpad], transformed
   void get_members(float* host);
   float* members_dev() { return mem_.get(); }
+  const float* dobs_dev() const { return dobs_.get(); }  // [ns][nt][ng]
   // EnkfAnalyze::pAnalyze: perturbation injected ([N][nt][ng], FP64) or, when
   // null, drawn N(0, sigma) from the counter-based generator with `seed`.
   void analyze(const double* e_host, uint64_t seed);

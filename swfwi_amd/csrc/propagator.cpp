@@ -148,6 +148,61 @@ int Propagator::tile_choice_cached(int kind, int E) const {
   return 0;
 }
 
+float Propagator::time_steps(int kind, int E, const float* rv, long long rvstride, int rvdiv, Fields& w, int width,
+                             int reps, bool clear) {
+  // `reps` back-to-back launches of one step kernel on scratch wavefields
+  // (values discarded), timed with one event pair; microseconds per launch
+  hipStream_t s = stream();
+  const long long np = g_.npts();
+  w.ensure(E, np, true);
+  if (clear) {
+    launch_fill(w.a.get(), (long long)E * np, 0.f, s);
+    launch_fill(w.b.get(), (long long)E * np, 0.f, s);
+    launch_fill(w.c.get(), (long long)E * np, 0.f, s);
+    launch_fill(w.d.get(), (long long)E * np, 0.f, s);
+  }
+  hipEvent_t e0, e1;
+  SWFWI_HIP(hipEventCreate(&e0));
+  SWFWI_HIP(hipEventCreate(&e1));
+  SWFWI_HIP(hipEventRecord(e0, s));
+  for (int r = 0; r < reps; r++) {
+    if (kind == 0) {
+      FwdStepArgs a;
+      a.g = grid(width);
+      a.fstride = np;
+      a.cur = (r & 1) ? w.a.get() : w.b.get();
+      a.prev = (r & 1) ? w.b.get() : w.a.get();
+      a.rv = rv;
+      a.rvstride = rvstride;
+      a.rvdiv = rvdiv;
+      launch_fwd_step(a, E, true, prec_, s);
+    } else {
+      BwdStepArgs a;
+      a.g = grid(width);
+      a.fstride = np;
+      a.s_cur = (r & 1) ? w.a.get() : w.b.get();
+      a.s_prev = (r & 1) ? w.b.get() : w.a.get();
+      a.g_cur = (r & 1) ? w.c.get() : w.d.get();
+      a.g_prev = (r & 1) ? w.d.get() : w.c.get();
+      a.rv = rv;
+      a.rvstride = rvstride;
+      a.rvdiv = rvdiv;
+      a.grad = w.c.get();  // scratch only
+      a.gstride = np;
+      a.do_xcorr = 0;
+      launch_bwd_step(a, E, prec_, s);
+    }
+  }
+  SWFWI_HIP(hipEventRecord(e1, s));
+  SWFWI_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  SWFWI_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  SWFWI_HIP(hipGetLastError());
+  return 1e3f * ms / (reps > 0 ? reps : 1);
+}
+
 int Propagator::tile_choice(int kind, int E, const float* rv, long long rvstride, int rvdiv, Fields& w,
                             bool may_time) {
   if (tile_override_) {
@@ -164,65 +219,18 @@ int Propagator::tile_choice(int kind, int E, const float* rv, long long rvstride
     best = tiles16 >= 4 * 256 ? 2 : (tiles16 < 256 ? 0 : 1);
   } else {
     // timed trial on scratch wavefields (the caller's workspace is free at
-    // this point: forward() tunes before it zeroes and runs)
-    hipStream_t s = stream();
-    const long long np = g_.npts();
-    w.ensure(E, np, true);
-    launch_fill(w.a.get(), (long long)E * np, 0.f, s);
-    launch_fill(w.b.get(), (long long)E * np, 0.f, s);
-    launch_fill(w.c.get(), (long long)E * np, 0.f, s);
-    launch_fill(w.d.get(), (long long)E * np, 0.f, s);
-    hipEvent_t e0, e1;
-    SWFWI_HIP(hipEventCreate(&e0));
-    SWFWI_HIP(hipEventCreate(&e1));
-    // three interleaved rounds (the first warms clocks and caches), the best
-    // of the last two per width
+    // this point: forward() tunes before it zeroes and runs); three
+    // interleaved rounds (the first warms clocks and caches), the best of
+    // the last two per width
     float best_of[3] = {0, 0, 0};
-    const int reps = 6;
-    for (int round = 0; round < 3; round++) {
+    for (int round = 0; round < 3; round++)
       for (int i = 0; i < 3; i++) {
-        float ms = 0;
-        SWFWI_HIP(hipEventRecord(e0, s));
-        for (int r = 0; r < reps; r++) {
-          if (kind == 0) {
-            FwdStepArgs a;
-            a.g = grid(i);
-            a.fstride = np;
-            a.cur = (r & 1) ? w.a.get() : w.b.get();
-            a.prev = (r & 1) ? w.b.get() : w.a.get();
-            a.rv = rv;
-            a.rvstride = rvstride;
-            a.rvdiv = rvdiv;
-            launch_fwd_step(a, E, true, prec_, s);
-          } else {
-            BwdStepArgs a;
-            a.g = grid(i);
-            a.fstride = np;
-            a.s_cur = (r & 1) ? w.a.get() : w.b.get();
-            a.s_prev = (r & 1) ? w.b.get() : w.a.get();
-            a.g_cur = (r & 1) ? w.c.get() : w.d.get();
-            a.g_prev = (r & 1) ? w.d.get() : w.c.get();
-            a.rv = rv;
-            a.rvstride = rvstride;
-            a.rvdiv = rvdiv;
-            a.grad = w.c.get();  // scratch only: the timing run's values are discarded
-            a.gstride = np;
-            a.do_xcorr = 0;
-            launch_bwd_step(a, E, prec_, s);
-          }
-        }
-        SWFWI_HIP(hipEventRecord(e1, s));
-        SWFWI_HIP(hipEventSynchronize(e1));
-        SWFWI_HIP(hipEventElapsedTime(&ms, e0, e1));
-        if (round == 1 || (round == 2 && ms < best_of[i])) best_of[i] = ms;
+        const float us = time_steps(kind, E, rv, rvstride, rvdiv, w, i, 8, round == 0 && i == 0);
+        if (round == 1 || (round == 2 && us < best_of[i])) best_of[i] = us;
       }
-    }
     best = 0;
     for (int i = 1; i < 3; i++)
       if (best_of[i] < best_of[best]) best = i;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    SWFWI_HIP(hipGetLastError());
   }
   tile_cache_.push_back({key, best});
   return best;
@@ -317,7 +325,7 @@ void Propagator::forward(int E, const float* rv, long long rvstride, const Sourc
                            (uintptr_t)w.b.get(),
                            (uintptr_t)prec_,
                            (uintptr_t)timing_};
-  run_maybe_graph(std::vector<uintptr_t>(std::begin(key), std::end(key)),
+  run_maybe_graph(std::vector<uintptr_t>(std::begin(key), std::end(key)), E,
                   [&] { forward_body(E, rv, rvstride, src, dcal, band, w, rvdiv, tw); });
   w.s_cur = ((g_.nt - 1) % 2 == 0) ? w.b.get() : w.a.get();
   w.s_nxt = ((g_.nt - 1) % 2 == 0) ? w.a.get() : w.b.get();
@@ -366,13 +374,14 @@ void Propagator::read_events(const std::vector<hipEvent_t>* ev) {
     }
 }
 
-void Propagator::run_maybe_graph(const std::vector<uintptr_t>& key, const std::function<void()>& body) {
+void Propagator::run_maybe_graph(const std::vector<uintptr_t>& key, int E, const std::function<void()>& body) {
   hipStream_t s = stream();
   active_events_[0].clear();
   active_events_[1].clear();
   // HIP cannot time events recorded inside a captured graph, so the timed
-  // mode launches the loop directly (same kernels, same stream).
-  if (!graphs_ || timing_) {
+  // mode launches the loop directly (same kernels, same stream).  Large
+  // batches are not launch-bound: a graph would only add its capture.
+  if (!graphs_ || timing_ || E > kGraphMaxEntries) {
     body();
     if (timing_) {
       read_events(active_events_);
@@ -457,7 +466,7 @@ int Propagator::backward(int E, const float* rv, long long rvstride, const Sourc
                            (uintptr_t)w.d.get(),
                            (uintptr_t)prec_,
                            (uintptr_t)timing_};
-  run_maybe_graph(std::vector<uintptr_t>(std::begin(key), std::end(key)), [&] {
+  run_maybe_graph(std::vector<uintptr_t>(std::begin(key), std::end(key)), E, [&] {
     backward_body(E, rv, rvstride, src, vsrc, vstride, band, grad, gstride, w, rvdiv, it_min, wts, tw);
   });
   const int iters = g_.nt - it_min;

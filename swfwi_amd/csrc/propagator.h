@@ -115,6 +115,11 @@ class Propagator {
   void set_tile_x(int tx);
   void set_tuning(bool on) { tune_ = on; }
   int tile_choice_cached(int kind, int E) const;
+  // Microseconds per launch of `reps` back-to-back step kernels (kind 0
+  // forward, 1 backward) of tile width kTileWidths[width] on E scratch
+  // wavefields of model rv (entry e uses rv + (e / rvdiv) * rvstride).
+  float time_steps(int kind, int E, const float* rv, long long rvstride, int rvdiv, Fields& w, int width, int reps,
+                   bool clear);
   bool graphs() const { return graphs_; }
   // Per-launch HIP events around every stencil kernel (forward = 0,
   // backward = 1), read back after each time loop into kernel_ms / kernel_n.
@@ -138,7 +143,8 @@ class Propagator {
   // 1 backward) and batch size: fixed override, cached choice, or a timed
   // trial of every width on scratch wavefields (the tile tuner).
   int tile_choice(int kind, int E, const float* rv, long long rvstride, int rvdiv, Fields& w, bool may_time);
-  void run_maybe_graph(const std::vector<uintptr_t>& key, const std::function<void()>& body);
+  static constexpr int kGraphMaxEntries = 8;
+  void run_maybe_graph(const std::vector<uintptr_t>& key, int E, const std::function<void()>& body);
   template <typename F>
   void timed_launch(int kind, F&& launch);
 

@@ -1,5 +1,7 @@
 #include "solver.h"
 
+#include "linesearch.h"
+
 #include <algorithm>
 #include <cfloat>
 #include <chrono>
@@ -24,8 +26,6 @@ T read_scalar(const T* dptr, hipStream_t s) {
   SWFWI_HIP(hipStreamSynchronize(s));
   return v;
 }
-
-inline float minf_std(float a, float b) { return (b < a) ? b : a; }  // std::min
 
 }  // namespace
 
@@ -229,18 +229,6 @@ float Inversion::max_alpha3() {
   return 2 * a2;
 }
 
-void Inversion::init_alpha23(float maxAlpha3, float& a2, float& a3) {
-  // initAlpha23 (updatesteplenop.cpp:273-285)
-  const float minAlpha = 1.0E-7, resetAlpha = 1.0E-4;
-  if (!preserved_init_) {
-    preserved_init_ = true;
-    preserved_alpha_ = maxAlpha3;
-  }
-  a3 = preserved_alpha_;
-  a3 = a3 < minAlpha ? resetAlpha : a3;
-  a2 = a3 * 0.5;
-}
-
 std::vector<float> Inversion::calobjvals(const std::vector<float>& steps) {
   // UpdateSteplenOp::calobjval (updatesteplenop.cpp:59-86) for several trial
   // step lengths in one batched launch sequence.
@@ -260,88 +248,6 @@ std::vector<float> Inversion::calobjvals(const std::vector<float>& steps) {
   std::vector<float> o = s_.residual(s_.encobs.get(), 0, s_.dcal.get(), s_.res.get(), E);
   st_.ntrial += E;
   return o;
-}
-
-float Inversion::calobjval(float steplen) { return calobjvals({steplen})[0]; }
-
-namespace {
-// begin() of the parabolicLessComp-ordered set (updatesteplenop.cpp:19-21,107)
-struct Best {
-  float a = 0, o = 0;
-  bool any = false;
-  void insert(float na, float no) {
-    if (!any || (no - o) < 1e-10) { a = na; o = no; any = true; }
-  }
-};
-}  // namespace
-
-int Inversion::refine_alpha(float obj_val1, float maxAlpha3, float& _alpha2, float& _obj_val2, float& _alpha3,
-                            float& _obj_val3) {
-  // refineAlpha (updatesteplenop.cpp:88-214), as written, including the
-  // fallbacks that evaluate the local alpha3/alpha2.
-  float alpha3 = _alpha3, alpha2 = _alpha2;
-  float obj_val2, obj_val3 = 0;
-  {
-    std::vector<float> o = calobjvals({alpha2, alpha3});
-    obj_val2 = o[0];
-    obj_val3 = o[1];
-  }
-  Best tuned;
-  tuned.insert(alpha2, obj_val2);
-  int iter = 0;
-  for (; iter < p_.nita && obj_val2 > obj_val1; iter++) {
-    alpha3 = alpha2;
-    obj_val3 = obj_val2;
-    alpha2 /= 2;
-    obj_val2 = calobjval(alpha2);
-    tuned.insert(alpha2, obj_val2);
-  }
-  if (obj_val2 > obj_val1) {
-    _alpha2 = tuned.a;
-    _obj_val2 = tuned.o;
-    _alpha3 = minf_std(_alpha2 * 2, maxAlpha3);
-    _obj_val3 = calobjval(alpha3);
-    return 0;
-  }
-  const float alpha1 = 0;
-  float linearFitAlph3 = (obj_val2 - obj_val1) / (alpha2 - alpha1) * (alpha3 - alpha1) + obj_val1;
-  tuned = Best();
-  tuned.insert(alpha3, obj_val3);
-  while (obj_val3 < linearFitAlph3 && obj_val3 < obj_val1 && alpha3 < maxAlpha3) {
-    alpha2 = alpha3;
-    obj_val2 = obj_val3;
-    alpha3 = minf_std(alpha3 * 2, maxAlpha3);
-    obj_val3 = calobjval(alpha3);
-    tuned.insert(alpha3, obj_val3);
-  }
-  if (alpha3 > maxAlpha3 + 0.1) {
-    _alpha3 = tuned.a;
-    _obj_val3 = tuned.o;
-    _alpha2 = _alpha3 / 2;
-    _obj_val2 = calobjval(alpha2);
-    return 0;
-  }
-  _alpha2 = alpha2;
-  _alpha3 = alpha3;
-  _obj_val2 = obj_val2;
-  _obj_val3 = obj_val3;
-  return 1;
-}
-
-// calParabolaVertex + parabolaVertex (parabola-vertex.cpp:12-33), FP64
-static void parabola_vertex(float x1, float y1, float x2, float y2, float x3, float y3, float& xv, float& yv) {
-  double k2 = (y3 - y2) / (x3 - x2);
-  double k1 = (y2 - y1) / (x2 - x1);
-  double denom = (x1 - x2) * (x1 - x3) * (x2 - x3);
-  double A = (x3 * (y2 - y1) + x2 * (y1 - y3) + x1 * (y3 - y2)) / denom;
-  double B = (x3 * x3 * (y1 - y2) + x2 * x2 * (y3 - y1) + x1 * x1 * (y2 - y3)) / denom;
-  double C = (x2 * x3 * (x2 - x3) * y1 + x3 * x1 * (x3 - x1) * y2 + x1 * x2 * (x1 - x2) * y3) / denom;
-  xv = -B / (2 * A);
-  yv = C - B * B / (4 * A);
-  if (std::abs(k2 - k1) < 0.001 * (std::max(std::abs(k2), std::abs(k1))) || std::isnan(-xv)) {
-    xv = x3;
-    yv = y3;
-  }
 }
 
 void Inversion::apply_update(float steplen) {
@@ -392,21 +298,23 @@ void Inversion::ess_epoch(int iter) {
   update_grad(g1_.get(), iter);
   SWFWI_HIP(hipEventRecord(e1, s));
 
-  // UpdateSteplenOp::calsteplen (updatesteplenop.cpp:216-265)
+  // UpdateSteplenOp::calsteplen (updatesteplenop.cpp:216-265); the first two
+  // trials run as one batch of two models
   const float max3 = max_alpha3();
-  float alpha1 = 0, alpha2, alpha3;
-  init_alpha23(max3, alpha2, alpha3);
-  float obj_val2, obj_val3;
-  const int toParabolic = refine_alpha(obj1, max3, alpha2, obj_val2, alpha3, obj_val3);
-  float alpha4, obj_val4;
-  if (toParabolic) {
-    parabola_vertex(alpha1, obj1, alpha2, obj_val2, alpha3, obj_val3, alpha4, obj_val4);
-    if (alpha4 > max3) alpha4 = max3;
-  } else {
-    alpha4 = alpha3;
-    obj_val4 = obj_val3;
+  float alpha2, alpha3;
+  preserved_.init_alpha23(max3, alpha2, alpha3);
+  RefineAlpha ra(obj1, max3, alpha2, alpha3, p_.nita);
+  while (!ra.done()) {
+    const std::vector<float> o = calobjvals(ra.pending());
+    ra.feed(o.data());
   }
-  preserved_alpha_ = alpha4;
+  float alpha4, obj_val4;
+  final_step(ra, obj1, max3, alpha4, obj_val4);
+  const int toParabolic = ra.parabolic;
+  alpha2 = ra.ra2;
+  alpha3 = ra.ra3;
+  const float obj_val2 = ra.ro2, obj_val3 = ra.ro3;
+  preserved_.alpha = alpha4;
   apply_update(alpha4);
   SWFWI_HIP(hipEventRecord(e2, s));
   SWFWI_HIP(hipEventSynchronize(e2));
@@ -478,7 +386,7 @@ void Inversion::fwi_epoch(int iter) {
   // every shot in one batch, per-shot objectives summed, parabola
   const float max3 = max_alpha3();
   float alpha2, alpha3;
-  init_alpha23(max3, alpha2, alpha3);
+  preserved_.init_alpha23(max3, alpha2, alpha3);
   const float steps[2] = {alpha2, alpha3};
   s_.trial_vel.resize((size_t)2 * n);
   s_.trial_rv.resize((size_t)2 * n);
@@ -499,7 +407,7 @@ void Inversion::fwi_epoch(int iter) {
   float alpha4, obj4;
   parabola_vertex(0, obj1, alpha2, obj[0], alpha3, obj[1], alpha4, obj4);
   if (alpha4 > max3) alpha4 = max3;
-  preserved_alpha_ = alpha4;
+  preserved_.alpha = alpha4;
   apply_update(alpha4);
   SWFWI_HIP(hipEventRecord(e2, s));
   SWFWI_HIP(hipEventSynchronize(e2));

@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <vector>
 
+#include "linesearch.h"
 #include "propagator.h"
 
 namespace swfwi {
@@ -116,10 +117,9 @@ class Inversion {
   CodeStream& codes() { return codes_; }
 
  private:
-  float calobjval(float steplen);                   // ESS trial (updatesteplenop.cpp:59-86)
+  // UpdateSteplenOp::calobjval (updatesteplenop.cpp:59-86) for several trial
+  // step lengths in one batched forward run
   std::vector<float> calobjvals(const std::vector<float>& steps);
-  int refine_alpha(float obj1, float maxAlpha3, float& a2, float& o2, float& a3, float& o3);
-  void init_alpha23(float maxAlpha3, float& a2, float& a3);
   float max_alpha3();
   void update_grad(const float* g1, int iter);
   void apply_update(float steplen);
@@ -129,8 +129,7 @@ class Inversion {
   float tmin_ = 0, tmax_ = 0;
   CodeStream codes_{1};
   DBuf<float> dobs_, g0_, dir_, g1_;
-  float preserved_alpha_ = 0;
-  bool preserved_init_ = false;
+  PreservedAlpha preserved_;
   EpochStats st_;
 };
 

@@ -4,6 +4,7 @@
     python -m swfwi_amd.profile_workloads shots     # 60-shot batched forward, 301x921, nt=400
     python -m swfwi_amd.profile_workloads ess       # ESS epochs, 301x921, nt=1000 (--no-graphs for counters)
     python -m swfwi_amd.profile_workloads enkf      # EnKF analyses, 301x921, N members
+    python -m swfwi_amd.profile_workloads sweep     # step-kernel time per tile width and batch size
 """
 import argparse
 import sys
@@ -32,6 +33,25 @@ def run_shots(nt):
     dt = time.perf_counter() - t0
     print(f"shots: {case.ns} x {nt} steps in {dt * 1e3:.1f} ms "
           f"({case.ns * nt * case.update_points / dt / 1e9:.1f} Gpts/s incl. transfer)", flush=True)
+
+
+def run_sweep(entries, reps):
+    """Per-launch time of every step kernel width over batch sizes on the
+    301x921 grid (back-to-back launches on scratch wavefields)."""
+    case = workloads.ess_301x921(nt=8)
+    vtrue, _ = workloads.models(case)
+    s = capi.Solver(case.nx, case.nz, case.nb, case.dt, case.dx, case.fm, case.nt, case.sx, case.sz, case.gx,
+                    case.gz)
+    s.bind_velocity(capi.expand_domain(vtrue, case.nb, case.dx, case.dt))
+    pts = case.update_points
+    for E in entries:
+        for kind, byts in ((0, 16.0), (1, 40.0)):
+            row = []
+            for tx in (8, 16, 32):
+                s.time_steps(kind, E, tx, 4)  # warm
+                us = s.time_steps(kind, E, tx, reps)
+                row.append(f"tx{tx} {us:8.1f} us {E * pts * byts / (us * 1e3):7.0f} GB/s")
+            print(f"{'fwd' if kind == 0 else 'bwd'} E={E:4d}: " + " | ".join(row), flush=True)
 
 
 def run_ess(nt, epochs, graphs=True):
@@ -91,7 +111,9 @@ def run_enkf(nt, members):
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["copy", "shots", "ess", "enkf"])
+    ap.add_argument("mode", choices=["copy", "shots", "ess", "enkf", "sweep"])
+    ap.add_argument("--entries", default="1,2,3,4,8,16,60,100,200")
+    ap.add_argument("--reps", type=int, default=100)
     ap.add_argument("--members", type=int, default=100)
     ap.add_argument("--n", type=int, default=1 << 28)
     ap.add_argument("--nt", type=int, default=None)
@@ -105,6 +127,8 @@ def main(argv=None):
         run_copy(a.n)
     elif a.mode == "shots":
         run_shots(a.nt or 400)
+    elif a.mode == "sweep":
+        run_sweep([int(x) for x in a.entries.split(",")], a.reps)
     elif a.mode == "enkf":
         run_enkf(a.nt or 4000, a.members)
     else:

@@ -1,0 +1,87 @@
+"""The ensemble-Kalman FWI driver (enfwi-damp): every member's batched ESS
+epoch against the oracle's per-member EssFwiFramework epochs, and the
+ensemble-mean objective against the oracle's calobj."""
+import numpy as np
+import pytest
+
+from oracle import enkf_oracle as eo
+from swfwi_amd import capi, workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
+
+
+@pytest.fixture(scope="module")
+def setup():
+    if capi.device_count() < 1:
+        pytest.skip("no HIP device")
+    from scipy.ndimage import gaussian_filter
+    case = workloads.small_case(nx=48, nz=32, nb=8, nt=360, ns=3)
+    vtrue, v0 = workloads.models(case)
+    rng = np.random.default_rng(21)
+    N = 3
+    mem = []
+    for _ in range(N):
+        f = gaussian_filter(rng.standard_normal(v0.shape), 3.0)
+        f = f / np.abs(f).max() * 0.03
+        mem.append(capi.expand_domain((v0 * (1 + f)).astype(np.float32), case.nb, case.dx, case.dt))
+    wlt = capi.ricker(case.nt, case.fm, case.dt, 1.0)
+    s = capi.Solver(case.nx, case.nz, case.nb, case.dt, case.dx, case.fm, case.nt, case.sx, case.sz, case.gx,
+                    case.gz)
+    s.bind_velocity(capi.expand_domain(vtrue, case.nb, case.dx, case.dt))
+    dobs = s.forward_shots(wlt)
+    return case, s, np.stack(mem), wlt, dobs
+
+
+def test_ensemble_epochs_match_per_member_oracle(oracle, setup):
+    from swfwi_amd.enkf import Enkf, EnsembleFwi
+    case, s, members, wlt, dobs = setup
+    N = members.shape[0]
+    e = Enkf(s, wlt, dobs, 0.2, N)
+    e.set_members(members)
+    fwi = EnsembleFwi(e, 1400.0, 3600.0, 3, 200.0)
+    g = oracle.Geom(case.nx, case.nz, case.nb, case.dt, case.dx, case.fm, case.nt, case.sx, case.sz, case.gx,
+                    case.gz)
+    oracle.set_sum_mode(1)  # FP64 accumulation, as the GPU reductions
+    try:
+        ref = [oracle.Ess(g, members[i].copy(), wlt, dobs, 1400.0, 3600.0, 3, 200.0) for i in range(N)]
+        for it in range(2):
+            print(f"ensemble epoch {it} N={N}", flush=True)
+            fwi.epoch(it)
+            st = fwi.stats()
+            got_vel = e.members()
+            got_grad = fwi.gradients()
+            for i in range(N):
+                ref[i].epoch(it)
+                a, b = st["members"][i], ref[i].scalars()
+                for k in ("obj1", "steplen", "alpha2", "alpha3", "obj2", "obj3", "updateobj", "ntrial", "parabolic"):
+                    assert abs(a[k] - b[k]) <= 1e-6 * max(abs(b[k]), 1e-30), (it, i, k, a[k], b[k])
+                assert rel_l2(got_grad[i], ref[i].grad()) < 1e-6, (it, i)
+                assert rel_l2(got_vel[i], ref[i].vel) < 1e-6, (it, i)
+            # one batched trial run per lockstep round; each member's trials counted once
+            assert st["trials"] == sum(int(m["ntrial"]) for m in st["members"])
+            assert st["rounds"] >= 1
+        # the members took different steps (they are different models)
+        assert len({round(m["steplen"], 12) for m in st["members"]}) > 1
+
+        # calobj of the ensemble mean: first codes of seed 1, muted with the mean
+        obj, mean = fwi.mean_objective(want_mean=True)
+        want_mean = eo.member_mean(e.members())
+        assert np.array_equal(mean, want_mean)
+        code = oracle.gen_codes(1, case.ns)
+        enc = oracle.encode_source(code, wlt)
+        encobs = oracle.encode_obs(code, dobs)
+        dcal = g.forward_modeling(want_mean, enc, shot=-1)
+        g.remove_direct_arrival(want_mean, encobs)
+        g.remove_direct_arrival(want_mean, dcal)
+        want = oracle.cal_objective(encobs - dcal)
+        assert abs(obj - want) <= 1e-6 * abs(want), (obj, want)
+    finally:
+        oracle.set_sum_mode(0)
+    fwi.close()
+    e.close()
