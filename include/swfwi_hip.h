@@ -44,6 +44,9 @@ int swfwi_device_count(int *count);
  * `iters` launches; *gbs = (read + write bytes) / time.  No counterpart in
  * the reference: it calibrates the roofline and the HBM byte counters. */
 int swfwi_copy_bandwidth(long long n, int iters, double *gbs);
+/* Microseconds per launch of `reps` back-to-back near-empty kernels of
+ * `blocks` blocks, launched on a stream or replayed from a captured graph. */
+int swfwi_launch_latency(int blocks, int reps, int graph, double *us);
 
 /* ---- drop-in per-step kernels (host pointers, in place prev <- next) ----- */
 /* Replaces fd4t10s_damp_zjh_2d_vtrans (fd4t10s-damp-zjh.h:11,
@@ -113,8 +116,12 @@ int swfwi_solver_get_tile_x(swfwi_solver h, int kind, int entries, int *tile_x);
  * counts of the forward [0] and backward [1] step kernels. */
 int swfwi_solver_set_timing(swfwi_solver h, int on);
 int swfwi_solver_kernel_times(swfwi_solver h, double *ms, long long *launches);
+/* Forward time loops advance 1 step per launch (the default) or 2: the
+ * second step of a tile computed from the first without leaving the block,
+ * at the cost of recomputing a 6-cell ring of the first. */
+int swfwi_solver_set_steps_per_launch(swfwi_solver h, int n);
 /* Microseconds per launch of `reps` back-to-back step kernels (kind 0 forward,
- * 1 backward) of width tile_x (8, 16, 32) over `entries` scratch wavefields of
+ * 1 backward, 2 two-step forward) of width tile_x (8, 16, 32) over `entries` scratch wavefields of
  * the bound model: the tile tuner's measurement, for sweeps.  Uses the
  * handle's workspace, so a following calgradient needs a new forward run. */
 int swfwi_solver_time_steps(swfwi_solver h, int kind, int entries, int tile_x, int reps, float *us);

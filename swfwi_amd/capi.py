@@ -25,6 +25,7 @@ SIGNATURES = {
     "swfwi_version": (C.c_char_p, []),
     "swfwi_device_count": (_I, [C.POINTER(_I)]),
     "swfwi_copy_bandwidth": (_I, [C.c_longlong, _I, C.POINTER(C.c_double)]),
+    "swfwi_launch_latency": (_I, [_I, _I, _I, C.POINTER(C.c_double)]),
     "fd4t10s_damp_zjh_2d_vtrans": (None, [_f32, _f32, _f32, _f32, _I, _I, _I]),
     "fd4t10s_zjh_2d_vtrans": (None, [_f32, _f32, _f32, _f32, _I, _I]),
     "swfwi_step_damped": (_I, [_f32, _f32, _f32, _f32, _I, _I, _I, _I]),
@@ -48,6 +49,7 @@ SIGNATURES = {
     "swfwi_solver_set_timing": (_I, [_VP, _I]),
     "swfwi_solver_kernel_times": (_I, [_VP, _f64, np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")]),
     "swfwi_solver_time_steps": (_I, [_VP, _I, _I, _I, _I, C.POINTER(_F)]),
+    "swfwi_solver_set_steps_per_launch": (_I, [_VP, _I]),
     "swfwi_enkf_create": (_I, [C.POINTER(_VP), _VP, _f32, _f32, _F, _I]),
     "swfwi_enkf_destroy": (_I, [_VP]),
     "swfwi_enkf_set_members": (_I, [_VP, _f32]),
@@ -111,6 +113,13 @@ def copy_bandwidth(n=1 << 28, iters=20):
     g = C.c_double(0)
     check(load().swfwi_copy_bandwidth(n, iters, C.byref(g)))
     return g.value
+
+
+def launch_latency(blocks=738, reps=200, graph=False):
+    """Microseconds per back-to-back launch of a near-empty kernel."""
+    us = C.c_double(0)
+    check(load().swfwi_launch_latency(int(blocks), int(reps), int(bool(graph)), C.byref(us)))
+    return us.value
 
 
 def _f(a):
@@ -222,6 +231,9 @@ class Solver:
         t = _I(0)
         check(load().swfwi_solver_get_tile_x(self.h, kind, entries, C.byref(t)))
         return t.value
+
+    def set_steps_per_launch(self, n):
+        check(load().swfwi_solver_set_steps_per_launch(self.h, int(n)))
 
     def time_steps(self, kind, entries, tile_x, reps=100):
         """Microseconds per launch of back-to-back step kernels on scratch fields."""

@@ -147,6 +147,15 @@ int swfwi_copy_bandwidth(long long n, int iters, double* gbs) {
   });
 }
 
+int swfwi_launch_latency(int blocks, int reps, int graph, double* us) {
+  return guarded([&] {
+    NEED(us);
+    if (blocks < 1 || reps < 1) fail(kErrArg, "blocks and reps must be >= 1");
+    Stream st;
+    *us = op_launch_latency(blocks, reps, graph != 0, st.get());
+  });
+}
+
 void fd4t10s_damp_zjh_2d_vtrans(float* prev_wave, const float* curr_wave, const float* vel, float* u2, int nx, int nz,
                                 int nb) {
   if (run_step(prev_wave, curr_wave, vel, u2, nx, nz, nb, true, kExact) != SWFWI_OK) {
@@ -336,7 +345,7 @@ int swfwi_solver_time_steps(swfwi_solver h, int kind, int entries, int tile_x, i
   return guarded([&] {
     NEED(h);
     NEED(us);
-    if (kind != 0 && kind != 1) fail(kErrArg, "kind must be 0 (forward) or 1 (backward)");
+    if (kind < 0 || kind > 2) fail(kErrArg, "kind must be 0 (forward), 1 (backward) or 2 (two-step forward)");
     if (entries < 1 || reps < 1) fail(kErrArg, "entries and reps must be >= 1");
     int width = -1;
     for (int i = 0; i < 3; i++)
@@ -346,6 +355,13 @@ int swfwi_solver_time_steps(swfwi_solver h, int kind, int entries, int tile_x, i
     if (!s.rv()) fail(kErrState, "bind the velocity first");
     *us = s.prop().time_steps(kind, entries, s.rv(), 0, 1, s.w, width, reps, true);
     s.w.s_cur = s.w.s_nxt = nullptr;  // the workspace no longer holds a forward state
+  });
+}
+
+int swfwi_solver_set_steps_per_launch(swfwi_solver h, int n) {
+  return guarded([&] {
+    NEED(h);
+    h->s->prop().set_steps_per_launch(n);
   });
 }
 

@@ -71,6 +71,10 @@ void op_reciprocal(const float* vel, float* rv, long long n, hipStream_t s);
 // float4 grid-stride copy of n floats, `iters` back-to-back launches.
 double op_copy_bandwidth(long long n, int iters, hipStream_t s);
 
+// Microseconds per launch of `reps` back-to-back kernels of `blocks` blocks
+// that each write one float (from a captured graph when `graph`).
+double op_launch_latency(int blocks, int reps, bool graph, hipStream_t s);
+
 // Add `src` (+e*stride) into dst (+e*stride) elementwise (FP32).
 void op_axpy1(const float* src, float* dst, long long n, hipStream_t s);
 

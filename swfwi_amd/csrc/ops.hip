@@ -447,6 +447,46 @@ double op_copy_bandwidth(long long n, int iters, hipStream_t s) {
   return 2.0 * 16.0 * (double)n4 * iters / (ms * 1e-3) / 1e9;
 }
 
+namespace {
+__global__ void k_touch(float* p) {
+  if (threadIdx.x == 0) p[blockIdx.x] = 1.0f;
+}
+}  // namespace
+
+double op_launch_latency(int blocks, int reps, bool graph, hipStream_t s) {
+  DBuf<float> buf((size_t)blocks);
+  hipEvent_t e0, e1;
+  SWFWI_HIP(hipEventCreate(&e0));
+  SWFWI_HIP(hipEventCreate(&e1));
+  hipGraphExec_t exec = nullptr;
+  if (graph) {
+    hipGraph_t g = nullptr;
+    SWFWI_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < reps; i++) hipLaunchKernelGGL(k_touch, dim3(blocks), dim3(kB), 0, s, buf.get());
+    SWFWI_HIP(hipStreamEndCapture(s, &g));
+    SWFWI_HIP(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+    SWFWI_HIP(hipGraphLaunch(exec, s));  // warm
+  } else {
+    for (int i = 0; i < 8; i++) hipLaunchKernelGGL(k_touch, dim3(blocks), dim3(kB), 0, s, buf.get());
+  }
+  SWFWI_HIP(hipEventRecord(e0, s));
+  if (graph) {
+    SWFWI_HIP(hipGraphLaunch(exec, s));
+  } else {
+    for (int i = 0; i < reps; i++) hipLaunchKernelGGL(k_touch, dim3(blocks), dim3(kB), 0, s, buf.get());
+  }
+  SWFWI_HIP(hipEventRecord(e1, s));
+  SWFWI_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  SWFWI_HIP(hipEventElapsedTime(&ms, e0, e1));
+  if (exec) (void)hipGraphExecDestroy(exec);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  SWFWI_HIP(hipGetLastError());
+  return 1e3 * ms / reps;
+}
+
 void op_gram_f64(const double* A, const double* B, int N, long long m, double* C, hipStream_t s) {
   const int nt = (N + kGT - 1) / kGT;
   long long nchunk = 64;

@@ -51,29 +51,41 @@ void build_damping(int nx, int nz, int nb, DampTables& t, hipStream_t s) {
   t.zdamp = nz - nb;
 }
 
-void build_points(const std::vector<long long>& idx, int ld, int ntz, int ntx, int tile_x, DevPoints& out,
+void build_points(const std::vector<long long>& idx, int ld, int ntz, int ntx, int tile_x, int halo, DevPoints& out,
                   hipStream_t s) {
+  // tile t = tx*ntz + tz covers outputs [kHalo + tx*tile_x, +tile_x) x
+  // [kHalo + tz*kTZ, +kTZ); a point is listed under every tile whose cover
+  // grown by `halo` cells holds it, in ascending point id
   const int ntiles = ntz * ntx;
   const int n = (int)idx.size();
-  std::vector<int> tile(n);
+  std::vector<std::vector<int>> lists(ntiles);
   for (int p = 0; p < n; p++) {
     const int ix = (int)(idx[p] / ld), iz = (int)(idx[p] % ld);
     const int tz = (iz - kHalo) / kTZ, tx = (ix - kHalo) / tile_x;
     if (iz < kHalo || ix < kHalo || tz >= ntz || tx >= ntx) fail(kErrArg, "point outside the stencil update region");
-    tile[p] = tx * ntz + tz;
+    const int z_lo = std::max(0, (iz - kHalo - halo) / kTZ - 1), z_hi = std::min(ntz - 1, tz + 1 + halo / kTZ);
+    const int x_lo = std::max(0, (ix - kHalo - halo) / tile_x - 1), x_hi = std::min(ntx - 1, tx + 1 + halo / tile_x);
+    for (int x = x_lo; x <= x_hi; x++)
+      for (int z = z_lo; z <= z_hi; z++) {
+        const int gx0 = kHalo + x * tile_x, gz0 = kHalo + z * kTZ;
+        if (ix >= gx0 - halo && ix < gx0 + tile_x + halo && iz >= gz0 - halo && iz < gz0 + kTZ + halo)
+          lists[(size_t)x * ntz + z].push_back(p);
+      }
   }
-  std::vector<int> order(n);
-  std::iota(order.begin(), order.end(), 0);
-  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return tile[a] < tile[b]; });
-  std::vector<int> ptr(ntiles + 1, 0);
-  for (int p = 0; p < n; p++) ptr[tile[p] + 1]++;
-  for (int t = 0; t < ntiles; t++) ptr[t + 1] += ptr[t];
+  std::vector<int> ptr(ntiles + 1, 0), items;
+  for (int t = 0; t < ntiles; t++) {
+    ptr[t + 1] = ptr[t] + (int)lists[t].size();
+    items.insert(items.end(), lists[t].begin(), lists[t].end());
+  }
   std::vector<int> idx32(n);
   for (int p = 0; p < n; p++) idx32[p] = (int)idx[p];
   out.tile_ptr.upload(ptr, s);
-  out.items.upload(order, s);
+  out.items.upload(items, s);
   out.idx.upload(idx32, s);
   out.n = n;
+  out.max_per_tile = 0;
+  for (int t = 0; t < ntiles; t++) out.max_per_tile = std::max(out.max_per_tile, ptr[t + 1] - ptr[t]);
+  SWFWI_HIP(hipStreamSynchronize(s));  // the host vectors end here
 }
 
 void Fields::ensure(int e, long long npts, bool receiver) {
@@ -106,8 +118,10 @@ Propagator::Propagator(const Geometry& g, int precision) : g_(g), prec_(precisio
   }
   for (int w = 0; w < 3; w++) {
     const int ntx = tile_count_x(g_.nxpad, kTileWidths[w]);
-    build_points(si, g_.ld, g_.ntz, ntx, kTileWidths[w], src_[w], s);
-    build_points(gi, g_.ld, g_.ntz, ntx, kTileWidths[w], rec_[w], s);
+    build_points(si, g_.ld, g_.ntz, ntx, kTileWidths[w], 0, src_[w], s);
+    build_points(gi, g_.ld, g_.ntz, ntx, kTileWidths[w], 0, rec_[w], s);
+    build_points(si, g_.ld, g_.ntz, ntx, kTileWidths[w], kHalo, srch_[w], s);
+    build_points(gi, g_.ld, g_.ntz, ntx, kTileWidths[w], kHalo, rech_[w], s);
   }
   sxp_.upload(sxp, s);
   szp_.upload(szp, s);
@@ -166,7 +180,19 @@ float Propagator::time_steps(int kind, int E, const float* rv, long long rvstrid
   SWFWI_HIP(hipEventCreate(&e1));
   SWFWI_HIP(hipEventRecord(e0, s));
   for (int r = 0; r < reps; r++) {
-    if (kind == 0) {
+    if (kind == 2) {
+      Fwd2StepArgs a;
+      a.g = grid(width);
+      a.fstride = np;
+      a.cur = (r & 1) ? w.c.get() : w.a.get();
+      a.prev = (r & 1) ? w.d.get() : w.b.get();
+      a.out1 = (r & 1) ? w.b.get() : w.d.get();
+      a.out2 = (r & 1) ? w.a.get() : w.c.get();
+      a.rv = rv;
+      a.rvstride = rvstride;
+      a.rvdiv = rvdiv;
+      launch_fwd_step2(a, E, prec_, s);
+    } else if (kind == 0) {
       FwdStepArgs a;
       a.g = grid(width);
       a.fstride = np;
@@ -214,9 +240,14 @@ int Propagator::tile_choice(int kind, int E, const float* rv, long long rvstride
     if (kv.first == key) return kv.second;
   int best = 1;
   if (!tune_ || !may_time) {
-    // heuristic: enough blocks to fill 256 CUs with the widest tile that does
+    // measured on the 301x921 grid: the narrow tile wins while the batch
+    // does not fill the chip (more, shorter blocks), the wide one once it
+    // is bandwidth-bound (less halo traffic per output)
     const long long tiles16 = (long long)g_.ntz * tile_count_x(g_.nxpad, 16) * E;
-    best = tiles16 >= 4 * 256 ? 2 : (tiles16 < 256 ? 0 : 1);
+    if (kind == 1)
+      best = tiles16 < 1000 ? 0 : 1;
+    else
+      best = tiles16 < 600 ? 0 : (tiles16 < 2000 ? 1 : 2);
   } else {
     // timed trial on scratch wavefields (the caller's workspace is free at
     // this point: forward() tunes before it zeroes and runs); three
@@ -254,16 +285,52 @@ int Propagator::xcorr_range(float dt, int nt, std::vector<float>& weights) {
   return it_min;
 }
 
+std::vector<Propagator::FwdLaunch> Propagator::plan_forward(Fields& fw) const {
+  // Buffer rotation of a forward run: b holds p(0) (the injected zero field),
+  // a holds p(-1) = 0.  A single step writes next over prev; a two-step
+  // launch writes p(t+1), p(t+2) into the free pair.
+  const int nt = g_.nt;
+  std::vector<FwdLaunch> plan;
+  float* cur = fw.b.get();
+  float* prv = fw.a.get();
+  float* f0 = fw.c.get();
+  float* f1 = fw.d.get();
+  int it = 0;
+  if (steps_per_launch_ == 2) {
+    for (; it + 1 < nt; it += 2) {
+      plan.push_back({it, 2, cur, prv, f0, f1});
+      float* oc = cur;
+      float* op = prv;
+      prv = f0;  // p(t+1)
+      cur = f1;  // p(t+2)
+      f0 = op;
+      f1 = oc;
+    }
+  }
+  for (; it < nt; it++) {
+    plan.push_back({it, 1, cur, prv, nullptr, nullptr});
+    std::swap(cur, prv);  // next was written over prev
+  }
+  // after the last launch: s_cur = p(nt-1), s_nxt = p(nt)
+  fw.s_cur = prv;
+  fw.s_nxt = cur;
+  fw.spare0 = f0;
+  fw.spare1 = f1;
+  return plan;
+}
+
 void Propagator::forward_body(int E, const float* rv, long long rvstride, const Sources& src, float* dcal,
-                              float* band, Fields& fw, int rvdiv, int w) {
+                              float* band, Fields& fw, int rvdiv, int w1, int w2) {
   hipStream_t s = stream();
   const long long np = g_.npts();
   const int nt = g_.nt, ng = g_.ng();
+  // the field frame outside the update region stays 0 in all four buffers
   launch_fill(fw.a.get(), (long long)E * np, 0.f, s);
   launch_fill(fw.b.get(), (long long)E * np, 0.f, s);
+  launch_fill(fw.c.get(), (long long)E * np, 0.f, s);
+  launch_fill(fw.d.get(), (long long)E * np, 0.f, s);
 
   Inject inj;
-  inj.pts = src_[w].view();
   inj.amp = src.amp;
   inj.ld = src.ld;
   inj.estride = src.estride;
@@ -272,44 +339,79 @@ void Propagator::forward_body(int E, const float* rv, long long rvstride, const 
   inj.hi = src.hi;
 
   // step 0 injects into the initially zero cur (addSource before stepForward)
+  inj.pts = src_[w1].view();
   inj.t = 0;
   launch_point_inject(fw.b.get(), np, inj, g_.ns(), E, s);
 
   FwdStepArgs a;
-  a.g = grid(w);
+  a.g = grid(w1);
   a.fstride = np;
   a.rv = rv;
   a.rvstride = rvstride;
   a.rvdiv = rvdiv;
   a.inj = inj;
-  a.rec.pts = rec_[w].view();
+  a.rec.pts = rec_[w1].view();
   a.rec.estride = (long long)nt * ng;
   a.band_estride = (long long)nt * g_.bsize;
-  for (int it = 0; it < nt; it++) {
-    float* cur = (it % 2 == 0) ? fw.b.get() : fw.a.get();
-    float* prv = (it % 2 == 0) ? fw.a.get() : fw.b.get();
-    a.cur = cur;
-    a.prev = prv;
-    a.inj.t = it + 1 < nt ? it + 1 : -1;
-    a.rec.out = dcal ? dcal + (long long)it * ng : nullptr;
-    a.band = band ? band + (long long)it * g_.bsize : nullptr;
-    timed_launch(0, [&] { launch_fwd_step(a, E, true, prec_, s); });
+
+  Fwd2StepArgs b2;
+  b2.g = grid(w2);
+  b2.fstride = np;
+  b2.rv = rv;
+  b2.rvstride = rvstride;
+  b2.rvdiv = rvdiv;
+  b2.inj1 = inj;
+  b2.inj1.pts = srch_[w2].view();
+  b2.inj2 = inj;
+  b2.inj2.pts = src_[w2].view();
+  b2.rec.pts = rec_[w2].view();
+  b2.rec.estride = (long long)nt * ng;
+  b2.rec_step = ng;
+  b2.band_step = g_.bsize;
+  b2.band_estride = (long long)nt * g_.bsize;
+
+  for (const FwdLaunch& l : plan_forward(fw)) {
+    const int it = l.it;
+    if (l.steps == 2) {
+      b2.cur = l.cur;
+      b2.prev = l.prev;
+      b2.out1 = l.out1;
+      b2.out2 = l.out2;
+      b2.inj1.t = it + 1;
+      b2.inj2.t = it + 2 < nt ? it + 2 : -1;
+      b2.rec.out = dcal ? dcal + (long long)it * ng : nullptr;
+      b2.band = band ? band + (long long)it * g_.bsize : nullptr;
+      timed_launch(0, [&] { launch_fwd_step2(b2, E, prec_, s); });
+    } else {
+      a.cur = l.cur;
+      a.prev = l.prev;
+      a.inj.t = it + 1 < nt ? it + 1 : -1;
+      a.rec.out = dcal ? dcal + (long long)it * ng : nullptr;
+      a.band = band ? band + (long long)it * g_.bsize : nullptr;
+      timed_launch(0, [&] { launch_fwd_step(a, E, true, prec_, s); });
+    }
   }
-  fw.s_cur = ((nt - 1) % 2 == 0) ? fw.b.get() : fw.a.get();
-  fw.s_nxt = ((nt - 1) % 2 == 0) ? fw.a.get() : fw.b.get();
   SWFWI_HIP(hipGetLastError());
+}
+
+void Propagator::set_steps_per_launch(int n) {
+  if (n != 1 && n != 2) fail(kErrArg, "steps per launch must be 1 or 2");
+  steps_per_launch_ = n;
 }
 
 void Propagator::forward(int E, const float* rv, long long rvstride, const Sources& src, float* dcal, float* band,
                          Fields& w, int rvdiv) {
   const long long np = g_.npts();
-  w.ensure(E, np, false);
-  // tune both step kernels for this batch now, while the workspace is free
+  w.ensure(E, np, true);
+  // tile widths for this batch, chosen now while the workspace is free
   // (backward() runs on the state this forward leaves behind)
-  const int tw = tile_choice(0, E, rv, rvstride, rvdiv, w, true);
+  const int w1 = tile_choice(0, E, rv, rvstride, rvdiv, w, true);
+  const int w2 = steps_per_launch_ == 2 ? tile_choice(2, E, rv, rvstride, rvdiv, w, true) : w1;
   tile_choice(1, E, rv, rvstride, rvdiv, w, true);
   const uintptr_t key[] = {1,
-                           (uintptr_t)tw,
+                           (uintptr_t)w1,
+                           (uintptr_t)w2,
+                           (uintptr_t)steps_per_launch_,
                            (uintptr_t)E,
                            (uintptr_t)rv,
                            (uintptr_t)rvstride,
@@ -323,13 +425,14 @@ void Propagator::forward(int E, const float* rv, long long rvstride, const Sourc
                            (uintptr_t)band,
                            (uintptr_t)w.a.get(),
                            (uintptr_t)w.b.get(),
+                           (uintptr_t)w.c.get(),
+                           (uintptr_t)w.d.get(),
                            (uintptr_t)prec_,
                            (uintptr_t)timing_};
   run_maybe_graph(std::vector<uintptr_t>(std::begin(key), std::end(key)), E,
-                  [&] { forward_body(E, rv, rvstride, src, dcal, band, w, rvdiv, tw); });
-  w.s_cur = ((g_.nt - 1) % 2 == 0) ? w.b.get() : w.a.get();
-  w.s_nxt = ((g_.nt - 1) % 2 == 0) ? w.a.get() : w.b.get();
-  fwd_launches += g_.nt;
+                  [&] { forward_body(E, rv, rvstride, src, dcal, band, w, rvdiv, w1, w2); });
+  const std::vector<FwdLaunch> plan = plan_forward(w);  // sets the final state
+  fwd_launches += (long long)plan.size();
   fwd_point_updates += (double)E * (g_.nxpad - 2 * kHalo) * (g_.nzpad - 2 * kHalo) * g_.nt;
 }
 
@@ -462,8 +565,8 @@ int Propagator::backward(int E, const float* rv, long long rvstride, const Sourc
                            (uintptr_t)gstride,
                            (uintptr_t)w.s_cur,
                            (uintptr_t)w.s_nxt,
-                           (uintptr_t)w.c.get(),
-                           (uintptr_t)w.d.get(),
+                           (uintptr_t)w.spare0,
+                           (uintptr_t)w.spare1,
                            (uintptr_t)prec_,
                            (uintptr_t)timing_};
   run_maybe_graph(std::vector<uintptr_t>(std::begin(key), std::end(key)), E, [&] {
@@ -483,8 +586,8 @@ void Propagator::backward_body(int E, const float* rv, long long rvstride, const
   const int nt = g_.nt, ng = g_.ng();
   float* s_cur = fw.s_cur;
   float* s_prv = fw.s_nxt;
-  float* g_cur = fw.c.get();
-  float* g_prv = fw.d.get();
+  float* g_cur = fw.spare0;
+  float* g_prv = fw.spare1;
   launch_fill(g_cur, (long long)E * np, 0.f, s);
   launch_fill(g_prv, (long long)E * np, 0.f, s);
 

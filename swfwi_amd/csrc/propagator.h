@@ -39,6 +39,7 @@ struct Geometry {
 struct DevPoints {
   DBuf<int> tile_ptr, items, idx;
   int n = 0;
+  int max_per_tile = 0;
   PointList view() const { return PointList{tile_ptr.get(), items.get(), idx.get()}; }
 };
 
@@ -48,15 +49,19 @@ struct DampTables {
   int zdamp = 0;
 };
 void build_damping(int nx, int nz, int nb, DampTables& t, hipStream_t s);
-void build_points(const std::vector<long long>& idx, int ld, int ntz, int ntx, int tile_x, DevPoints& out,
+// Bucket point indices (ix*ld + iz) by the tiles of width tile_x whose
+// output region, grown by `halo` cells, contains them.
+void build_points(const std::vector<long long>& idx, int ld, int ntz, int ntx, int tile_x, int halo, DevPoints& out,
                   hipStream_t s);
 
 // Wavefield workspace for E entries (two buffers per wavefield).
 struct Fields {
   int E = 0;
-  DBuf<float> a, b, c, d;       // source field pair, receiver field pair
+  DBuf<float> a, b, c, d;       // four field buffers per entry
   float* s_cur = nullptr;       // source field state after forward(): cur of the last step
   float* s_nxt = nullptr;       //   ... and the field it produced
+  float* spare0 = nullptr;      // the two buffers forward() left free (the
+  float* spare1 = nullptr;      //   receiver field pair of backward())
   void ensure(int E, long long npts, bool receiver);
 };
 
@@ -114,9 +119,12 @@ class Propagator {
   // to a size heuristic.
   void set_tile_x(int tx);
   void set_tuning(bool on) { tune_ = on; }
+  // Forward time loops advance 2 steps per launch (temporal blocking) or 1.
+  void set_steps_per_launch(int n);
+  int steps_per_launch() const { return steps_per_launch_; }
   int tile_choice_cached(int kind, int E) const;
   // Microseconds per launch of `reps` back-to-back step kernels (kind 0
-  // forward, 1 backward) of tile width kTileWidths[width] on E scratch
+  // forward, 1 backward, 2 two-step forward) of tile width kTileWidths[width] on E scratch
   // wavefields of model rv (entry e uses rv + (e / rvdiv) * rvstride).
   float time_steps(int kind, int E, const float* rv, long long rvstride, int rvdiv, Fields& w, int width, int reps,
                    bool clear);
@@ -134,8 +142,16 @@ class Propagator {
   Propagator& operator=(const Propagator&) = delete;
 
  private:
+  // one launch of a forward time loop: steps 1 (next over prev) or 2
+  // (p(it+1), p(it+2) into out1, out2)
+  struct FwdLaunch {
+    int it, steps;
+    float *cur, *prev, *out1, *out2;
+  };
+  // the launches of a forward run over fw's buffers; sets fw's final state
+  std::vector<FwdLaunch> plan_forward(Fields& fw) const;
   void forward_body(int E, const float* rv, long long rvstride, const Sources& src, float* dcal, float* band,
-                    Fields& fw, int rvdiv, int w);
+                    Fields& fw, int rvdiv, int w1, int w2);
   void backward_body(int E, const float* rv, long long rvstride, const Sources& src, const float* vsrc,
                      long long vstride, const float* band, float* grad, long long gstride, Fields& fw, int rvdiv,
                      int it_min, const std::vector<float>& wts, int w);
@@ -163,9 +179,11 @@ class Propagator {
   int prec_;
   Stream stream_;
   DampTables damp_;
-  DevPoints src_[3], rec_[3];
+  DevPoints src_[3], rec_[3];    // points per tile
+  DevPoints srch_[3], rech_[3];  // points per tile grown by one step's reach (two-step kernels)
   int tile_override_ = 0;                 // 0 = tune, else a width from kTileWidths
-  bool tune_ = true;
+  bool tune_ = false;
+  int steps_per_launch_ = 1;
   std::vector<std::pair<long long, int>> tile_cache_;  // (kind*2^32 + E) -> width index
   DBuf<int> sxp_, szp_, gxp_, gzp_;
 };

@@ -337,6 +337,183 @@ __global__ __launch_bounds__(kThreads) void k_fwd_step(FwdStepArgs a, int nentri
   }
 }
 
+// ---- two steps per launch -------------------------------------------------
+//
+// The first step is computed on the tile grown by one step's reach (6 cells),
+// so the second needs nothing from other blocks.  Regions, in cells relative
+// to the tile's first output point:
+//   S0  staged p(t)          [-12, T+12)   (T = TX along x, kTZ along z)
+//   U1  u2 of p(t)           [-7, T+7)
+//   R1  p(t+1) and the model [-6, T+6)
+//   U2  u2 of p(t+1)         [-1, T+1)     (aliases U1)
+// Points of R1 outside the update region [6, n-6) are 0, as in the fields.
+template <int TX>
+struct Tile2 {
+  static constexpr int LZ0 = kTZ + 4 * kHalo, LX0 = TX + 4 * kHalo;
+  static constexpr int UZ1 = kTZ + 2 * kHalo + 2, UX1 = TX + 2 * kHalo + 2;
+  static constexpr int RZ = kTZ + 2 * kHalo, RX = TX + 2 * kHalo;
+  static constexpr int UZ2 = kTZ + 2, UX2 = TX + 2;
+  static constexpr int NS0 = (LX0 * LZ0 + kThreads - 1) / kThreads;
+  static constexpr int NR = (RX * RZ + kThreads - 1) / kThreads;
+  static constexpr int PER = TX / kRows;
+};
+
+__device__ __forceinline__ bool in_update(const Grid& g, int gx, int gz) {
+  return gx >= kHalo && gx < g.nx - kHalo && gz >= kHalo && gz < g.nz - kHalo;
+}
+
+template <int P, int TX>
+__global__ __launch_bounds__(kThreads) void k_fwd_step2(Fwd2StepArgs a, int nentries) {
+  using T = Tile2<TX>;
+  __shared__ float s0[T::LX0 * T::LZ0];
+  __shared__ float su[T::UX1 * T::UZ1];  // U1, then U2
+  __shared__ float r1[T::RX * T::RZ];
+  __shared__ float rvl[T::RX * T::RZ];
+  __shared__ TilePts s_inj1, s_inj2, s_rec;
+  const Grid& g = a.g;
+  int tile, e;
+  block_work(g.ntz * g.ntx, nentries, tile, e);
+  const int tz = tile % g.ntz, tx = tile / g.ntz;
+  const int z0 = kHalo + tz * kTZ, x0 = kHalo + tx * TX;
+  const long long fo = (long long)e * a.fstride;
+  const float* __restrict__ prev = a.prev + fo;
+  const float* __restrict__ rv = a.rv + (long long)(e / a.rvdiv) * a.rvstride;
+
+  // every global load of the block is issued here, before the first barrier
+  PtsLoad l1{0, -1, 0, 0.f}, l2{0, -1, 0, 0.f}, lr{0, -1, 0, 0.f};
+  if (a.inj1.t >= 0) l1 = load_points(a.inj1.pts, &a.inj1, tile, e);
+  if (a.inj2.t >= 0) l2 = load_points(a.inj2.pts, &a.inj2, tile, e);
+  if (a.rec.out != nullptr) lr = load_points(a.rec.pts, nullptr, tile, e);
+  float pv[T::NR], rr[T::NR], dl[T::NR];
+#pragma unroll
+  for (int k = 0; k < T::NR; k++) {
+    const int i = threadIdx.x + k * kThreads;
+    const int rx = i / T::RZ, rz = i - rx * T::RZ;
+    const int gx = x0 - kHalo + rx, gz = z0 - kHalo + rz;
+    const bool ok = i < T::RX * T::RZ && in_update(g, gx, gz);
+    const long long q = (long long)gx * g.ld + gz;
+    pv[k] = ok ? prev[q] : 0.f;
+    rr[k] = ok ? rv[q] : 1.f;
+    dl[k] = ok ? damping_delta(g, gx, gz) : 0.f;
+  }
+  const int lz = threadIdx.x % kTZ, row = threadIdx.x / kTZ;
+  const int iz = z0 + lz;
+  float dl2[T::PER];
+#pragma unroll
+  for (int k = 0; k < T::PER; k++) {
+    const int ix = x0 + row + kRows * k;
+    dl2[k] = (ix < g.nx - kHalo && iz < g.nz - kHalo) ? damping_delta(g, ix, iz) : 0.f;
+  }
+  {
+    const float* __restrict__ cur = a.cur + fo;
+    float v[T::NS0];
+#pragma unroll
+    for (int k = 0; k < T::NS0; k++) {
+      const int i = threadIdx.x + k * kThreads;
+      const int lx = i / T::LZ0, l = i - lx * T::LZ0;
+      const int gx = x0 - 2 * kHalo + lx, gz = z0 - 2 * kHalo + l;
+      const bool ok = i < T::LX0 * T::LZ0 && gx >= 0 && gx < g.nx && gz >= 0 && gz < g.nz;
+      v[k] = ok ? cur[(long long)gx * g.ld + gz] : 0.f;
+    }
+#pragma unroll
+    for (int k = 0; k < T::NS0; k++) {
+      const int i = threadIdx.x + k * kThreads;
+      if (i < T::LX0 * T::LZ0) s0[i] = v[k];
+    }
+  }
+  store_points(l1, s_inj1);
+  store_points(l2, s_inj2);
+  store_points(lr, s_rec);
+  __syncthreads();
+
+  // u2 of p(t) over U1
+  for (int i = threadIdx.x; i < T::UX1 * T::UZ1; i += kThreads) {
+    const int ux = i / T::UZ1, uz = i - ux * T::UZ1;
+    const int gx = x0 - kHalo - 1 + ux, gz = z0 - kHalo - 1 + uz;
+    float v = 0.f;
+    if (gx >= 5 && gx < g.nx - 5 && gz >= 5 && gz < g.nz - 5) v = u2_at<P, T::LZ0>(s0 + (ux + 5) * T::LZ0 + uz + 5);
+    su[i] = v;
+  }
+  __syncthreads();
+
+  // p(t+1) over R1, with the sources of step t+1
+#pragma unroll
+  for (int k = 0; k < T::NR; k++) {
+    const int i = threadIdx.x + k * kThreads;
+    if (i >= T::RX * T::RZ) break;
+    const int rx = i / T::RZ, rz = i - rx * T::RZ;
+    const int gx = x0 - kHalo + rx, gz = z0 - kHalo + rz;
+    float v = 0.f;
+    if (in_update(g, gx, gz)) {
+      const float c = s0[(rx + kHalo) * T::LZ0 + rz + kHalo];
+      v = update_at<true, P, T::UZ1>(c, pv[k], rr[k], su + (rx + 1) * T::UZ1 + rz + 1, dl[k]);
+      if (l1.n > 0) {
+        const long long q = (long long)gx * g.ld + gz;
+        v = l1.n <= kTilePts ? apply_staged(s_inj1, l1.n, a.inj1.sign, q, v) : apply_points(a.inj1, tile, e, q, v);
+      }
+    }
+    r1[i] = v;
+    rvl[i] = rr[k];
+  }
+  __syncthreads();
+
+  // u2 of p(t+1) over U2 (U1 is dead)
+  for (int i = threadIdx.x; i < T::UX2 * T::UZ2; i += kThreads) {
+    const int ux = i / T::UZ2, uz = i - ux * T::UZ2;
+    const int gx = x0 - 1 + ux, gz = z0 - 1 + uz;
+    float v = 0.f;
+    if (gx >= 5 && gx < g.nx - 5 && gz >= 5 && gz < g.nz - 5) v = u2_at<P, T::RZ>(r1 + (ux + 5) * T::RZ + uz + 5);
+    su[i] = v;
+  }
+  __syncthreads();
+
+  // p(t+2) over the tile; both new fields, records and band rows out
+  float* __restrict__ out1 = a.out1 + fo;
+  float* __restrict__ out2 = a.out2 + fo;
+#pragma unroll
+  for (int k = 0; k < T::PER; k++) {
+    const int lx = row + kRows * k;
+    const int ix = x0 + lx;
+    if (ix >= g.nx - kHalo || iz >= g.nz - kHalo) continue;
+    const long long q = (long long)ix * g.ld + iz;
+    const int lr1 = (lx + kHalo) * T::RZ + lz + kHalo;
+    const float p0 = s0[(lx + 2 * kHalo) * T::LZ0 + lz + 2 * kHalo];
+    const float p1 = r1[lr1];
+    float p2 = update_at<true, P, T::UZ2>(p1, p0, rvl[lr1], su + (lx + 1) * T::UZ2 + lz + 1, dl2[k]);
+    if (l2.n > 0)
+      p2 = l2.n <= kTilePts ? apply_staged(s_inj2, l2.n, a.inj2.sign, q, p2) : apply_points(a.inj2, tile, e, q, p2);
+    out1[q] = p1;
+    out2[q] = p2;
+    if (a.band) {
+      const int bi = band_index(g, ix, iz);
+      if (bi >= 0) {
+        float* b = a.band + (long long)e * a.band_estride + bi;
+        b[0] = p0;
+        b[a.band_step] = p1;
+      }
+    }
+    if (lr.n > 0) {
+      float* ro = a.rec.out + (long long)e * a.rec.estride;
+      if (lr.n <= kTilePts) {
+        for (int j = 0; j < lr.n; j++)
+          if (s_rec.q[j] == q) {
+            ro[s_rec.pid[j]] = p0;
+            ro[a.rec_step + s_rec.pid[j]] = p1;
+          }
+      } else {
+        const int b = a.rec.pts.tile_ptr[tile], eidx = a.rec.pts.tile_ptr[tile + 1];
+        for (int j = b; j < eidx; j++) {
+          const int pid = a.rec.pts.items[j];
+          if (a.rec.pts.idx[pid] == q) {
+            ro[pid] = p0;
+            ro[a.rec_step + pid] = p1;
+          }
+        }
+      }
+    }
+  }
+}
+
 template <int P, int TX>
 __global__ __launch_bounds__(kThreads) void k_bwd_step(BwdStepArgs a, int nentries) {
   using T = Tile<TX>;
@@ -473,7 +650,24 @@ void bwd_dispatch(const BwdStepArgs& a, int E, int precision, hipStream_t s) {
     hipLaunchKernelGGL((k_bwd_step<kFp32, TX>), grid, dim3(kThreads), 0, s, a, E);
 }
 
+template <int TX>
+void fwd2_dispatch(const Fwd2StepArgs& a, int E, int precision, hipStream_t s) {
+  const dim3 grid(a.g.ntz * a.g.ntx * E);
+  if (precision == kExact)
+    hipLaunchKernelGGL((k_fwd_step2<kExact, TX>), grid, dim3(kThreads), 0, s, a, E);
+  else
+    hipLaunchKernelGGL((k_fwd_step2<kFp32, TX>), grid, dim3(kThreads), 0, s, a, E);
+}
+
 }  // namespace
+
+void launch_fwd_step2(const Fwd2StepArgs& a, int nentries, int precision, hipStream_t stream) {
+  switch (a.g.tile_x) {
+    case 8: fwd2_dispatch<8>(a, nentries, precision, stream); break;
+    case 32: fwd2_dispatch<32>(a, nentries, precision, stream); break;
+    default: fwd2_dispatch<16>(a, nentries, precision, stream); break;
+  }
+}
 
 int tile_count_x(int nx, int tile_x) { return (nx - 2 * kHalo + tile_x - 1) / tile_x; }
 

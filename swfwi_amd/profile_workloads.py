@@ -35,23 +35,26 @@ def run_shots(nt):
           f"({case.ns * nt * case.update_points / dt / 1e9:.1f} Gpts/s incl. transfer)", flush=True)
 
 
-def run_sweep(entries, reps):
+def run_sweep(entries, reps, precision):
     """Per-launch time of every step kernel width over batch sizes on the
     301x921 grid (back-to-back launches on scratch wavefields)."""
     case = workloads.ess_301x921(nt=8)
     vtrue, _ = workloads.models(case)
     s = capi.Solver(case.nx, case.nz, case.nb, case.dt, case.dx, case.fm, case.nt, case.sx, case.sz, case.gx,
-                    case.gz)
+                    case.gz, precision)
     s.bind_velocity(capi.expand_domain(vtrue, case.nb, case.dx, case.dt))
     pts = case.update_points
+    for blocks in (256, 738, 2952):
+        print(f"empty kernel, {blocks} blocks: stream {capi.launch_latency(blocks, reps):.2f} us, "
+              f"graph {capi.launch_latency(blocks, reps, graph=True):.2f} us per launch", flush=True)
     for E in entries:
-        for kind, byts in ((0, 16.0), (1, 40.0)):
+        for kind, byts in ((0, 16.0), (2, 32.0), (1, 40.0)):
             row = []
             for tx in (8, 16, 32):
                 s.time_steps(kind, E, tx, 4)  # warm
                 us = s.time_steps(kind, E, tx, reps)
                 row.append(f"tx{tx} {us:8.1f} us {E * pts * byts / (us * 1e3):7.0f} GB/s")
-            print(f"{'fwd' if kind == 0 else 'bwd'} E={E:4d}: " + " | ".join(row), flush=True)
+            print(f"{['fwd', 'bwd', 'fwd2'][kind]:4s} E={E:4d}: " + " | ".join(row), flush=True)
 
 
 def run_ess(nt, epochs, graphs=True):
@@ -114,6 +117,7 @@ def main(argv=None):
     ap.add_argument("mode", choices=["copy", "shots", "ess", "enkf", "sweep"])
     ap.add_argument("--entries", default="1,2,3,4,8,16,60,100,200")
     ap.add_argument("--reps", type=int, default=100)
+    ap.add_argument("--precision", choices=["exact", "fp32"], default="exact")
     ap.add_argument("--members", type=int, default=100)
     ap.add_argument("--n", type=int, default=1 << 28)
     ap.add_argument("--nt", type=int, default=None)
@@ -128,7 +132,8 @@ def main(argv=None):
     elif a.mode == "shots":
         run_shots(a.nt or 400)
     elif a.mode == "sweep":
-        run_sweep([int(x) for x in a.entries.split(",")], a.reps)
+        run_sweep([int(x) for x in a.entries.split(",")], a.reps,
+                  capi.PREC_FP32 if a.precision == "fp32" else capi.PREC_EXACT)
     elif a.mode == "enkf":
         run_enkf(a.nt or 4000, a.members)
     else:
