@@ -111,17 +111,14 @@ int swfwi_solver_set_tile_x(swfwi_solver h, int tile_x);
 /* Tile width in use for a kernel kind (0 forward, 1 backward) and batch
  * size, 0 when that batch size has not run yet. */
 int swfwi_solver_get_tile_x(swfwi_solver h, int kind, int entries, int *tile_x);
-/* Record a HIP event pair around every stencil launch on the handle's stream;
- * swfwi_solver_kernel_times returns the accumulated milliseconds and launch
- * counts of the forward [0] and backward [1] step kernels. */
+/* Record a HIP event pair around every forward / backward time loop on the
+ * handle's stream (graph replays included); swfwi_solver_kernel_times waits
+ * for them and returns the accumulated milliseconds and step-kernel launch
+ * counts of the forward [0] and backward [1] loops. */
 int swfwi_solver_set_timing(swfwi_solver h, int on);
 int swfwi_solver_kernel_times(swfwi_solver h, double *ms, long long *launches);
-/* Forward time loops advance 1 step per launch (the default) or 2: the
- * second step of a tile computed from the first without leaving the block,
- * at the cost of recomputing a 6-cell ring of the first. */
-int swfwi_solver_set_steps_per_launch(swfwi_solver h, int n);
 /* Microseconds per launch of `reps` back-to-back step kernels (kind 0 forward,
- * 1 backward, 2 two-step forward) of width tile_x (8, 16, 32) over `entries` scratch wavefields of
+ * 1 backward) of width tile_x (8, 16, 32) over `entries` scratch wavefields of
  * the bound model: the tile tuner's measurement, for sweeps.  Uses the
  * handle's workspace, so a following calgradient needs a new forward run. */
 int swfwi_solver_time_steps(swfwi_solver h, int kind, int entries, int tile_x, int reps, float *us);

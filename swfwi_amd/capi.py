@@ -49,7 +49,6 @@ SIGNATURES = {
     "swfwi_solver_set_timing": (_I, [_VP, _I]),
     "swfwi_solver_kernel_times": (_I, [_VP, _f64, np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")]),
     "swfwi_solver_time_steps": (_I, [_VP, _I, _I, _I, _I, C.POINTER(_F)]),
-    "swfwi_solver_set_steps_per_launch": (_I, [_VP, _I]),
     "swfwi_enkf_create": (_I, [C.POINTER(_VP), _VP, _f32, _f32, _F, _I]),
     "swfwi_enkf_destroy": (_I, [_VP]),
     "swfwi_enkf_set_members": (_I, [_VP, _f32]),
@@ -232,9 +231,6 @@ class Solver:
         check(load().swfwi_solver_get_tile_x(self.h, kind, entries, C.byref(t)))
         return t.value
 
-    def set_steps_per_launch(self, n):
-        check(load().swfwi_solver_set_steps_per_launch(self.h, int(n)))
-
     def time_steps(self, kind, entries, tile_x, reps=100):
         """Microseconds per launch of back-to-back step kernels on scratch fields."""
         us = _F(0)
@@ -245,7 +241,7 @@ class Solver:
         check(load().swfwi_solver_set_timing(self.h, int(bool(on))))
 
     def kernel_times(self):
-        """{'fwd': (ms_total, launches), 'bwd': (...)} of the timed stencil launches."""
+        """{'fwd': (ms_total, launches), 'bwd': (...)} of the timed time loops."""
         ms = np.zeros(2, np.float64)
         n = np.zeros(2, np.int64)
         check(load().swfwi_solver_kernel_times(self.h, ms, n))

@@ -70,6 +70,7 @@ int run_step(float* prev, const float* cur, const float* vel, float* u2, int nx,
     NEED(cur);
     NEED(vel);
     if (nx < 2 * kHalo + 1 || nz < 2 * kHalo + 1) fail(kErrArg, "grid smaller than the stencil");
+    if ((long long)nx * nz >= (1LL << 31)) fail(kErrRange, "grid too large: a field is indexed with 32-bit offsets");
     if (precision != kExact && precision != kFp32) fail(kErrArg, "unknown precision mode");
     StepContext& c = step_ctx();
     std::lock_guard<std::mutex> lock(c.mu);
@@ -345,7 +346,7 @@ int swfwi_solver_time_steps(swfwi_solver h, int kind, int entries, int tile_x, i
   return guarded([&] {
     NEED(h);
     NEED(us);
-    if (kind < 0 || kind > 2) fail(kErrArg, "kind must be 0 (forward), 1 (backward) or 2 (two-step forward)");
+    if (kind != 0 && kind != 1) fail(kErrArg, "kind must be 0 (forward) or 1 (backward)");
     if (entries < 1 || reps < 1) fail(kErrArg, "entries and reps must be >= 1");
     int width = -1;
     for (int i = 0; i < 3; i++)
@@ -355,13 +356,6 @@ int swfwi_solver_time_steps(swfwi_solver h, int kind, int entries, int tile_x, i
     if (!s.rv()) fail(kErrState, "bind the velocity first");
     *us = s.prop().time_steps(kind, entries, s.rv(), 0, 1, s.w, width, reps, true);
     s.w.s_cur = s.w.s_nxt = nullptr;  // the workspace no longer holds a forward state
-  });
-}
-
-int swfwi_solver_set_steps_per_launch(swfwi_solver h, int n) {
-  return guarded([&] {
-    NEED(h);
-    h->s->prop().set_steps_per_launch(n);
   });
 }
 

@@ -26,10 +26,16 @@ enum Precision : int {
 // Points of interest (sources, receivers) bucketed by stencil tile, so that
 // the thread owning a grid point finds the points that touch it with a short
 // scan of its tile's bucket.
+constexpr int kTilePts = 64;      // points of one tile a block stages in LDS
+
 struct PointList {
   const int* tile_ptr = nullptr;  // [ntiles+1] CSR offsets
   const int* items = nullptr;     // point ids, grouped by tile, ascending id inside a tile
   const int* idx = nullptr;       // [npoints] flattened grid index ix*ld + iz
+  // The same lists as fixed-size slots, read with one level of loads: set
+  // when no tile holds more than kTilePts points, else null (CSR only).
+  const int* count = nullptr;     // [ntiles] points per tile
+  const int2* slots = nullptr;    // [ntiles][kTilePts] (grid index, point id); grid index -1 past count
 };
 
 struct Grid {
@@ -77,27 +83,6 @@ struct FwdStepArgs {
   float* u2out = nullptr;         // optional: the u2 scratch of the reference kernel
 };
 
-// Two forward steps in one launch (t and t+1): from cur = p(t) and prev =
-// p(t-1) to out1 = p(t+1) and out2 = p(t+2), never in place.
-struct Fwd2StepArgs {
-  Grid g;
-  long long fstride = 0;
-  const float* cur = nullptr;
-  const float* prev = nullptr;
-  float* out1 = nullptr;
-  float* out2 = nullptr;
-  const float* rv = nullptr;
-  long long rvstride = 0;
-  int rvdiv = 1;
-  Inject inj1;                    // into p(t+1): lists of tiles grown by one step's reach
-  Inject inj2;                    // into p(t+2): lists of tiles
-  Record rec;                     // row t from p(t); row t+1 (out + rec_step) from p(t+1)
-  long long rec_step = 0;
-  float* band = nullptr;          // band row t of p(t); row t+1 at + band_step
-  long long band_step = 0;
-  long long band_estride = 0;
-};
-
 struct BwdStepArgs {
   Grid g;
   long long fstride = 0;
@@ -127,7 +112,6 @@ constexpr int kTileWidths[3] = {8, 16, 32};
 int tile_count_x(int nx, int tile_x);
 void launch_fwd_step(const FwdStepArgs& a, int nentries, bool damped, int precision, hipStream_t stream);
 void launch_bwd_step(const BwdStepArgs& a, int nentries, int precision, hipStream_t stream);
-void launch_fwd_step2(const Fwd2StepArgs& a, int nentries, int precision, hipStream_t stream);
 
 void launch_fill(float* p, long long n, float v, hipStream_t s);
 void launch_fill_d(double* p, long long n, double v, hipStream_t s);

@@ -10,6 +10,8 @@ void Geometry::derive() {
   if (nx <= 0 || nz <= 0 || nt <= 0) fail(kErrArg, "grid and time dimensions must be positive");
   if (nb < 6) fail(kErrArg, "sponge width nb must be >= 6 (the 6-wide boundary band lives inside the sponge)");
   if (!(dt > 0) || !(dx > 0)) fail(kErrArg, "dt and dx must be positive");
+  if ((long long)(nx + 2 * nb + 12) * (nz + nb + 12) >= (1LL << 31))
+    fail(kErrRange, "padded grid too large: a field is indexed with 32-bit offsets");
   if (sx.size() != sz.size() || gx.size() != gz.size()) fail(kErrArg, "position arrays differ in length");
   if (sx.empty() || gx.empty()) fail(kErrArg, "need at least one source and one receiver");
   // Params::check (main-fm-damp.cpp:183-194): every position inside the model
@@ -49,6 +51,7 @@ void build_damping(int nx, int nz, int nb, DampTables& t, hipStream_t s) {
   t.dx.upload(dxv, s);
   t.dz.upload(dzv, s);
   t.zdamp = nz - nb;
+  SWFWI_HIP(hipStreamSynchronize(s));  // the host vectors end here
 }
 
 void build_points(const std::vector<long long>& idx, int ld, int ntz, int ntx, int tile_x, int halo, DevPoints& out,
@@ -85,6 +88,20 @@ void build_points(const std::vector<long long>& idx, int ld, int ntz, int ntx, i
   out.n = n;
   out.max_per_tile = 0;
   for (int t = 0; t < ntiles; t++) out.max_per_tile = std::max(out.max_per_tile, ptr[t + 1] - ptr[t]);
+  std::vector<int> count(ntiles);
+  std::vector<int2> slots;
+  if (out.max_per_tile <= kTilePts) {
+    slots.assign((size_t)ntiles * kTilePts, make_int2(-1, 0));
+    for (int t = 0; t < ntiles; t++) {
+      count[t] = ptr[t + 1] - ptr[t];
+      for (int j = ptr[t]; j < ptr[t + 1]; j++) {
+        const int pid = items[j];
+        slots[(size_t)t * kTilePts + (j - ptr[t])] = make_int2(idx32[pid], pid);
+      }
+    }
+    out.count.upload(count, s);
+    out.slots.upload(slots, s);
+  }
   SWFWI_HIP(hipStreamSynchronize(s));  // the host vectors end here
 }
 
@@ -120,8 +137,6 @@ Propagator::Propagator(const Geometry& g, int precision) : g_(g), prec_(precisio
     const int ntx = tile_count_x(g_.nxpad, kTileWidths[w]);
     build_points(si, g_.ld, g_.ntz, ntx, kTileWidths[w], 0, src_[w], s);
     build_points(gi, g_.ld, g_.ntz, ntx, kTileWidths[w], 0, rec_[w], s);
-    build_points(si, g_.ld, g_.ntz, ntx, kTileWidths[w], kHalo, srch_[w], s);
-    build_points(gi, g_.ld, g_.ntz, ntx, kTileWidths[w], kHalo, rech_[w], s);
   }
   sxp_.upload(sxp, s);
   szp_.upload(szp, s);
@@ -180,19 +195,7 @@ float Propagator::time_steps(int kind, int E, const float* rv, long long rvstrid
   SWFWI_HIP(hipEventCreate(&e1));
   SWFWI_HIP(hipEventRecord(e0, s));
   for (int r = 0; r < reps; r++) {
-    if (kind == 2) {
-      Fwd2StepArgs a;
-      a.g = grid(width);
-      a.fstride = np;
-      a.cur = (r & 1) ? w.c.get() : w.a.get();
-      a.prev = (r & 1) ? w.d.get() : w.b.get();
-      a.out1 = (r & 1) ? w.b.get() : w.d.get();
-      a.out2 = (r & 1) ? w.a.get() : w.c.get();
-      a.rv = rv;
-      a.rvstride = rvstride;
-      a.rvdiv = rvdiv;
-      launch_fwd_step2(a, E, prec_, s);
-    } else if (kind == 0) {
+    if (kind == 0) {
       FwdStepArgs a;
       a.g = grid(width);
       a.fstride = np;
@@ -287,50 +290,33 @@ int Propagator::xcorr_range(float dt, int nt, std::vector<float>& weights) {
 
 std::vector<Propagator::FwdLaunch> Propagator::plan_forward(Fields& fw) const {
   // Buffer rotation of a forward run: b holds p(0) (the injected zero field),
-  // a holds p(-1) = 0.  A single step writes next over prev; a two-step
-  // launch writes p(t+1), p(t+2) into the free pair.
+  // a holds p(-1) = 0; each step writes next over prev.
   const int nt = g_.nt;
   std::vector<FwdLaunch> plan;
   float* cur = fw.b.get();
   float* prv = fw.a.get();
-  float* f0 = fw.c.get();
-  float* f1 = fw.d.get();
-  int it = 0;
-  if (steps_per_launch_ == 2) {
-    for (; it + 1 < nt; it += 2) {
-      plan.push_back({it, 2, cur, prv, f0, f1});
-      float* oc = cur;
-      float* op = prv;
-      prv = f0;  // p(t+1)
-      cur = f1;  // p(t+2)
-      f0 = op;
-      f1 = oc;
-    }
+  for (int it = 0; it < nt; it++) {
+    plan.push_back({it, cur, prv});
+    std::swap(cur, prv);
   }
-  for (; it < nt; it++) {
-    plan.push_back({it, 1, cur, prv, nullptr, nullptr});
-    std::swap(cur, prv);  // next was written over prev
-  }
-  // after the last launch: s_cur = p(nt-1), s_nxt = p(nt)
+  // after the last step: s_cur = p(nt-1), s_nxt = p(nt)
   fw.s_cur = prv;
   fw.s_nxt = cur;
-  fw.spare0 = f0;
-  fw.spare1 = f1;
+  fw.spare0 = fw.c.get();
+  fw.spare1 = fw.d.get();
   return plan;
 }
 
 void Propagator::forward_body(int E, const float* rv, long long rvstride, const Sources& src, float* dcal,
-                              float* band, Fields& fw, int rvdiv, int w1, int w2) {
+                              float* band, Fields& fw, int rvdiv, int w) {
   hipStream_t s = stream();
   const long long np = g_.npts();
   const int nt = g_.nt, ng = g_.ng();
-  // the field frame outside the update region stays 0 in all four buffers
   launch_fill(fw.a.get(), (long long)E * np, 0.f, s);
   launch_fill(fw.b.get(), (long long)E * np, 0.f, s);
-  launch_fill(fw.c.get(), (long long)E * np, 0.f, s);
-  launch_fill(fw.d.get(), (long long)E * np, 0.f, s);
 
   Inject inj;
+  inj.pts = src_[w].view();
   inj.amp = src.amp;
   inj.ld = src.ld;
   inj.estride = src.estride;
@@ -339,64 +325,29 @@ void Propagator::forward_body(int E, const float* rv, long long rvstride, const 
   inj.hi = src.hi;
 
   // step 0 injects into the initially zero cur (addSource before stepForward)
-  inj.pts = src_[w1].view();
   inj.t = 0;
   launch_point_inject(fw.b.get(), np, inj, g_.ns(), E, s);
 
   FwdStepArgs a;
-  a.g = grid(w1);
+  a.g = grid(w);
   a.fstride = np;
   a.rv = rv;
   a.rvstride = rvstride;
   a.rvdiv = rvdiv;
   a.inj = inj;
-  a.rec.pts = rec_[w1].view();
+  a.rec.pts = rec_[w].view();
   a.rec.estride = (long long)nt * ng;
   a.band_estride = (long long)nt * g_.bsize;
-
-  Fwd2StepArgs b2;
-  b2.g = grid(w2);
-  b2.fstride = np;
-  b2.rv = rv;
-  b2.rvstride = rvstride;
-  b2.rvdiv = rvdiv;
-  b2.inj1 = inj;
-  b2.inj1.pts = srch_[w2].view();
-  b2.inj2 = inj;
-  b2.inj2.pts = src_[w2].view();
-  b2.rec.pts = rec_[w2].view();
-  b2.rec.estride = (long long)nt * ng;
-  b2.rec_step = ng;
-  b2.band_step = g_.bsize;
-  b2.band_estride = (long long)nt * g_.bsize;
-
   for (const FwdLaunch& l : plan_forward(fw)) {
     const int it = l.it;
-    if (l.steps == 2) {
-      b2.cur = l.cur;
-      b2.prev = l.prev;
-      b2.out1 = l.out1;
-      b2.out2 = l.out2;
-      b2.inj1.t = it + 1;
-      b2.inj2.t = it + 2 < nt ? it + 2 : -1;
-      b2.rec.out = dcal ? dcal + (long long)it * ng : nullptr;
-      b2.band = band ? band + (long long)it * g_.bsize : nullptr;
-      timed_launch(0, [&] { launch_fwd_step2(b2, E, prec_, s); });
-    } else {
-      a.cur = l.cur;
-      a.prev = l.prev;
-      a.inj.t = it + 1 < nt ? it + 1 : -1;
-      a.rec.out = dcal ? dcal + (long long)it * ng : nullptr;
-      a.band = band ? band + (long long)it * g_.bsize : nullptr;
-      timed_launch(0, [&] { launch_fwd_step(a, E, true, prec_, s); });
-    }
+    a.cur = l.cur;
+    a.prev = l.prev;
+    a.inj.t = it + 1 < nt ? it + 1 : -1;
+    a.rec.out = dcal ? dcal + (long long)it * ng : nullptr;
+    a.band = band ? band + (long long)it * g_.bsize : nullptr;
+    launch_fwd_step(a, E, true, prec_, s);
   }
   SWFWI_HIP(hipGetLastError());
-}
-
-void Propagator::set_steps_per_launch(int n) {
-  if (n != 1 && n != 2) fail(kErrArg, "steps per launch must be 1 or 2");
-  steps_per_launch_ = n;
 }
 
 void Propagator::forward(int E, const float* rv, long long rvstride, const Sources& src, float* dcal, float* band,
@@ -405,13 +356,10 @@ void Propagator::forward(int E, const float* rv, long long rvstride, const Sourc
   w.ensure(E, np, true);
   // tile widths for this batch, chosen now while the workspace is free
   // (backward() runs on the state this forward leaves behind)
-  const int w1 = tile_choice(0, E, rv, rvstride, rvdiv, w, true);
-  const int w2 = steps_per_launch_ == 2 ? tile_choice(2, E, rv, rvstride, rvdiv, w, true) : w1;
+  const int tw = tile_choice(0, E, rv, rvstride, rvdiv, w, true);
   tile_choice(1, E, rv, rvstride, rvdiv, w, true);
   const uintptr_t key[] = {1,
-                           (uintptr_t)w1,
-                           (uintptr_t)w2,
-                           (uintptr_t)steps_per_launch_,
+                           (uintptr_t)tw,
                            (uintptr_t)E,
                            (uintptr_t)rv,
                            (uintptr_t)rvstride,
@@ -427,113 +375,101 @@ void Propagator::forward(int E, const float* rv, long long rvstride, const Sourc
                            (uintptr_t)w.b.get(),
                            (uintptr_t)w.c.get(),
                            (uintptr_t)w.d.get(),
-                           (uintptr_t)prec_,
-                           (uintptr_t)timing_};
-  run_maybe_graph(std::vector<uintptr_t>(std::begin(key), std::end(key)), E,
-                  [&] { forward_body(E, rv, rvstride, src, dcal, band, w, rvdiv, w1, w2); });
+                           (uintptr_t)prec_};
+  run_maybe_graph(std::vector<uintptr_t>(std::begin(key), std::end(key)), E, 0, g_.nt,
+                  [&] { forward_body(E, rv, rvstride, src, dcal, band, w, rvdiv, tw); });
   const std::vector<FwdLaunch> plan = plan_forward(w);  // sets the final state
   fwd_launches += (long long)plan.size();
   fwd_point_updates += (double)E * (g_.nxpad - 2 * kHalo) * (g_.nzpad - 2 * kHalo) * g_.nt;
 }
 
-template <typename F>
-void Propagator::timed_launch(int kind, F&& launch) {
-  if (!timing_) {
-    launch();
-    return;
-  }
-  hipEvent_t ev[2];
-  for (hipEvent_t& e : ev) {
-    if (!event_pool_.empty()) {
-      e = event_pool_.back();
-      event_pool_.pop_back();
-    } else {
-      SWFWI_HIP(hipEventCreate(&e));
-    }
-  }
-  SWFWI_HIP(hipEventRecord(ev[0], stream()));
-  launch();
-  SWFWI_HIP(hipEventRecord(ev[1], stream()));
-  active_events_[kind].push_back(ev[0]);
-  active_events_[kind].push_back(ev[1]);
-}
-
 void Propagator::set_timing(bool on) {
+  collect_timing();
   timing_ = on;
   kernel_ms[0] = kernel_ms[1] = 0;
   kernel_n[0] = kernel_n[1] = 0;
 }
 
-void Propagator::collect_timing() {}
-
-void Propagator::read_events(const std::vector<hipEvent_t>* ev) {
+void Propagator::collect_timing() {
+  // timed loops still in flight: wait for them and add their durations
+  if (pending_.empty()) return;
   SWFWI_HIP(hipStreamSynchronize(stream()));
-  for (int k = 0; k < 2; k++)
-    for (size_t i = 0; i + 1 < ev[k].size(); i += 2) {
-      float ms = 0;
-      SWFWI_HIP(hipEventElapsedTime(&ms, ev[k][i], ev[k][i + 1]));
-      kernel_ms[k] += ms;
-      kernel_n[k] += 1;
-    }
+  for (const TimedLoop& t : pending_) {
+    float ms = 0;
+    SWFWI_HIP(hipEventElapsedTime(&ms, t.start, t.stop));
+    kernel_ms[t.kind] += ms;
+    kernel_n[t.kind] += t.launches;
+    event_pool_.push_back(t.start);
+    event_pool_.push_back(t.stop);
+  }
+  pending_.clear();
 }
 
-void Propagator::run_maybe_graph(const std::vector<uintptr_t>& key, int E, const std::function<void()>& body) {
+hipEvent_t Propagator::pooled_event() {
+  hipEvent_t e;
+  if (!event_pool_.empty()) {
+    e = event_pool_.back();
+    event_pool_.pop_back();
+  } else {
+    SWFWI_HIP(hipEventCreate(&e));
+  }
+  return e;
+}
+
+void Propagator::run_maybe_graph(const std::vector<uintptr_t>& key, int E, int kind, long long launches,
+                                 const std::function<void()>& body) {
   hipStream_t s = stream();
-  active_events_[0].clear();
-  active_events_[1].clear();
-  // HIP cannot time events recorded inside a captured graph, so the timed
-  // mode launches the loop directly (same kernels, same stream).  Large
-  // batches are not launch-bound: a graph would only add its capture.
-  if (!graphs_ || timing_ || E > kGraphMaxEntries) {
+  // timing: one event pair around the whole time loop (outside any graph),
+  // so the measured loop is the production one, launch gaps included
+  hipEvent_t t0 = nullptr;
+  if (timing_) {
+    t0 = pooled_event();
+    SWFWI_HIP(hipEventRecord(t0, s));
+  }
+  // Large batches are not launch-bound: a graph would only add its capture.
+  if (!graphs_ || E > kGraphMaxEntries) {
     body();
-    if (timing_) {
-      read_events(active_events_);
-      for (int k = 0; k < 2; k++) event_pool_.insert(event_pool_.end(), active_events_[k].begin(), active_events_[k].end());
+  } else {
+    GraphEntry* hit = nullptr;
+    for (auto& ge : graph_cache_)
+      if (ge.key == key) hit = &ge;
+    if (!hit) {
+      hipGraph_t graph = nullptr;
+      SWFWI_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      try {
+        body();
+      } catch (...) {
+        (void)hipStreamEndCapture(s, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+      }
+      SWFWI_HIP(hipStreamEndCapture(s, &graph));
+      hipGraphExec_t exec = nullptr;
+      SWFWI_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+      (void)hipGraphDestroy(graph);
+      if (graph_cache_.size() >= 16) {
+        (void)hipGraphExecDestroy(graph_cache_.front().exec);
+        graph_cache_.erase(graph_cache_.begin());
+      }
+      graph_cache_.push_back(GraphEntry{key, exec});
+      hit = &graph_cache_.back();
     }
-    return;
+    SWFWI_HIP(hipGraphLaunch(hit->exec, s));
   }
-  for (auto& ge : graph_cache_) {
-    if (ge.key == key) {
-      SWFWI_HIP(hipGraphLaunch(ge.exec, s));
-      if (timing_) read_events(ge.events);
-      return;
-    }
+  if (timing_) {
+    hipEvent_t t1 = pooled_event();
+    SWFWI_HIP(hipEventRecord(t1, s));
+    pending_.push_back(TimedLoop{t0, t1, kind, launches});
+    if (pending_.size() >= 64) collect_timing();
   }
-  hipGraph_t graph = nullptr;
-  SWFWI_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  try {
-    body();
-  } catch (...) {
-    (void)hipStreamEndCapture(s, &graph);
-    if (graph) (void)hipGraphDestroy(graph);
-    throw;
-  }
-  SWFWI_HIP(hipStreamEndCapture(s, &graph));
-  hipGraphExec_t exec = nullptr;
-  SWFWI_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
-  (void)hipGraphDestroy(graph);
-  if (graph_cache_.size() >= 16) {
-    drop_graph(graph_cache_.front());
-    graph_cache_.erase(graph_cache_.begin());
-  }
-  GraphEntry ge;
-  ge.key = key;
-  ge.exec = exec;
-  ge.events[0] = active_events_[0];
-  ge.events[1] = active_events_[1];
-  graph_cache_.push_back(ge);
-  SWFWI_HIP(hipGraphLaunch(exec, s));
-  if (timing_) read_events(ge.events);
-}
-
-void Propagator::drop_graph(GraphEntry& ge) {
-  (void)hipGraphExecDestroy(ge.exec);
-  for (int k = 0; k < 2; k++)
-    for (hipEvent_t e : ge.events[k]) (void)hipEventDestroy(e);
 }
 
 Propagator::~Propagator() {
-  for (auto& ge : graph_cache_) drop_graph(ge);
+  for (auto& ge : graph_cache_) (void)hipGraphExecDestroy(ge.exec);
+  for (const TimedLoop& t : pending_) {
+    (void)hipEventDestroy(t.start);
+    (void)hipEventDestroy(t.stop);
+  }
   for (hipEvent_t e : event_pool_) (void)hipEventDestroy(e);
 }
 
@@ -567,9 +503,8 @@ int Propagator::backward(int E, const float* rv, long long rvstride, const Sourc
                            (uintptr_t)w.s_nxt,
                            (uintptr_t)w.spare0,
                            (uintptr_t)w.spare1,
-                           (uintptr_t)prec_,
-                           (uintptr_t)timing_};
-  run_maybe_graph(std::vector<uintptr_t>(std::begin(key), std::end(key)), E, [&] {
+                           (uintptr_t)prec_};
+  run_maybe_graph(std::vector<uintptr_t>(std::begin(key), std::end(key)), E, 1, g_.nt - it_min, [&] {
     backward_body(E, rv, rvstride, src, vsrc, vstride, band, grad, gstride, w, rvdiv, it_min, wts, tw);
   });
   const int iters = g_.nt - it_min;
@@ -628,7 +563,7 @@ void Propagator::backward_body(int E, const float* rv, long long rvstride, const
     a.g_inj.t = more ? it - 1 : -1;
     a.band = more ? band + (long long)(it - 1) * g_.bsize : nullptr;
     a.weight = wts[it];
-    timed_launch(1, [&] { launch_bwd_step(a, E, prec_, s); });
+    launch_bwd_step(a, E, prec_, s);
     std::swap(s_cur, s_prv);
     std::swap(g_cur, g_prv);
   }

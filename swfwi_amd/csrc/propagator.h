@@ -37,10 +37,21 @@ struct Geometry {
 
 // Per-tile CSR of a point set on the device.
 struct DevPoints {
-  DBuf<int> tile_ptr, items, idx;
+  DBuf<int> tile_ptr, items, idx, count;
+  DBuf<int2> slots;
   int n = 0;
   int max_per_tile = 0;
-  PointList view() const { return PointList{tile_ptr.get(), items.get(), idx.get()}; }
+  PointList view() const {
+    PointList p;
+    p.tile_ptr = tile_ptr.get();
+    p.items = items.get();
+    p.idx = idx.get();
+    if (max_per_tile <= kTilePts) {
+      p.count = count.get();
+      p.slots = slots.get();
+    }
+    return p;
+  }
 };
 
 // Damping tables of the damped kernel for a padded grid and width nb.
@@ -119,19 +130,17 @@ class Propagator {
   // to a size heuristic.
   void set_tile_x(int tx);
   void set_tuning(bool on) { tune_ = on; }
-  // Forward time loops advance 2 steps per launch (temporal blocking) or 1.
-  void set_steps_per_launch(int n);
-  int steps_per_launch() const { return steps_per_launch_; }
   int tile_choice_cached(int kind, int E) const;
   // Microseconds per launch of `reps` back-to-back step kernels (kind 0
-  // forward, 1 backward, 2 two-step forward) of tile width kTileWidths[width] on E scratch
+  // forward, 1 backward) of tile width kTileWidths[width] on E scratch
   // wavefields of model rv (entry e uses rv + (e / rvdiv) * rvstride).
   float time_steps(int kind, int E, const float* rv, long long rvstride, int rvdiv, Fields& w, int width, int reps,
                    bool clear);
   bool graphs() const { return graphs_; }
-  // Per-launch HIP events around every stencil kernel (forward = 0,
-  // backward = 1), read back after each time loop into kernel_ms / kernel_n.
-  // Timed loops are launched directly, not from a graph.
+  // Timing: one HIP event pair around every forward (0) / backward (1) time
+  // loop on the handle's stream, graph replays included; collect_timing()
+  // waits for the loops in flight and adds them to kernel_ms / kernel_n
+  // (milliseconds, step launches).
   void set_timing(bool on);
   void collect_timing();
   double kernel_ms[2] = {0, 0};
@@ -142,16 +151,15 @@ class Propagator {
   Propagator& operator=(const Propagator&) = delete;
 
  private:
-  // one launch of a forward time loop: steps 1 (next over prev) or 2
-  // (p(it+1), p(it+2) into out1, out2)
+  // one launch of a forward time loop: step `it` writes next over prev
   struct FwdLaunch {
-    int it, steps;
-    float *cur, *prev, *out1, *out2;
+    int it;
+    float *cur, *prev;
   };
   // the launches of a forward run over fw's buffers; sets fw's final state
   std::vector<FwdLaunch> plan_forward(Fields& fw) const;
   void forward_body(int E, const float* rv, long long rvstride, const Sources& src, float* dcal, float* band,
-                    Fields& fw, int rvdiv, int w1, int w2);
+                    Fields& fw, int rvdiv, int w);
   void backward_body(int E, const float* rv, long long rvstride, const Sources& src, const float* vsrc,
                      long long vstride, const float* band, float* grad, long long gstride, Fields& fw, int rvdiv,
                      int it_min, const std::vector<float>& wts, int w);
@@ -160,30 +168,34 @@ class Propagator {
   // trial of every width on scratch wavefields (the tile tuner).
   int tile_choice(int kind, int E, const float* rv, long long rvstride, int rvdiv, Fields& w, bool may_time);
   static constexpr int kGraphMaxEntries = 8;
-  void run_maybe_graph(const std::vector<uintptr_t>& key, int E, const std::function<void()>& body);
-  template <typename F>
-  void timed_launch(int kind, F&& launch);
+  // Run a time loop (`launches` step launches of kind 0 forward / 1
+  // backward): captured into a HIP graph on first use and replayed when the
+  // same buffers come back, or launched directly.
+  void run_maybe_graph(const std::vector<uintptr_t>& key, int E, int kind, long long launches,
+                       const std::function<void()>& body);
+  hipEvent_t pooled_event();
 
   struct GraphEntry {
     std::vector<uintptr_t> key;
     hipGraphExec_t exec = nullptr;
-    std::vector<hipEvent_t> events[2];  // timing events captured into the graph
   };
-  void drop_graph(GraphEntry& ge);
-  void read_events(const std::vector<hipEvent_t>* ev);
+  struct TimedLoop {
+    hipEvent_t start, stop;
+    int kind;
+    long long launches;
+  };
   std::vector<GraphEntry> graph_cache_;
   bool graphs_ = false, timing_ = false;
-  std::vector<hipEvent_t> active_events_[2], event_pool_;
+  std::vector<TimedLoop> pending_;
+  std::vector<hipEvent_t> event_pool_;
 
   Geometry g_;
   int prec_;
   Stream stream_;
   DampTables damp_;
   DevPoints src_[3], rec_[3];    // points per tile
-  DevPoints srch_[3], rech_[3];  // points per tile grown by one step's reach (two-step kernels)
   int tile_override_ = 0;                 // 0 = tune, else a width from kTileWidths
   bool tune_ = false;
-  int steps_per_launch_ = 1;
   std::vector<std::pair<long long, int>> tile_cache_;  // (kind*2^32 + E) -> width index
   DBuf<int> sxp_, szp_, gxp_, gzp_;
 };

@@ -135,6 +135,13 @@ __device__ __forceinline__ float damping_delta(const Grid& g, int ix, int iz) {
   return iz >= g.zdamp ? g.delta_z[iz] : g.delta_x[ix];
 }
 
+// Whether any point of [x0, x1) x [z0, z1) lies in the sponge (delta != 0:
+// ix < nb, ix >= nx - nb, iz >= nz - nb).  Where delta is 0 the damped update
+// reduces exactly to the undamped one: (2 - 2*0) + 0*0 = 2 and 1 - 2*0 = 1.
+__device__ __forceinline__ bool touches_sponge(const Grid& g, int x0, int x1, int z0, int z1) {
+  return x0 < g.nb || x1 > g.nx - g.nb || z1 > g.zdamp;
+}
+
 // Block -> (tile, entry): XCD-contiguous runs, entries fastest.
 __device__ __forceinline__ void block_work(int ntiles, int nentries, int& tile, int& e) {
   const int n = ntiles * nentries;
@@ -147,8 +154,9 @@ __device__ __forceinline__ void block_work(int ntiles, int nentries, int& tile, 
 }
 
 // Stage cur (tile + halo) into LDS, zero outside the grid; all loads of the
-// thread are issued before the first LDS store.
-template <int TX>
+// thread are issued before the first LDS store.  FULL: the staged region lies
+// inside the grid (every tile but the last row and column), no checks.
+template <int TX, bool FULL>
 __device__ __forceinline__ void stage_tile(const Grid& g, const float* __restrict__ src, float* __restrict__ lds,
                                            int x0, int z0) {
   using T = Tile<TX>;
@@ -158,8 +166,8 @@ __device__ __forceinline__ void stage_tile(const Grid& g, const float* __restric
     const int i = threadIdx.x + k * kThreads;
     const int lx = i / T::LZ, lz = i - lx * T::LZ;
     const int gx = x0 - kHalo + lx, gz = z0 - kHalo + lz;
-    const bool ok = i < T::LX * T::LZ && gx >= 0 && gx < g.nx && gz >= 0 && gz < g.nz;
-    v[k] = ok ? src[(long long)gx * g.ld + gz] : 0.f;
+    const bool ok = i < T::LX * T::LZ && (FULL || (gx >= 0 && gx < g.nx && gz >= 0 && gz < g.nz));
+    v[k] = ok ? src[(unsigned)(gx * g.ld + gz)] : 0.f;
   }
 #pragma unroll
   for (int k = 0; k < T::NSTAGE; k++) {
@@ -170,7 +178,7 @@ __device__ __forceinline__ void stage_tile(const Grid& g, const float* __restric
 
 // u2 over the tile plus a one-cell ring, only where the reference defines it
 // ([5, n-5) in both directions); elsewhere 0 (never read by a valid output).
-template <int P, int TX>
+template <int P, int TX, bool FULL>
 __device__ __forceinline__ void form_u2(const Grid& g, const float* __restrict__ cur, float* __restrict__ u2,
                                         int x0, int z0) {
   using T = Tile<TX>;
@@ -181,10 +189,24 @@ __device__ __forceinline__ void form_u2(const Grid& g, const float* __restrict__
     const int ux = i / T::UZ, uz = i - ux * T::UZ;
     const int gx = x0 - 1 + ux, gz = z0 - 1 + uz;
     float v = 0.f;
-    if (gx >= 5 && gx < g.nx - 5 && gz >= 5 && gz < g.nz - 5)
+    if (FULL || (gx >= 5 && gx < g.nx - 5 && gz >= 5 && gz < g.nz - 5))
       v = u2_at<P, T::LZ>(cur + (ux + kHalo - 1) * T::LZ + (uz + kHalo - 1));
     u2[i] = v;
   }
+}
+
+// Whether the tile may hold points of the boundary band (a superset test;
+// band_index decides per point).
+template <int TX>
+__device__ __forceinline__ bool touches_band(const Grid& g, int x0, int z0) {
+  return x0 < g.bx0 || x0 + TX > g.nx - g.bxn || z0 + kTZ > g.nz - g.bzn;
+}
+
+// Whether the tile's staged region (tile + 6 cells) lies inside the grid:
+// then every load, u2 point and output of the tile is in range.
+template <int TX>
+__device__ __forceinline__ bool tile_is_full(const Grid& g, int x0, int z0) {
+  return x0 + TX + kHalo <= g.nx && z0 + kTZ + kHalo <= g.nz;
 }
 
 __device__ __forceinline__ bool entry_active(const Inject& in, int e, int pid) {
@@ -195,7 +217,6 @@ __device__ __forceinline__ bool entry_active(const Inject& in, int e, int pid) {
 // load depends on the stencil result: grid index (-1 = inactive for this
 // entry) and, for injections, the amplitude of this step.  Tiles with more
 // than kTilePts points fall back to scanning the global list.
-constexpr int kTilePts = 64;
 struct TilePts {
   int q[kTilePts];
   int pid[kTilePts];
@@ -211,21 +232,30 @@ struct PtsLoad {
 };
 __device__ __forceinline__ PtsLoad load_points(const PointList& pts, const Inject* in, int tile, int e) {
   PtsLoad r;
-  const int b = pts.tile_ptr[tile];
-  r.n = pts.tile_ptr[tile + 1] - b;
   r.q = -1;
   r.pid = 0;
   r.amp = 0.f;
-  if (r.n <= kTilePts && (int)threadIdx.x < r.n) {
-    const int pid = pts.items[b + threadIdx.x];
-    r.pid = pid;
-    r.q = pts.idx[pid];
-    if (in) {
-      if (entry_active(*in, e, pid))
-        r.amp = in->amp[e * in->estride + (long long)in->t * in->ld + pid];
-      else
-        r.q = -1;
+  if (pts.slots) {
+    // count and slot in parallel (slots past the count hold -1)
+    r.n = pts.count[tile];
+    if ((int)threadIdx.x < kTilePts) {
+      const int2 sl = pts.slots[(long long)tile * kTilePts + threadIdx.x];
+      r.q = sl.x;
+      r.pid = sl.y;
     }
+  } else {
+    const int b = pts.tile_ptr[tile];
+    r.n = pts.tile_ptr[tile + 1] - b;
+    if (r.n <= kTilePts && (int)threadIdx.x < r.n) {
+      r.pid = pts.items[b + threadIdx.x];
+      r.q = pts.idx[r.pid];
+    }
+  }
+  if (in && r.n <= kTilePts && (int)threadIdx.x < r.n) {
+    if (entry_active(*in, e, r.pid))
+      r.amp = in->amp[e * in->estride + (long long)in->t * in->ld + r.pid];
+    else
+      r.q = -1;
   }
   return r;
 }
@@ -237,7 +267,7 @@ __device__ __forceinline__ void store_points(const PtsLoad& r, TilePts& tp) {
   }
 }
 // Add or subtract, in point order, every staged amplitude that lands on q.
-__device__ __forceinline__ float apply_staged(const TilePts& tp, int n, int sign, long long q, float v) {
+__device__ __forceinline__ float apply_staged(const TilePts& tp, int n, int sign, int q, float v) {
   for (int j = 0; j < n; j++)
     if (tp.q[j] == q) v = sign > 0 ? v + tp.amp[j] : v - tp.amp[j];
   return v;
@@ -245,7 +275,7 @@ __device__ __forceinline__ float apply_staged(const TilePts& tp, int n, int sign
 
 // Apply all active point amplitudes of this tile that land on grid index q,
 // in point order (manipSource, damp4t10d.cpp:204-214).
-__device__ __forceinline__ float apply_points(const Inject& in, int tile, int e, long long q, float v) {
+__device__ __forceinline__ float apply_points(const Inject& in, int tile, int e, int q, float v) {
   const int b = in.pts.tile_ptr[tile], eidx = in.pts.tile_ptr[tile + 1];
   for (int j = b; j < eidx; j++) {
     const int pid = in.pts.items[j];
@@ -257,17 +287,11 @@ __device__ __forceinline__ float apply_points(const Inject& in, int tile, int e,
   return v;
 }
 
-template <bool DAMPED, int P, int TX>
-__global__ __launch_bounds__(kThreads) void k_fwd_step(FwdStepArgs a, int nentries) {
+template <bool DAMPED, int P, int TX, bool FULL>
+__device__ __forceinline__ void fwd_tile(const FwdStepArgs& a, int tile, int e, int x0, int z0, float* s_cur,
+                                         float* s_u2, TilePts& s_inj, TilePts& s_rec) {
   using T = Tile<TX>;
-  __shared__ float s_cur[T::LX * T::LZ];
-  __shared__ float s_u2[T::UX * T::UZ];
-  __shared__ TilePts s_inj, s_rec;
   const Grid& g = a.g;
-  int tile, e;
-  block_work(g.ntz * g.ntx, nentries, tile, e);
-  const int tz = tile % g.ntz, tx = tile / g.ntz;
-  const int z0 = kHalo + tz * kTZ, x0 = kHalo + tx * TX;
   const long long fo = (long long)e * a.fstride;
   float* __restrict__ prev = a.prev + fo;
   const float* __restrict__ rv = a.rv + (long long)(e / a.rvdiv) * a.rvstride;
@@ -278,48 +302,51 @@ __global__ __launch_bounds__(kThreads) void k_fwd_step(FwdStepArgs a, int nentri
   PtsLoad li{0, -1, 0, 0.f}, lr{0, -1, 0, 0.f};
   if (a.inj.t >= 0) li = load_points(a.inj.pts, &a.inj, tile, e);
   if (a.rec.out != nullptr) lr = load_points(a.rec.pts, nullptr, tile, e);
+  const bool sponge = DAMPED && touches_sponge(g, x0, x0 + TX, z0, z0 + kTZ);
+  const bool band_tile = a.band != nullptr && touches_band<TX>(g, x0, z0);
   float pv[T::PER], rr[T::PER], dl[T::PER];
 #pragma unroll
   for (int k = 0; k < T::PER; k++) {
     const int ix = x0 + row + kRows * k;
-    const bool ok = ix < g.nx - kHalo && iz < g.nz - kHalo;
-    const long long q = (long long)ix * g.ld + iz;
+    const bool ok = FULL || (ix < g.nx - kHalo && iz < g.nz - kHalo);
+    const unsigned q = (unsigned)(ix * g.ld + iz);
     pv[k] = ok ? prev[q] : 0.f;
     rr[k] = ok ? rv[q] : 1.f;
-    dl[k] = (DAMPED && ok) ? damping_delta(g, ix, iz) : 0.f;
+    dl[k] = (sponge && ok) ? damping_delta(g, ix, iz) : 0.f;
   }
-  stage_tile<TX>(g, a.cur + fo, s_cur, x0, z0);
+  stage_tile<TX, FULL>(g, a.cur + fo, s_cur, x0, z0);
   store_points(li, s_inj);
   store_points(lr, s_rec);
   __syncthreads();
-  form_u2<P, TX>(g, s_cur, s_u2, x0, z0);
+  form_u2<P, TX, FULL>(g, s_cur, s_u2, x0, z0);
   __syncthreads();
 
 #pragma unroll
   for (int k = 0; k < T::PER; k++) {
     const int lx = row + kRows * k;
     const int ix = x0 + lx;
-    if (ix >= g.nx - kHalo || iz >= g.nz - kHalo) continue;
-    const long long q = (long long)ix * g.ld + iz;
+    if (!FULL && (ix >= g.nx - kHalo || iz >= g.nz - kHalo)) continue;
+    const unsigned q = (unsigned)(ix * g.ld + iz);
     const float c = s_cur[(lx + kHalo) * T::LZ + lz + kHalo];
     const float* u = s_u2 + (lx + 1) * T::UZ + lz + 1;
     if (a.u2out) a.u2out[fo + q] = u[0];
-    float nxt = update_at<DAMPED, P, T::UZ>(c, pv[k], rr[k], u, dl[k]);
+    float nxt = sponge ? update_at<true, P, T::UZ>(c, pv[k], rr[k], u, dl[k])
+                       : update_at<false, P, T::UZ>(c, pv[k], rr[k], u, 0.f);
     if (li.n > 0)
       nxt = li.n <= kTilePts ? apply_staged(s_inj, li.n, a.inj.sign, q, nxt) : apply_points(a.inj, tile, e, q, nxt);
     prev[q] = nxt;
-    if (a.band) {
+    if (band_tile) {
       const int bi = band_index(g, ix, iz);
       if (bi >= 0) a.band[(long long)e * a.band_estride + bi] = c;
     }
     if (lr.n > 0 && lr.n <= kTilePts) {
       for (int j = 0; j < lr.n; j++)
-        if (s_rec.q[j] == q) a.rec.out[(long long)e * a.rec.estride + s_rec.pid[j]] = c;
+        if (s_rec.q[j] == (int)q) a.rec.out[(long long)e * a.rec.estride + s_rec.pid[j]] = c;
     } else if (lr.n > 0) {
       const int b = a.rec.pts.tile_ptr[tile], eidx = a.rec.pts.tile_ptr[tile + 1];
       for (int j = b; j < eidx; j++) {
         const int pid = a.rec.pts.items[j];
-        if (a.rec.pts.idx[pid] == q) a.rec.out[(long long)e * a.rec.estride + pid] = c;
+        if (a.rec.pts.idx[pid] == (int)q) a.rec.out[(long long)e * a.rec.estride + pid] = c;
       }
     }
   }
@@ -337,196 +364,26 @@ __global__ __launch_bounds__(kThreads) void k_fwd_step(FwdStepArgs a, int nentri
   }
 }
 
-// ---- two steps per launch -------------------------------------------------
-//
-// The first step is computed on the tile grown by one step's reach (6 cells),
-// so the second needs nothing from other blocks.  Regions, in cells relative
-// to the tile's first output point:
-//   S0  staged p(t)          [-12, T+12)   (T = TX along x, kTZ along z)
-//   U1  u2 of p(t)           [-7, T+7)
-//   R1  p(t+1) and the model [-6, T+6)
-//   U2  u2 of p(t+1)         [-1, T+1)     (aliases U1)
-// Points of R1 outside the update region [6, n-6) are 0, as in the fields.
-template <int TX>
-struct Tile2 {
-  static constexpr int LZ0 = kTZ + 4 * kHalo, LX0 = TX + 4 * kHalo;
-  static constexpr int UZ1 = kTZ + 2 * kHalo + 2, UX1 = TX + 2 * kHalo + 2;
-  static constexpr int RZ = kTZ + 2 * kHalo, RX = TX + 2 * kHalo;
-  static constexpr int UZ2 = kTZ + 2, UX2 = TX + 2;
-  static constexpr int NS0 = (LX0 * LZ0 + kThreads - 1) / kThreads;
-  static constexpr int NR = (RX * RZ + kThreads - 1) / kThreads;
-  static constexpr int PER = TX / kRows;
-};
-
-__device__ __forceinline__ bool in_update(const Grid& g, int gx, int gz) {
-  return gx >= kHalo && gx < g.nx - kHalo && gz >= kHalo && gz < g.nz - kHalo;
-}
-
-template <int P, int TX>
-__global__ __launch_bounds__(kThreads) void k_fwd_step2(Fwd2StepArgs a, int nentries) {
-  using T = Tile2<TX>;
-  __shared__ float s0[T::LX0 * T::LZ0];
-  __shared__ float su[T::UX1 * T::UZ1];  // U1, then U2
-  __shared__ float r1[T::RX * T::RZ];
-  __shared__ float rvl[T::RX * T::RZ];
-  __shared__ TilePts s_inj1, s_inj2, s_rec;
-  const Grid& g = a.g;
-  int tile, e;
-  block_work(g.ntz * g.ntx, nentries, tile, e);
-  const int tz = tile % g.ntz, tx = tile / g.ntz;
-  const int z0 = kHalo + tz * kTZ, x0 = kHalo + tx * TX;
-  const long long fo = (long long)e * a.fstride;
-  const float* __restrict__ prev = a.prev + fo;
-  const float* __restrict__ rv = a.rv + (long long)(e / a.rvdiv) * a.rvstride;
-
-  // every global load of the block is issued here, before the first barrier
-  PtsLoad l1{0, -1, 0, 0.f}, l2{0, -1, 0, 0.f}, lr{0, -1, 0, 0.f};
-  if (a.inj1.t >= 0) l1 = load_points(a.inj1.pts, &a.inj1, tile, e);
-  if (a.inj2.t >= 0) l2 = load_points(a.inj2.pts, &a.inj2, tile, e);
-  if (a.rec.out != nullptr) lr = load_points(a.rec.pts, nullptr, tile, e);
-  float pv[T::NR], rr[T::NR], dl[T::NR];
-#pragma unroll
-  for (int k = 0; k < T::NR; k++) {
-    const int i = threadIdx.x + k * kThreads;
-    const int rx = i / T::RZ, rz = i - rx * T::RZ;
-    const int gx = x0 - kHalo + rx, gz = z0 - kHalo + rz;
-    const bool ok = i < T::RX * T::RZ && in_update(g, gx, gz);
-    const long long q = (long long)gx * g.ld + gz;
-    pv[k] = ok ? prev[q] : 0.f;
-    rr[k] = ok ? rv[q] : 1.f;
-    dl[k] = ok ? damping_delta(g, gx, gz) : 0.f;
-  }
-  const int lz = threadIdx.x % kTZ, row = threadIdx.x / kTZ;
-  const int iz = z0 + lz;
-  float dl2[T::PER];
-#pragma unroll
-  for (int k = 0; k < T::PER; k++) {
-    const int ix = x0 + row + kRows * k;
-    dl2[k] = (ix < g.nx - kHalo && iz < g.nz - kHalo) ? damping_delta(g, ix, iz) : 0.f;
-  }
-  {
-    const float* __restrict__ cur = a.cur + fo;
-    float v[T::NS0];
-#pragma unroll
-    for (int k = 0; k < T::NS0; k++) {
-      const int i = threadIdx.x + k * kThreads;
-      const int lx = i / T::LZ0, l = i - lx * T::LZ0;
-      const int gx = x0 - 2 * kHalo + lx, gz = z0 - 2 * kHalo + l;
-      const bool ok = i < T::LX0 * T::LZ0 && gx >= 0 && gx < g.nx && gz >= 0 && gz < g.nz;
-      v[k] = ok ? cur[(long long)gx * g.ld + gz] : 0.f;
-    }
-#pragma unroll
-    for (int k = 0; k < T::NS0; k++) {
-      const int i = threadIdx.x + k * kThreads;
-      if (i < T::LX0 * T::LZ0) s0[i] = v[k];
-    }
-  }
-  store_points(l1, s_inj1);
-  store_points(l2, s_inj2);
-  store_points(lr, s_rec);
-  __syncthreads();
-
-  // u2 of p(t) over U1
-  for (int i = threadIdx.x; i < T::UX1 * T::UZ1; i += kThreads) {
-    const int ux = i / T::UZ1, uz = i - ux * T::UZ1;
-    const int gx = x0 - kHalo - 1 + ux, gz = z0 - kHalo - 1 + uz;
-    float v = 0.f;
-    if (gx >= 5 && gx < g.nx - 5 && gz >= 5 && gz < g.nz - 5) v = u2_at<P, T::LZ0>(s0 + (ux + 5) * T::LZ0 + uz + 5);
-    su[i] = v;
-  }
-  __syncthreads();
-
-  // p(t+1) over R1, with the sources of step t+1
-#pragma unroll
-  for (int k = 0; k < T::NR; k++) {
-    const int i = threadIdx.x + k * kThreads;
-    if (i >= T::RX * T::RZ) break;
-    const int rx = i / T::RZ, rz = i - rx * T::RZ;
-    const int gx = x0 - kHalo + rx, gz = z0 - kHalo + rz;
-    float v = 0.f;
-    if (in_update(g, gx, gz)) {
-      const float c = s0[(rx + kHalo) * T::LZ0 + rz + kHalo];
-      v = update_at<true, P, T::UZ1>(c, pv[k], rr[k], su + (rx + 1) * T::UZ1 + rz + 1, dl[k]);
-      if (l1.n > 0) {
-        const long long q = (long long)gx * g.ld + gz;
-        v = l1.n <= kTilePts ? apply_staged(s_inj1, l1.n, a.inj1.sign, q, v) : apply_points(a.inj1, tile, e, q, v);
-      }
-    }
-    r1[i] = v;
-    rvl[i] = rr[k];
-  }
-  __syncthreads();
-
-  // u2 of p(t+1) over U2 (U1 is dead)
-  for (int i = threadIdx.x; i < T::UX2 * T::UZ2; i += kThreads) {
-    const int ux = i / T::UZ2, uz = i - ux * T::UZ2;
-    const int gx = x0 - 1 + ux, gz = z0 - 1 + uz;
-    float v = 0.f;
-    if (gx >= 5 && gx < g.nx - 5 && gz >= 5 && gz < g.nz - 5) v = u2_at<P, T::RZ>(r1 + (ux + 5) * T::RZ + uz + 5);
-    su[i] = v;
-  }
-  __syncthreads();
-
-  // p(t+2) over the tile; both new fields, records and band rows out
-  float* __restrict__ out1 = a.out1 + fo;
-  float* __restrict__ out2 = a.out2 + fo;
-#pragma unroll
-  for (int k = 0; k < T::PER; k++) {
-    const int lx = row + kRows * k;
-    const int ix = x0 + lx;
-    if (ix >= g.nx - kHalo || iz >= g.nz - kHalo) continue;
-    const long long q = (long long)ix * g.ld + iz;
-    const int lr1 = (lx + kHalo) * T::RZ + lz + kHalo;
-    const float p0 = s0[(lx + 2 * kHalo) * T::LZ0 + lz + 2 * kHalo];
-    const float p1 = r1[lr1];
-    float p2 = update_at<true, P, T::UZ2>(p1, p0, rvl[lr1], su + (lx + 1) * T::UZ2 + lz + 1, dl2[k]);
-    if (l2.n > 0)
-      p2 = l2.n <= kTilePts ? apply_staged(s_inj2, l2.n, a.inj2.sign, q, p2) : apply_points(a.inj2, tile, e, q, p2);
-    out1[q] = p1;
-    out2[q] = p2;
-    if (a.band) {
-      const int bi = band_index(g, ix, iz);
-      if (bi >= 0) {
-        float* b = a.band + (long long)e * a.band_estride + bi;
-        b[0] = p0;
-        b[a.band_step] = p1;
-      }
-    }
-    if (lr.n > 0) {
-      float* ro = a.rec.out + (long long)e * a.rec.estride;
-      if (lr.n <= kTilePts) {
-        for (int j = 0; j < lr.n; j++)
-          if (s_rec.q[j] == q) {
-            ro[s_rec.pid[j]] = p0;
-            ro[a.rec_step + s_rec.pid[j]] = p1;
-          }
-      } else {
-        const int b = a.rec.pts.tile_ptr[tile], eidx = a.rec.pts.tile_ptr[tile + 1];
-        for (int j = b; j < eidx; j++) {
-          const int pid = a.rec.pts.items[j];
-          if (a.rec.pts.idx[pid] == q) {
-            ro[pid] = p0;
-            ro[a.rec_step + pid] = p1;
-          }
-        }
-      }
-    }
-  }
-}
-
-template <int P, int TX>
-__global__ __launch_bounds__(kThreads) void k_bwd_step(BwdStepArgs a, int nentries) {
+template <bool DAMPED, int P, int TX>
+__global__ __launch_bounds__(kThreads) void k_fwd_step(FwdStepArgs a, int nentries) {
   using T = Tile<TX>;
-  __shared__ float s_sc[T::LX * T::LZ];
-  __shared__ float s_gc[T::LX * T::LZ];
-  __shared__ float s_su[T::UX * T::UZ];
-  __shared__ float s_gu[T::UX * T::UZ];
-  __shared__ TilePts s_sub, s_inj;
-  const Grid& g = a.g;
+  __shared__ float s_cur[T::LX * T::LZ];
+  __shared__ float s_u2[T::UX * T::UZ];
+  __shared__ TilePts s_inj, s_rec;
   int tile, e;
-  block_work(g.ntz * g.ntx, nentries, tile, e);
-  const int tz = tile % g.ntz, tx = tile / g.ntz;
-  const int z0 = kHalo + tz * kTZ, x0 = kHalo + tx * TX;
+  block_work(a.g.ntz * a.g.ntx, nentries, tile, e);
+  const int z0 = kHalo + (tile % a.g.ntz) * kTZ, x0 = kHalo + (tile / a.g.ntz) * TX;
+  if (tile_is_full<TX>(a.g, x0, z0))
+    fwd_tile<DAMPED, P, TX, true>(a, tile, e, x0, z0, s_cur, s_u2, s_inj, s_rec);
+  else
+    fwd_tile<DAMPED, P, TX, false>(a, tile, e, x0, z0, s_cur, s_u2, s_inj, s_rec);
+}
+
+template <int P, int TX, bool FULL>
+__device__ __forceinline__ void bwd_tile(const BwdStepArgs& a, int tile, int e, int x0, int z0, float* s_sc,
+                                         float* s_gc, float* s_su, float* s_gu, TilePts& s_sub, TilePts& s_inj) {
+  using T = Tile<TX>;
+  const Grid& g = a.g;
   const long long fo = (long long)e * a.fstride;
   const float* __restrict__ rv = a.rv + (long long)(e / a.rvdiv) * a.rvstride;
   float* __restrict__ s_prev = a.s_prev + fo;
@@ -539,37 +396,39 @@ __global__ __launch_bounds__(kThreads) void k_bwd_step(BwdStepArgs a, int nentri
   PtsLoad ls{0, -1, 0, 0.f}, li{0, -1, 0, 0.f};
   if (a.s_sub.t >= 0) ls = load_points(a.s_sub.pts, &a.s_sub, tile, e);
   if (a.g_inj.t >= 0) li = load_points(a.g_inj.pts, &a.g_inj, tile, e);
+  const bool sponge = touches_sponge(g, x0, x0 + TX, z0, z0 + kTZ);
+  const bool band_tile = a.band != nullptr && touches_band<TX>(g, x0, z0);
   float sp[T::PER], gp[T::PER], rr[T::PER], gr[T::PER], dl[T::PER], bv[T::PER];
   unsigned inband = 0;
 #pragma unroll
   for (int k = 0; k < T::PER; k++) {
     const int ix = x0 + row + kRows * k;
-    const bool ok = ix < g.nx - kHalo && iz < g.nz - kHalo;
-    const long long q = (long long)ix * g.ld + iz;
+    const bool ok = FULL || (ix < g.nx - kHalo && iz < g.nz - kHalo);
+    const unsigned q = (unsigned)(ix * g.ld + iz);
     sp[k] = ok ? s_prev[q] : 0.f;
     gp[k] = ok ? g_prev[q] : 0.f;
     rr[k] = ok ? rv[q] : 1.f;
     gr[k] = (ok && a.do_xcorr) ? grad[q] : 0.f;
-    dl[k] = ok ? damping_delta(g, ix, iz) : 0.f;
-    const int bi = (ok && a.band) ? band_index(g, ix, iz) : -1;
+    dl[k] = (sponge && ok) ? damping_delta(g, ix, iz) : 0.f;
+    const int bi = (ok && band_tile) ? band_index(g, ix, iz) : -1;
     bv[k] = bi >= 0 ? a.band[(long long)e * a.band_estride + bi] : 0.f;
     if (bi >= 0) inband |= 1u << k;
   }
-  stage_tile<TX>(g, a.s_cur + fo, s_sc, x0, z0);
-  stage_tile<TX>(g, a.g_cur + fo, s_gc, x0, z0);
+  stage_tile<TX, FULL>(g, a.s_cur + fo, s_sc, x0, z0);
+  stage_tile<TX, FULL>(g, a.g_cur + fo, s_gc, x0, z0);
   store_points(ls, s_sub);
   store_points(li, s_inj);
   __syncthreads();
-  form_u2<P, TX>(g, s_sc, s_su, x0, z0);
-  form_u2<P, TX>(g, s_gc, s_gu, x0, z0);
+  form_u2<P, TX, FULL>(g, s_sc, s_su, x0, z0);
+  form_u2<P, TX, FULL>(g, s_gc, s_gu, x0, z0);
   __syncthreads();
 
 #pragma unroll
   for (int k = 0; k < T::PER; k++) {
     const int lx = row + kRows * k;
     const int ix = x0 + lx;
-    if (ix >= g.nx - kHalo || iz >= g.nz - kHalo) continue;
-    const long long q = (long long)ix * g.ld + iz;
+    if (!FULL && (ix >= g.nx - kHalo || iz >= g.nz - kHalo)) continue;
+    const unsigned q = (unsigned)(ix * g.ld + iz);
     const int l = (lx + kHalo) * T::LZ + lz + kHalo;
     const int lu = (lx + 1) * T::UZ + lz + 1;
     // source field: stepBackward (undamped) then subSource
@@ -578,7 +437,8 @@ __global__ __launch_bounds__(kThreads) void k_bwd_step(BwdStepArgs a, int nentri
       sn = ls.n <= kTilePts ? apply_staged(s_sub, ls.n, a.s_sub.sign, q, sn) : apply_points(a.s_sub, tile, e, q, sn);
     // receiver field: stepForward (damped) of the already injected cur
     const float gc = s_gc[l];
-    float gn = update_at<true, P, T::UZ>(gc, gp[k], rr[k], s_gu + lu, dl[k]);
+    float gn = sponge ? update_at<true, P, T::UZ>(gc, gp[k], rr[k], s_gu + lu, dl[k])
+                      : update_at<false, P, T::UZ>(gc, gp[k], rr[k], s_gu + lu, 0.f);
     // zero-lag imaging condition with the new source field and the injected
     // receiver field (cross_correlation, fwiframework.cpp:107-112)
     if (a.do_xcorr) grad[q] = gr[k] - sn * gc * a.weight;
@@ -590,6 +450,23 @@ __global__ __launch_bounds__(kThreads) void k_bwd_step(BwdStepArgs a, int nentri
     s_prev[q] = sn;
     g_prev[q] = gn;
   }
+}
+
+template <int P, int TX>
+__global__ __launch_bounds__(kThreads) void k_bwd_step(BwdStepArgs a, int nentries) {
+  using T = Tile<TX>;
+  __shared__ float s_sc[T::LX * T::LZ];
+  __shared__ float s_gc[T::LX * T::LZ];
+  __shared__ float s_su[T::UX * T::UZ];
+  __shared__ float s_gu[T::UX * T::UZ];
+  __shared__ TilePts s_sub, s_inj;
+  int tile, e;
+  block_work(a.g.ntz * a.g.ntx, nentries, tile, e);
+  const int z0 = kHalo + (tile % a.g.ntz) * kTZ, x0 = kHalo + (tile / a.g.ntz) * TX;
+  if (tile_is_full<TX>(a.g, x0, z0))
+    bwd_tile<P, TX, true>(a, tile, e, x0, z0, s_sc, s_gc, s_su, s_gu, s_sub, s_inj);
+  else
+    bwd_tile<P, TX, false>(a, tile, e, x0, z0, s_sc, s_gc, s_su, s_gu, s_sub, s_inj);
 }
 
 __global__ void k_fill(float* p, long long n, float v) {
@@ -650,24 +527,7 @@ void bwd_dispatch(const BwdStepArgs& a, int E, int precision, hipStream_t s) {
     hipLaunchKernelGGL((k_bwd_step<kFp32, TX>), grid, dim3(kThreads), 0, s, a, E);
 }
 
-template <int TX>
-void fwd2_dispatch(const Fwd2StepArgs& a, int E, int precision, hipStream_t s) {
-  const dim3 grid(a.g.ntz * a.g.ntx * E);
-  if (precision == kExact)
-    hipLaunchKernelGGL((k_fwd_step2<kExact, TX>), grid, dim3(kThreads), 0, s, a, E);
-  else
-    hipLaunchKernelGGL((k_fwd_step2<kFp32, TX>), grid, dim3(kThreads), 0, s, a, E);
-}
-
 }  // namespace
-
-void launch_fwd_step2(const Fwd2StepArgs& a, int nentries, int precision, hipStream_t stream) {
-  switch (a.g.tile_x) {
-    case 8: fwd2_dispatch<8>(a, nentries, precision, stream); break;
-    case 32: fwd2_dispatch<32>(a, nentries, precision, stream); break;
-    default: fwd2_dispatch<16>(a, nentries, precision, stream); break;
-  }
-}
 
 int tile_count_x(int nx, int tile_x) { return (nx - 2 * kHalo + tile_x - 1) / tile_x; }
 

@@ -131,16 +131,13 @@ def test_calgradient_bitexact(oracle, small, shot):
     assert np.array_equal(bits(g_gpu), bits(g_cpu))
 
 
-@pytest.mark.parametrize("steps", [1, 2])
 @pytest.mark.parametrize("tile_x", [8, 16, 32])
-def test_every_tile_width_is_bitexact(oracle, small, tile_x, steps):
-    """Each instantiated tile width (the tuner may pick any), with one or two
-    forward steps per launch, reproduces the oracle's seismogram and gradient
-    bit for bit."""
+def test_every_tile_width_is_bitexact(oracle, small, tile_x):
+    """Each instantiated tile width reproduces the oracle's seismogram and
+    gradient bit for bit."""
     case, s, g, vt, vi, wlt = small
-    print(f"tile_x={tile_x} steps={steps}", flush=True)
+    print(f"tile_x={tile_x}", flush=True)
     s.set_tile_x(tile_x)
-    s.set_steps_per_launch(steps)
     try:
         d_gpu = s.forward_modeling(wlt, shot=0)
         assert s.tile_x(0, 1) == tile_x
@@ -152,29 +149,30 @@ def test_every_tile_width_is_bitexact(oracle, small, tile_x, steps):
         assert np.array_equal(bits(dobs[case.ns - 1]), bits(g.forward_modeling(vt, wlt, shot=case.ns - 1)))
     finally:
         s.set_tile_x(0)
-        s.set_steps_per_launch(1)
 
 
 @pytest.mark.parametrize("nt", [361, 362])
-def test_two_step_launches_odd_and_even_nt(oracle, nt):
-    """Two steps per launch with a single step closing an odd nt: the ESS
-    seismogram and gradient stay bit-identical, with sources near tile edges
-    (so a grown tile injects its neighbours' sources)."""
+def test_sources_near_tile_edges(oracle, nt):
+    """Sources and receivers falling near tile edges, on a grid whose last
+    tile row and column are partial: the ESS seismogram and gradient stay
+    bit-identical for every tile width."""
     case = workloads.small_case(nx=70, nz=50, nb=8, nt=nt, ns=5)
     s, g, vt, vi, wlt = setup_case(oracle, case)
     s.bind_velocity(vt)
-    s.set_steps_per_launch(2)
     code = capi.random_codes(3, case.ns)
     enc = oracle.encode_source(code, wlt)
-    print(f"two-step nt={nt}", flush=True)
-    assert np.array_equal(bits(s.forward_modeling(enc, shot=-1)), bits(g.forward_modeling(vt, enc, shot=-1)))
+    print(f"tile edges nt={nt}", flush=True)
     rng = np.random.default_rng(nt)
     vsrc = rng.standard_normal((case.nt, case.ng)).astype(np.float32)
-    for tx in (8, 16, 32):
-        s.set_tile_x(tx)
-        got = s.calgradient(enc, vsrc, shot=-1)
-        assert np.array_equal(bits(got), bits(g.calgradient(vt, enc, vsrc, shot=-1))), tx
-    s.set_tile_x(0)
+    want_d = g.forward_modeling(vt, enc, shot=-1)
+    want_g = g.calgradient(vt, enc, vsrc, shot=-1)
+    try:
+        for tx in (8, 16, 32):
+            s.set_tile_x(tx)
+            assert np.array_equal(bits(s.forward_modeling(enc, shot=-1)), bits(want_d)), tx
+            assert np.array_equal(bits(s.calgradient(enc, vsrc, shot=-1)), bits(want_g)), tx
+    finally:
+        s.set_tile_x(0)
 
 
 def test_fp32_mode_within_budget(oracle, small):
