@@ -25,7 +25,7 @@ void check(int rc, const char* what) {
 float vel_recover(float v, float dx, float dt) { return std::sqrt(dx * dx / (dt * dt * v)); }
 float vel_trans(float v, float dx, float dt) { return dx * dx / (dt * dt * v * v); }
 
-// slownessL1L2Norm (main-enfwi-damp.cpp:367-402), a log line only: relative
+// slownessL1L2Norm (main-enfwi-damp.cpp:368-402), a log line only: relative
 // slowness misfit of a member against the real model, as written -- the
 // padded models indexed with the unpadded nz over ix in [nb, nx-nb),
 // iz in [0, nz-nb), FP32 sums.
@@ -85,7 +85,7 @@ int main(int argc, char** argv) {
   if (!shots.histfloat("vmin", vmin)) rsf::die("no vmin");
   if (!shots.histfloat("vmax", vmax)) rsf::die("no vmax");
   if (niterenkf < 1) rsf::die("niterenkf must be >= 1");
-  // Params::check (main-enfwi-damp.cpp:163-174)
+  // Params::check (main-enfwi-damp.cpp:169-179)
   if (!(sxbeg >= 0 && szbeg >= 0 && sxbeg + (ns - 1) * jsx < nx && szbeg + (ns - 1) * jsz < nz))
     rsf::die("sources exceeds the computing zone!");
   if (!(gxbeg >= 0 && gzbeg >= 0 && gxbeg + (ng - 1) * jgx < nx && gzbeg + (ng - 1) * jgz < nz))

@@ -36,6 +36,9 @@ struct PointList {
   // when no tile holds more than kTilePts points, else null (CSR only).
   const int* count = nullptr;     // [ntiles] points per tile
   const int2* slots = nullptr;    // [ntiles][kTilePts] (grid index, point id); grid index -1 past count
+  // points sharing a grid index, chained in id order (-1 ends a chain)
+  const int* prev_same = nullptr; // [npoints]
+  const int* next_same = nullptr; // [npoints]
 };
 
 struct Grid {

@@ -1,6 +1,7 @@
 #include "propagator.h"
 
 #include <algorithm>
+#include <map>
 #include <numeric>
 #include <string>
 
@@ -80,8 +81,21 @@ void build_points(const std::vector<long long>& idx, int ld, int ntz, int ntx, i
     ptr[t + 1] = ptr[t] + (int)lists[t].size();
     items.insert(items.end(), lists[t].begin(), lists[t].end());
   }
-  std::vector<int> idx32(n);
+  std::vector<int> idx32(n), prev_same(n, -1), next_same(n, -1);
   for (int p = 0; p < n; p++) idx32[p] = (int)idx[p];
+  {
+    std::map<int, int> last;  // grid index -> last point id seen there
+    for (int p = 0; p < n; p++) {
+      auto it = last.find(idx32[p]);
+      if (it != last.end()) {
+        prev_same[p] = it->second;
+        next_same[it->second] = p;
+      }
+      last[idx32[p]] = p;
+    }
+  }
+  out.prev_same.upload(prev_same, s);
+  out.next_same.upload(next_same, s);
   out.tile_ptr.upload(ptr, s);
   out.items.upload(items, s);
   out.idx.upload(idx32, s);

@@ -37,7 +37,7 @@ struct Geometry {
 
 // Per-tile CSR of a point set on the device.
 struct DevPoints {
-  DBuf<int> tile_ptr, items, idx, count;
+  DBuf<int> tile_ptr, items, idx, count, prev_same, next_same;
   DBuf<int2> slots;
   int n = 0;
   int max_per_tile = 0;
@@ -46,6 +46,8 @@ struct DevPoints {
     p.tile_ptr = tile_ptr.get();
     p.items = items.get();
     p.idx = idx.get();
+    p.prev_same = prev_same.get();
+    p.next_same = next_same.get();
     if (max_per_tile <= kTilePts) {
       p.count = count.get();
       p.slots = slots.get();

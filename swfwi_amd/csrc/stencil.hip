@@ -478,18 +478,22 @@ __global__ void k_fill_d(double* p, long long n, double v) {
     p[i] = v;
 }
 
-// Sequential per entry (one lane each), so coincident points accumulate in
-// the reference's order (manipSource, damp4t10d.cpp:204-214).
+// One lane per (point, entry).  Points sharing a grid index accumulate in
+// the reference's order (manipSource, damp4t10d.cpp:204-214): the first
+// active point of each chain applies the whole chain in id order.
 __global__ void k_point_inject(float* field, long long fstride, Inject in, int npoints, int nentries) {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nentries) return;
+  const int pid = blockIdx.x * blockDim.x + threadIdx.x, e = blockIdx.y;
+  if (pid >= npoints || e >= nentries) return;
+  if (!entry_active(in, e, pid)) return;
+  for (int j = in.pts.prev_same[pid]; j >= 0; j = in.pts.prev_same[j])
+    if (entry_active(in, e, j)) return;  // an earlier active point leads this chain
   float* f = field + (long long)e * fstride;
-  const int lo = in.lo ? in.lo[e] : 0, hi = in.hi ? in.hi[e] : npoints;
-  for (int pid = lo; pid < hi; pid++) {
-    const float a = in.amp[e * in.estride + (long long)in.t * in.ld + pid];
-    const long long q = in.pts.idx[pid];
-    f[q] = in.sign > 0 ? f[q] + a : f[q] - a;
-  }
+  const float* amp = in.amp + e * in.estride + (long long)in.t * in.ld;
+  const int q = in.pts.idx[pid];
+  float v = f[q];
+  for (int j = pid; j >= 0; j = in.pts.next_same[j])
+    if (entry_active(in, e, j)) v = in.sign > 0 ? v + amp[j] : v - amp[j];
+  f[q] = v;
 }
 
 __global__ void k_reciprocal(const float* vel, float* rv, long long n) {
@@ -556,8 +560,8 @@ void launch_fill_d(double* p, long long n, double v, hipStream_t s) {
 void launch_point_inject(float* field, long long fstride, const Inject& inj, int npoints, int nentries,
                          hipStream_t s) {
   if (inj.t < 0 || npoints <= 0 || nentries <= 0) return;
-  hipLaunchKernelGGL(k_point_inject, dim3((nentries + 63) / 64), dim3(64), 0, s, field, fstride, inj, npoints,
-                     nentries);
+  hipLaunchKernelGGL(k_point_inject, dim3((npoints + 255) / 256, nentries), dim3(256), 0, s, field, fstride, inj,
+                     npoints, nentries);
 }
 void launch_reciprocal(const float* vel, float* rv, long long n, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_reciprocal, dim3(grid_for(n)), dim3(256), 0, s, vel, rv, n);

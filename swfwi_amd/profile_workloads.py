@@ -4,6 +4,7 @@
     python -m swfwi_amd.profile_workloads shots     # 60-shot batched forward, 301x921, nt=400
     python -m swfwi_amd.profile_workloads ess       # ESS epochs, 301x921, nt=1000 (--no-graphs for counters)
     python -m swfwi_amd.profile_workloads enkf      # EnKF analyses, 301x921, N members
+    python -m swfwi_amd.profile_workloads enfwi     # ensemble-Kalman FWI epochs, 301x921, N members
     python -m swfwi_amd.profile_workloads sweep     # step-kernel time per tile width and batch size
 """
 import argparse
@@ -112,9 +113,56 @@ def run_enkf(nt, members):
               f"clip={info['clip']} sigma={info['sigma']:.3e}", flush=True)
 
 
+def run_enfwi(nt, members, epochs):
+    """Ensemble-Kalman FWI on the 301x921 grid: `members` velocity members
+    (initial model + 3% Gaussian random fields, correlation ~20 cells),
+    observed data with 1% RMS Gaussian noise; the initial analysis, then
+    `epochs` batched member epochs with an analysis after each."""
+    from scipy.ndimage import gaussian_filter
+
+    from .enkf import Enkf, EnsembleFwi
+
+    case = workloads.ess_301x921(nt=nt)
+    vtrue, v0 = workloads.models(case)
+    rng = np.random.default_rng(2)
+    mem = []
+    for _ in range(members):
+        f = gaussian_filter(rng.standard_normal(v0.shape), 20.0)
+        f = f / f.std() * 0.03
+        mem.append(capi.expand_domain((v0 * (1 + f)).astype(np.float32), case.nb, case.dx, case.dt))
+    wlt = capi.ricker(case.nt, case.fm, case.dt, case.amp)
+    s = capi.Solver(case.nx, case.nz, case.nb, case.dt, case.dx, case.fm, case.nt, case.sx, case.sz, case.gx,
+                    case.gz)
+    s.bind_velocity(capi.expand_domain(vtrue, case.nb, case.dx, case.dt))
+    dobs = s.forward_shots(wlt)
+    rms = float(np.sqrt(np.mean(dobs.astype(np.float64) ** 2)))
+    dobs = (dobs + rng.standard_normal(dobs.shape) * 0.01 * rms).astype(np.float32)
+    e = Enkf(s, wlt, dobs, 0.2, members)
+    e.set_members(np.stack(mem))
+    fwi = EnsembleFwi(e, case.vmin, case.vmax, 5, 200.0)
+    t0 = time.perf_counter()
+    e.analyze(None, seed=0)
+    print(f"initial analysis {1e3 * (time.perf_counter() - t0):.0f} ms, mean-model objective "
+          f"{fwi.mean_objective():.4e}", flush=True)
+    for it in range(epochs):
+        pu0 = sum(s.counters()[0])
+        t0 = time.perf_counter()
+        fwi.epoch(it)
+        t1 = time.perf_counter()
+        pu = sum(s.counters()[0]) - pu0
+        st = fwi.stats()
+        e.analyze(None, seed=it + 1)
+        t2 = time.perf_counter()
+        steps = [m["steplen"] for m in st["members"]]
+        print(f"epoch {it}: N={members} {1e3 * (t1 - t0):.0f} ms ({pu / (t1 - t0) / 1e9:.1f} Gpts/s; gradients "
+              f"{st['grad_ms']:.0f} ms, line search {st['linesearch_ms']:.0f} ms in {st['rounds']} rounds / "
+              f"{st['trials']} trial runs), analysis {1e3 * (t2 - t1):.0f} ms, mean-model objective "
+              f"{fwi.mean_objective():.4e}, steps {min(steps):.3g}..{max(steps):.3g}", flush=True)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["copy", "shots", "ess", "enkf", "sweep"])
+    ap.add_argument("mode", choices=["copy", "shots", "ess", "enkf", "enfwi", "sweep"])
     ap.add_argument("--entries", default="1,2,3,4,8,16,60,100,200")
     ap.add_argument("--reps", type=int, default=100)
     ap.add_argument("--precision", choices=["exact", "fp32"], default="exact")
@@ -134,6 +182,8 @@ def main(argv=None):
     elif a.mode == "sweep":
         run_sweep([int(x) for x in a.entries.split(",")], a.reps,
                   capi.PREC_FP32 if a.precision == "fp32" else capi.PREC_EXACT)
+    elif a.mode == "enfwi":
+        run_enfwi(a.nt or 4000, a.members, a.epochs)
     elif a.mode == "enkf":
         run_enkf(a.nt or 4000, a.members)
     else:

@@ -175,6 +175,29 @@ def test_sources_near_tile_edges(oracle, nt):
         s.set_tile_x(0)
 
 
+def test_coincident_sources_accumulate_in_id_order(oracle):
+    """Several coded sources on one grid point (and receivers doubled up):
+    the injected amplitudes add in source-id order, as the reference's
+    sequential loop does, at step 0 and inside the step kernels."""
+    from swfwi_amd.workloads import Case
+    nx, nz = 40, 30
+    sx = np.array([10, 10, 22, 10, 22], np.int32)
+    gx = np.concatenate([np.arange(nx), [5, 5]]).astype(np.int32)
+    case = Case("coincident", nz, nx, 10.0, 8, 0.001, 340, 25.0, sx, np.ones_like(sx), gx,
+                np.zeros(gx.size, np.int32), 1500.0, 3500.0, seed=9)
+    s, g, vt, vi, wlt = setup_case(oracle, case)
+    s.bind_velocity(vt)
+    code = capi.random_codes(4, case.ns)
+    enc = oracle.encode_source(code, wlt) * np.float32(1.37)
+    enc[:, 3] *= np.float32(0.3)
+    assert np.array_equal(bits(s.forward_modeling(enc, shot=-1)), bits(g.forward_modeling(vt, enc, shot=-1)))
+    rng = np.random.default_rng(3)
+    vsrc = rng.standard_normal((case.nt, case.ng)).astype(np.float32)
+    assert np.array_equal(bits(s.calgradient(enc, vsrc, shot=-1)), bits(g.calgradient(vt, enc, vsrc, shot=-1)))
+    for shot in (0, 3):
+        assert np.array_equal(bits(s.forward_modeling(wlt, shot=shot)), bits(g.forward_modeling(vt, wlt, shot=shot)))
+
+
 def test_fp32_mode_within_budget(oracle, small):
     case, s, g, vt, vi, wlt = small
     s.set_precision(capi.PREC_FP32)
