@@ -266,11 +266,41 @@ __device__ __forceinline__ void store_points(const PtsLoad& r, TilePts& tp) {
     tp.amp[threadIdx.x] = r.amp;
   }
 }
-// Add or subtract, in point order, every staged amplitude that lands on q.
-__device__ __forceinline__ float apply_staged(const TilePts& tp, int n, int sign, int q, float v) {
-  for (int j = 0; j < n; j++)
+// Per-tile map of staged injection points: for each output point of the
+// tile, the first staged slot whose point lands there (-1: none), so an
+// output point scans the slot list only when something is injected into it.
+// mark_clear: all threads, before the barrier that publishes the staged
+// slots; mark_slots: slot threads, after it.
+template <int TX>
+__device__ __forceinline__ void mark_clear(signed char* map) {
+  for (int i = threadIdx.x; i < kTZ * TX / 4; i += kThreads) reinterpret_cast<int*>(map)[i] = -1;
+}
+template <int TX>
+__device__ __forceinline__ void mark_slots(const TilePts& tp, int n, const Grid& g, int x0, int z0, signed char* map) {
+  const int j = threadIdx.x;
+  if (j >= n) return;
+  const int q = tp.q[j];
+  if (q < 0) return;
+  for (int k = 0; k < j; k++)
+    if (tp.q[k] == q) return;  // an earlier slot heads this point's chain
+  const int ix = q / g.ld, iz = q - ix * g.ld;
+  map[(ix - x0) * kTZ + (iz - z0)] = (signed char)j;
+}
+// Add or subtract, in slot (point id) order, the staged amplitudes landing on
+// q, starting at the slot the map names.
+__device__ __forceinline__ float apply_mapped(const TilePts& tp, int n, int head, int sign, int q, float v) {
+  for (int j = head; j < n; j++)
     if (tp.q[j] == q) v = sign > 0 ? v + tp.amp[j] : v - tp.amp[j];
   return v;
+}
+// Record the staged receivers of the tile from the staged field (slot threads).
+__device__ __forceinline__ void record_slots(const TilePts& tp, int n, const Grid& g, int x0, int z0,
+                                             const float* tile, int ldz, float* out) {
+  const int j = threadIdx.x;
+  if (j >= n) return;
+  const int q = tp.q[j];
+  const int ix = q / g.ld, iz = q - ix * g.ld;
+  out[tp.pid[j]] = tile[(ix - x0 + kHalo) * ldz + (iz - z0 + kHalo)];
 }
 
 // Apply all active point amplitudes of this tile that land on grid index q,
@@ -289,7 +319,7 @@ __device__ __forceinline__ float apply_points(const Inject& in, int tile, int e,
 
 template <bool DAMPED, int P, int TX, bool FULL>
 __device__ __forceinline__ void fwd_tile(const FwdStepArgs& a, int tile, int e, int x0, int z0, float* s_cur,
-                                         float* s_u2, TilePts& s_inj, TilePts& s_rec) {
+                                         float* s_u2, TilePts& s_inj, TilePts& s_rec, signed char* imap) {
   using T = Tile<TX>;
   const Grid& g = a.g;
   const long long fo = (long long)e * a.fstride;
@@ -317,7 +347,12 @@ __device__ __forceinline__ void fwd_tile(const FwdStepArgs& a, int tile, int e, 
   stage_tile<TX, FULL>(g, a.cur + fo, s_cur, x0, z0);
   store_points(li, s_inj);
   store_points(lr, s_rec);
+  const bool mapped = li.n > 0 && li.n <= kTilePts;
+  if (mapped) mark_clear<TX>(imap);
   __syncthreads();
+  if (mapped) mark_slots<TX>(s_inj, li.n, g, x0, z0, imap);
+  if (lr.n > 0 && lr.n <= kTilePts)
+    record_slots(s_rec, lr.n, g, x0, z0, s_cur, T::LZ, a.rec.out + (long long)e * a.rec.estride);
   form_u2<P, TX, FULL>(g, s_cur, s_u2, x0, z0);
   __syncthreads();
 
@@ -332,17 +367,18 @@ __device__ __forceinline__ void fwd_tile(const FwdStepArgs& a, int tile, int e, 
     if (a.u2out) a.u2out[fo + q] = u[0];
     float nxt = sponge ? update_at<true, P, T::UZ>(c, pv[k], rr[k], u, dl[k])
                        : update_at<false, P, T::UZ>(c, pv[k], rr[k], u, 0.f);
-    if (li.n > 0)
-      nxt = li.n <= kTilePts ? apply_staged(s_inj, li.n, a.inj.sign, q, nxt) : apply_points(a.inj, tile, e, q, nxt);
+    if (mapped) {
+      const int h = imap[lx * kTZ + lz];
+      if (h >= 0) nxt = apply_mapped(s_inj, li.n, h, a.inj.sign, q, nxt);
+    } else if (li.n > 0) {
+      nxt = apply_points(a.inj, tile, e, q, nxt);
+    }
     prev[q] = nxt;
     if (band_tile) {
       const int bi = band_index(g, ix, iz);
       if (bi >= 0) a.band[(long long)e * a.band_estride + bi] = c;
     }
-    if (lr.n > 0 && lr.n <= kTilePts) {
-      for (int j = 0; j < lr.n; j++)
-        if (s_rec.q[j] == (int)q) a.rec.out[(long long)e * a.rec.estride + s_rec.pid[j]] = c;
-    } else if (lr.n > 0) {
+    if (lr.n > kTilePts) {
       const int b = a.rec.pts.tile_ptr[tile], eidx = a.rec.pts.tile_ptr[tile + 1];
       for (int j = b; j < eidx; j++) {
         const int pid = a.rec.pts.items[j];
@@ -370,18 +406,20 @@ __global__ __launch_bounds__(kThreads) void k_fwd_step(FwdStepArgs a, int nentri
   __shared__ float s_cur[T::LX * T::LZ];
   __shared__ float s_u2[T::UX * T::UZ];
   __shared__ TilePts s_inj, s_rec;
+  __shared__ __align__(4) signed char imap[kTZ * TX];
   int tile, e;
   block_work(a.g.ntz * a.g.ntx, nentries, tile, e);
   const int z0 = kHalo + (tile % a.g.ntz) * kTZ, x0 = kHalo + (tile / a.g.ntz) * TX;
   if (tile_is_full<TX>(a.g, x0, z0))
-    fwd_tile<DAMPED, P, TX, true>(a, tile, e, x0, z0, s_cur, s_u2, s_inj, s_rec);
+    fwd_tile<DAMPED, P, TX, true>(a, tile, e, x0, z0, s_cur, s_u2, s_inj, s_rec, imap);
   else
-    fwd_tile<DAMPED, P, TX, false>(a, tile, e, x0, z0, s_cur, s_u2, s_inj, s_rec);
+    fwd_tile<DAMPED, P, TX, false>(a, tile, e, x0, z0, s_cur, s_u2, s_inj, s_rec, imap);
 }
 
 template <int P, int TX, bool FULL>
 __device__ __forceinline__ void bwd_tile(const BwdStepArgs& a, int tile, int e, int x0, int z0, float* s_sc,
-                                         float* s_gc, float* s_su, float* s_gu, TilePts& s_sub, TilePts& s_inj) {
+                                         float* s_gc, float* s_su, float* s_gu, TilePts& s_sub, TilePts& s_inj,
+                                         signed char* smap, signed char* gmap) {
   using T = Tile<TX>;
   const Grid& g = a.g;
   const long long fo = (long long)e * a.fstride;
@@ -418,7 +456,12 @@ __device__ __forceinline__ void bwd_tile(const BwdStepArgs& a, int tile, int e, 
   stage_tile<TX, FULL>(g, a.g_cur + fo, s_gc, x0, z0);
   store_points(ls, s_sub);
   store_points(li, s_inj);
+  const bool smapped = ls.n > 0 && ls.n <= kTilePts, gmapped = li.n > 0 && li.n <= kTilePts;
+  if (smapped) mark_clear<TX>(smap);
+  if (gmapped) mark_clear<TX>(gmap);
   __syncthreads();
+  if (smapped) mark_slots<TX>(s_sub, ls.n, g, x0, z0, smap);
+  if (gmapped) mark_slots<TX>(s_inj, li.n, g, x0, z0, gmap);
   form_u2<P, TX, FULL>(g, s_sc, s_su, x0, z0);
   form_u2<P, TX, FULL>(g, s_gc, s_gu, x0, z0);
   __syncthreads();
@@ -433,8 +476,12 @@ __device__ __forceinline__ void bwd_tile(const BwdStepArgs& a, int tile, int e, 
     const int lu = (lx + 1) * T::UZ + lz + 1;
     // source field: stepBackward (undamped) then subSource
     float sn = update_at<false, P, T::UZ>(s_sc[l], sp[k], rr[k], s_su + lu, 0.f);
-    if (ls.n > 0)
-      sn = ls.n <= kTilePts ? apply_staged(s_sub, ls.n, a.s_sub.sign, q, sn) : apply_points(a.s_sub, tile, e, q, sn);
+    if (smapped) {
+      const int h = smap[lx * kTZ + lz];
+      if (h >= 0) sn = apply_mapped(s_sub, ls.n, h, a.s_sub.sign, q, sn);
+    } else if (ls.n > 0) {
+      sn = apply_points(a.s_sub, tile, e, q, sn);
+    }
     // receiver field: stepForward (damped) of the already injected cur
     const float gc = s_gc[l];
     float gn = sponge ? update_at<true, P, T::UZ>(gc, gp[k], rr[k], s_gu + lu, dl[k])
@@ -445,8 +492,12 @@ __device__ __forceinline__ void bwd_tile(const BwdStepArgs& a, int tile, int e, 
     // what the next iteration (it-1) expects to find: the band of the source
     // field restored (readBndry) and the receivers injected (addSource)
     if (inband & (1u << k)) sn = bv[k];
-    if (li.n > 0)
-      gn = li.n <= kTilePts ? apply_staged(s_inj, li.n, a.g_inj.sign, q, gn) : apply_points(a.g_inj, tile, e, q, gn);
+    if (gmapped) {
+      const int h = gmap[lx * kTZ + lz];
+      if (h >= 0) gn = apply_mapped(s_inj, li.n, h, a.g_inj.sign, q, gn);
+    } else if (li.n > 0) {
+      gn = apply_points(a.g_inj, tile, e, q, gn);
+    }
     s_prev[q] = sn;
     g_prev[q] = gn;
   }
@@ -460,13 +511,15 @@ __global__ __launch_bounds__(kThreads) void k_bwd_step(BwdStepArgs a, int nentri
   __shared__ float s_su[T::UX * T::UZ];
   __shared__ float s_gu[T::UX * T::UZ];
   __shared__ TilePts s_sub, s_inj;
+  __shared__ __align__(4) signed char smap[kTZ * TX];
+  __shared__ __align__(4) signed char gmap[kTZ * TX];
   int tile, e;
   block_work(a.g.ntz * a.g.ntx, nentries, tile, e);
   const int z0 = kHalo + (tile % a.g.ntz) * kTZ, x0 = kHalo + (tile / a.g.ntz) * TX;
   if (tile_is_full<TX>(a.g, x0, z0))
-    bwd_tile<P, TX, true>(a, tile, e, x0, z0, s_sc, s_gc, s_su, s_gu, s_sub, s_inj);
+    bwd_tile<P, TX, true>(a, tile, e, x0, z0, s_sc, s_gc, s_su, s_gu, s_sub, s_inj, smap, gmap);
   else
-    bwd_tile<P, TX, false>(a, tile, e, x0, z0, s_sc, s_gc, s_su, s_gu, s_sub, s_inj);
+    bwd_tile<P, TX, false>(a, tile, e, x0, z0, s_sc, s_gc, s_su, s_gu, s_sub, s_inj, smap, gmap);
 }
 
 __global__ void k_fill(float* p, long long n, float v) {
