@@ -111,6 +111,10 @@ int swfwi_solver_set_tile_x(swfwi_solver h, int tile_x);
 /* Tile width in use for a kernel kind (0 forward, 1 backward) and batch
  * size, 0 when that batch size has not run yet. */
 int swfwi_solver_get_tile_x(swfwi_solver h, int kind, int entries, int *tile_x);
+/* Backward iterations as one fused block per tile (0), as separate source- and
+ * receiver-field blocks (1), or chosen by batch size (-1, the default: split
+ * while the fused launch would leave most of the chip idle).  Same results. */
+int swfwi_solver_set_bwd_split(swfwi_solver h, int mode);
 /* Record a HIP event pair around every forward / backward time loop on the
  * handle's stream (graph replays included); swfwi_solver_kernel_times waits
  * for them and returns the accumulated milliseconds and step-kernel launch

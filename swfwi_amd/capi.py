@@ -49,6 +49,7 @@ SIGNATURES = {
     "swfwi_solver_set_timing": (_I, [_VP, _I]),
     "swfwi_solver_kernel_times": (_I, [_VP, _f64, np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")]),
     "swfwi_solver_time_steps": (_I, [_VP, _I, _I, _I, _I, C.POINTER(_F)]),
+    "swfwi_solver_set_bwd_split": (_I, [_VP, _I]),
     "swfwi_enkf_create": (_I, [C.POINTER(_VP), _VP, _f32, _f32, _F, _I]),
     "swfwi_enkf_destroy": (_I, [_VP]),
     "swfwi_enkf_set_members": (_I, [_VP, _f32]),
@@ -230,6 +231,10 @@ class Solver:
         t = _I(0)
         check(load().swfwi_solver_get_tile_x(self.h, kind, entries, C.byref(t)))
         return t.value
+
+    def set_bwd_split(self, mode):
+        """-1: by batch size (default), 0: fused backward blocks, 1: split."""
+        check(load().swfwi_solver_set_bwd_split(self.h, int(mode)))
 
     def time_steps(self, kind, entries, tile_x, reps=100):
         """Microseconds per launch of back-to-back step kernels on scratch fields."""

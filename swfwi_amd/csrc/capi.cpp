@@ -359,6 +359,13 @@ int swfwi_solver_time_steps(swfwi_solver h, int kind, int entries, int tile_x, i
   });
 }
 
+int swfwi_solver_set_bwd_split(swfwi_solver h, int mode) {
+  return guarded([&] {
+    NEED(h);
+    h->s->prop().set_bwd_split(mode);
+  });
+}
+
 int swfwi_solver_set_timing(swfwi_solver h, int on) {
   return guarded([&] {
     NEED(h);

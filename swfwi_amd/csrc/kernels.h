@@ -106,6 +106,7 @@ struct BwdStepArgs {
   long long gstride = 0;
   float weight = 0.f;             // taper weight of this step; xcorr skipped when do_xcorr == 0
   int do_xcorr = 0;
+  int split = 0;                  // launch the two fields as separate blocks (small batches)
 };
 
 // ---- launchers (all asynchronous on `stream`) -------------------------------

@@ -178,6 +178,17 @@ Grid Propagator::grid(int w) const {
   return gr;
 }
 
+void Propagator::set_bwd_split(int mode) {
+  if (mode < -1 || mode > 1) fail(kErrArg, "backward split mode must be -1 (by batch size), 0 or 1");
+  bwd_split_ = mode;
+}
+
+bool Propagator::bwd_split(int E) const {
+  if (bwd_split_ >= 0) return bwd_split_ == 1;
+  const long long tiles16 = (long long)g_.ntz * tile_count_x(g_.nxpad, 16) * E;
+  return tiles16 < kBwdSplitBelow;
+}
+
 void Propagator::set_tile_x(int tx) {
   if (tx != 0 && tx != 8 && tx != 16 && tx != 32) fail(kErrArg, "tile width must be 0 (tuned), 8, 16 or 32");
   tile_override_ = tx;
@@ -233,6 +244,7 @@ float Propagator::time_steps(int kind, int E, const float* rv, long long rvstrid
       a.grad = w.c.get();  // scratch only
       a.gstride = np;
       a.do_xcorr = 0;
+      a.split = bwd_split(E);
       launch_bwd_step(a, E, prec_, s);
     }
   }
@@ -257,12 +269,14 @@ int Propagator::tile_choice(int kind, int E, const float* rv, long long rvstride
     if (kv.first == key) return kv.second;
   int best = 1;
   if (!tune_ || !may_time) {
-    // measured on the 301x921 grid: the narrow tile wins while the batch
-    // does not fill the chip (more, shorter blocks), the wide one once it
-    // is bandwidth-bound (less halo traffic per output)
+    // measured on the 301x921 grid: forward, the narrow tile wins while the
+    // batch does not fill the chip (more, shorter blocks), the wide one once
+    // it is bandwidth-bound (less halo traffic per output); backward, 64x16
+    // at every batch size (split into source and receiver blocks while the
+    // batch is small, fused beyond)
     const long long tiles16 = (long long)g_.ntz * tile_count_x(g_.nxpad, 16) * E;
     if (kind == 1)
-      best = tiles16 < 1000 ? 0 : 1;
+      best = 1;
     else
       best = tiles16 < 600 ? 0 : (tiles16 < 2000 ? 1 : 2);
   } else {
@@ -517,7 +531,8 @@ int Propagator::backward(int E, const float* rv, long long rvstride, const Sourc
                            (uintptr_t)w.s_nxt,
                            (uintptr_t)w.spare0,
                            (uintptr_t)w.spare1,
-                           (uintptr_t)prec_};
+                           (uintptr_t)prec_,
+                           (uintptr_t)bwd_split(E)};
   run_maybe_graph(std::vector<uintptr_t>(std::begin(key), std::end(key)), E, 1, g_.nt - it_min, [&] {
     backward_body(E, rv, rvstride, src, vsrc, vstride, band, grad, gstride, w, rvdiv, it_min, wts, tw);
   });
@@ -567,6 +582,7 @@ void Propagator::backward_body(int E, const float* rv, long long rvstride, const
   a.grad = grad;
   a.gstride = gstride;
   a.do_xcorr = 1;
+  a.split = bwd_split(E);
   for (int it = nt - 1; it >= it_min; it--) {
     a.s_cur = s_cur;
     a.s_prev = s_prv;

@@ -132,9 +132,14 @@ class Propagator {
   // to a size heuristic.
   void set_tile_x(int tx);
   void set_tuning(bool on) { tune_ = on; }
+  // Backward iterations as one fused block per tile (0), as separate source
+  // and receiver blocks (1), or by batch size (-1, the default: split while
+  // the fused tiles would not fill the chip).
+  void set_bwd_split(int mode);
+  bool bwd_split(int E) const;
   int tile_choice_cached(int kind, int E) const;
   // Microseconds per launch of `reps` back-to-back step kernels (kind 0
-  // forward, 1 backward) of tile width kTileWidths[width] on E scratch
+  // forward, 1 backward as backward() launches it) of tile width kTileWidths[width] on E scratch
   // wavefields of model rv (entry e uses rv + (e / rvdiv) * rvstride).
   float time_steps(int kind, int E, const float* rv, long long rvstride, int rvdiv, Fields& w, int width, int reps,
                    bool clear);
@@ -170,6 +175,7 @@ class Propagator {
   // trial of every width on scratch wavefields (the tile tuner).
   int tile_choice(int kind, int E, const float* rv, long long rvstride, int rvdiv, Fields& w, bool may_time);
   static constexpr int kGraphMaxEntries = 8;
+  static constexpr long long kBwdSplitBelow = 10000;  // 64x16 tiles over the batch
   // Run a time loop (`launches` step launches of kind 0 forward / 1
   // backward): captured into a HIP graph on first use and replayed when the
   // same buffers come back, or launched directly.
@@ -198,6 +204,7 @@ class Propagator {
   DevPoints src_[3], rec_[3];    // points per tile
   int tile_override_ = 0;                 // 0 = tune, else a width from kTileWidths
   bool tune_ = false;
+  int bwd_split_ = -1;
   std::vector<std::pair<long long, int>> tile_cache_;  // (kind*2^32 + E) -> width index
   DBuf<int> sxp_, szp_, gxp_, gzp_;
 };

@@ -503,6 +503,145 @@ __device__ __forceinline__ void bwd_tile(const BwdStepArgs& a, int tile, int e, 
   }
 }
 
+// The same backward iteration as two independent halves of one launch, for
+// small batches whose fused tiles leave most of the chip idle: the source
+// half (undamped step, subSource, imaging condition with the receiver field
+// read at the point, band restore) and the receiver half (damped step,
+// receiver injection).  Each half stages one field.
+template <int P, int TX, bool FULL>
+__device__ __forceinline__ void bwd_src_tile(const BwdStepArgs& a, int tile, int e, int x0, int z0, float* s_c,
+                                             float* s_u, TilePts& tp, signed char* map) {
+  using T = Tile<TX>;
+  const Grid& g = a.g;
+  const long long fo = (long long)e * a.fstride;
+  const float* __restrict__ rv = a.rv + (long long)(e / a.rvdiv) * a.rvstride;
+  const float* __restrict__ g_cur = a.g_cur + fo;
+  float* __restrict__ s_prev = a.s_prev + fo;
+  float* __restrict__ grad = a.grad + (long long)e * a.gstride;
+  const int lz = threadIdx.x % kTZ, row = threadIdx.x / kTZ;
+  const int iz = z0 + lz;
+  PtsLoad ls{0, -1, 0, 0.f};
+  if (a.s_sub.t >= 0) ls = load_points(a.s_sub.pts, &a.s_sub, tile, e);
+  const bool band_tile = a.band != nullptr && touches_band<TX>(g, x0, z0);
+  float sp[T::PER], rr[T::PER], gr[T::PER], gc[T::PER], bv[T::PER];
+  unsigned inband = 0;
+#pragma unroll
+  for (int k = 0; k < T::PER; k++) {
+    const int ix = x0 + row + kRows * k;
+    const bool ok = FULL || (ix < g.nx - kHalo && iz < g.nz - kHalo);
+    const unsigned q = (unsigned)(ix * g.ld + iz);
+    sp[k] = ok ? s_prev[q] : 0.f;
+    rr[k] = ok ? rv[q] : 1.f;
+    gr[k] = (ok && a.do_xcorr) ? grad[q] : 0.f;
+    gc[k] = (ok && a.do_xcorr) ? g_cur[q] : 0.f;
+    const int bi = (ok && band_tile) ? band_index(g, ix, iz) : -1;
+    bv[k] = bi >= 0 ? a.band[(long long)e * a.band_estride + bi] : 0.f;
+    if (bi >= 0) inband |= 1u << k;
+  }
+  stage_tile<TX, FULL>(g, a.s_cur + fo, s_c, x0, z0);
+  store_points(ls, tp);
+  const bool mapped = ls.n > 0 && ls.n <= kTilePts;
+  if (mapped) mark_clear<TX>(map);
+  __syncthreads();
+  if (mapped) mark_slots<TX>(tp, ls.n, g, x0, z0, map);
+  form_u2<P, TX, FULL>(g, s_c, s_u, x0, z0);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < T::PER; k++) {
+    const int lx = row + kRows * k;
+    const int ix = x0 + lx;
+    if (!FULL && (ix >= g.nx - kHalo || iz >= g.nz - kHalo)) continue;
+    const unsigned q = (unsigned)(ix * g.ld + iz);
+    float sn = update_at<false, P, T::UZ>(s_c[(lx + kHalo) * T::LZ + lz + kHalo], sp[k], rr[k],
+                                          s_u + (lx + 1) * T::UZ + lz + 1, 0.f);
+    if (mapped) {
+      const int h = map[lx * kTZ + lz];
+      if (h >= 0) sn = apply_mapped(tp, ls.n, h, a.s_sub.sign, q, sn);
+    } else if (ls.n > 0) {
+      sn = apply_points(a.s_sub, tile, e, q, sn);
+    }
+    if (a.do_xcorr) grad[q] = gr[k] - sn * gc[k] * a.weight;
+    if (inband & (1u << k)) sn = bv[k];
+    s_prev[q] = sn;
+  }
+}
+
+template <int P, int TX, bool FULL>
+__device__ __forceinline__ void bwd_rcv_tile(const BwdStepArgs& a, int tile, int e, int x0, int z0, float* s_c,
+                                             float* s_u, TilePts& tp, signed char* map) {
+  using T = Tile<TX>;
+  const Grid& g = a.g;
+  const long long fo = (long long)e * a.fstride;
+  const float* __restrict__ rv = a.rv + (long long)(e / a.rvdiv) * a.rvstride;
+  float* __restrict__ g_prev = a.g_prev + fo;
+  const int lz = threadIdx.x % kTZ, row = threadIdx.x / kTZ;
+  const int iz = z0 + lz;
+  PtsLoad li{0, -1, 0, 0.f};
+  if (a.g_inj.t >= 0) li = load_points(a.g_inj.pts, &a.g_inj, tile, e);
+  const bool sponge = touches_sponge(g, x0, x0 + TX, z0, z0 + kTZ);
+  float gp[T::PER], rr[T::PER], dl[T::PER];
+#pragma unroll
+  for (int k = 0; k < T::PER; k++) {
+    const int ix = x0 + row + kRows * k;
+    const bool ok = FULL || (ix < g.nx - kHalo && iz < g.nz - kHalo);
+    const unsigned q = (unsigned)(ix * g.ld + iz);
+    gp[k] = ok ? g_prev[q] : 0.f;
+    rr[k] = ok ? rv[q] : 1.f;
+    dl[k] = (sponge && ok) ? damping_delta(g, ix, iz) : 0.f;
+  }
+  stage_tile<TX, FULL>(g, a.g_cur + fo, s_c, x0, z0);
+  store_points(li, tp);
+  const bool mapped = li.n > 0 && li.n <= kTilePts;
+  if (mapped) mark_clear<TX>(map);
+  __syncthreads();
+  if (mapped) mark_slots<TX>(tp, li.n, g, x0, z0, map);
+  form_u2<P, TX, FULL>(g, s_c, s_u, x0, z0);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < T::PER; k++) {
+    const int lx = row + kRows * k;
+    const int ix = x0 + lx;
+    if (!FULL && (ix >= g.nx - kHalo || iz >= g.nz - kHalo)) continue;
+    const unsigned q = (unsigned)(ix * g.ld + iz);
+    const float c = s_c[(lx + kHalo) * T::LZ + lz + kHalo];
+    const float* u = s_u + (lx + 1) * T::UZ + lz + 1;
+    float gn = sponge ? update_at<true, P, T::UZ>(c, gp[k], rr[k], u, dl[k])
+                      : update_at<false, P, T::UZ>(c, gp[k], rr[k], u, 0.f);
+    if (mapped) {
+      const int h = map[lx * kTZ + lz];
+      if (h >= 0) gn = apply_mapped(tp, li.n, h, a.g_inj.sign, q, gn);
+    } else if (li.n > 0) {
+      gn = apply_points(a.g_inj, tile, e, q, gn);
+    }
+    g_prev[q] = gn;
+  }
+}
+
+template <int P, int TX>
+__global__ __launch_bounds__(kThreads) void k_bwd_split(BwdStepArgs a, int nentries) {
+  using T = Tile<TX>;
+  __shared__ float s_c[T::LX * T::LZ];
+  __shared__ float s_u[T::UX * T::UZ];
+  __shared__ TilePts tp;
+  __shared__ __align__(4) signed char map[kTZ * TX];
+  int tile, r;
+  block_work(a.g.ntz * a.g.ntx, 2 * nentries, tile, r);  // both halves of an entry adjacent
+  const int e = r >> 1;
+  const int z0 = kHalo + (tile % a.g.ntz) * kTZ, x0 = kHalo + (tile / a.g.ntz) * TX;
+  const bool full = tile_is_full<TX>(a.g, x0, z0);
+  if (r & 1) {
+    if (full)
+      bwd_rcv_tile<P, TX, true>(a, tile, e, x0, z0, s_c, s_u, tp, map);
+    else
+      bwd_rcv_tile<P, TX, false>(a, tile, e, x0, z0, s_c, s_u, tp, map);
+  } else {
+    if (full)
+      bwd_src_tile<P, TX, true>(a, tile, e, x0, z0, s_c, s_u, tp, map);
+    else
+      bwd_src_tile<P, TX, false>(a, tile, e, x0, z0, s_c, s_u, tp, map);
+  }
+}
+
 template <int P, int TX>
 __global__ __launch_bounds__(kThreads) void k_bwd_step(BwdStepArgs a, int nentries) {
   using T = Tile<TX>;
@@ -577,6 +716,14 @@ void fwd_dispatch(const FwdStepArgs& a, int E, bool damped, int precision, hipSt
 
 template <int TX>
 void bwd_dispatch(const BwdStepArgs& a, int E, int precision, hipStream_t s) {
+  if (a.split) {
+    const dim3 grid(a.g.ntz * a.g.ntx * E * 2);
+    if (precision == kExact)
+      hipLaunchKernelGGL((k_bwd_split<kExact, TX>), grid, dim3(kThreads), 0, s, a, E);
+    else
+      hipLaunchKernelGGL((k_bwd_split<kFp32, TX>), grid, dim3(kThreads), 0, s, a, E);
+    return;
+  }
   const dim3 grid(a.g.ntz * a.g.ntx * E);
   if (precision == kExact)
     hipLaunchKernelGGL((k_bwd_step<kExact, TX>), grid, dim3(kThreads), 0, s, a, E);

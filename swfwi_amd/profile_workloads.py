@@ -49,13 +49,15 @@ def run_sweep(entries, reps, precision):
         print(f"empty kernel, {blocks} blocks: stream {capi.launch_latency(blocks, reps):.2f} us, "
               f"graph {capi.launch_latency(blocks, reps, graph=True):.2f} us per launch", flush=True)
     for E in entries:
-        for kind, byts in ((0, 16.0), (1, 40.0)):
+        for kind, byts, split, label in ((0, 16.0, -1, "fwd"), (1, 40.0, 0, "bwd"), (1, 40.0, 1, "bwd/split")):
+            s.set_bwd_split(split)
             row = []
             for tx in (8, 16, 32):
                 s.time_steps(kind, E, tx, 4)  # warm
                 us = s.time_steps(kind, E, tx, reps)
                 row.append(f"tx{tx} {us:8.1f} us {E * pts * byts / (us * 1e3):7.0f} GB/s")
-            print(f"{['fwd', 'bwd'][kind]:4s} E={E:4d}: " + " | ".join(row), flush=True)
+            print(f"{label:9s} E={E:4d}: " + " | ".join(row), flush=True)
+        s.set_bwd_split(-1)
 
 
 def run_ess(nt, epochs, graphs=True):

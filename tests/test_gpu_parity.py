@@ -131,13 +131,16 @@ def test_calgradient_bitexact(oracle, small, shot):
     assert np.array_equal(bits(g_gpu), bits(g_cpu))
 
 
+@pytest.mark.parametrize("split", [0, 1])
 @pytest.mark.parametrize("tile_x", [8, 16, 32])
-def test_every_tile_width_is_bitexact(oracle, small, tile_x):
-    """Each instantiated tile width reproduces the oracle's seismogram and
-    gradient bit for bit."""
+def test_every_tile_width_is_bitexact(oracle, small, tile_x, split):
+    """Each instantiated tile width, with the backward iteration fused or
+    split into source and receiver blocks, reproduces the oracle's seismogram
+    and gradient bit for bit."""
     case, s, g, vt, vi, wlt = small
-    print(f"tile_x={tile_x}", flush=True)
+    print(f"tile_x={tile_x} split={split}", flush=True)
     s.set_tile_x(tile_x)
+    s.set_bwd_split(split)
     try:
         d_gpu = s.forward_modeling(wlt, shot=0)
         assert s.tile_x(0, 1) == tile_x
@@ -149,6 +152,7 @@ def test_every_tile_width_is_bitexact(oracle, small, tile_x):
         assert np.array_equal(bits(dobs[case.ns - 1]), bits(g.forward_modeling(vt, wlt, shot=case.ns - 1)))
     finally:
         s.set_tile_x(0)
+        s.set_bwd_split(-1)
 
 
 @pytest.mark.parametrize("nt", [361, 362])
