@@ -105,8 +105,8 @@ int swfwi_solver_counters(swfwi_solver h, double *point_updates, long long *laun
 /* Capture time loops into HIP graphs and replay them when the same buffers
  * recur (on by default inside the inversion drivers' epochs). */
 int swfwi_solver_set_graphs(swfwi_solver h, int on);
-/* Stencil tile width along x: 8, 16 or 32, or 0 to let the tile tuner time
- * all three on the first run of each batch size (the default). */
+/* Stencil tile width along x: 8, 16 or 32, or 0 for the measured rule
+ * by batch size (the default). */
 int swfwi_solver_set_tile_x(swfwi_solver h, int tile_x);
 /* Tile width in use for a kernel kind (0 forward, 1 backward) and batch
  * size, 0 when that batch size has not run yet. */

@@ -349,7 +349,7 @@ int swfwi_solver_time_steps(swfwi_solver h, int kind, int entries, int tile_x, i
     if (kind != 0 && kind != 1) fail(kErrArg, "kind must be 0 (forward) or 1 (backward)");
     if (entries < 1 || reps < 1) fail(kErrArg, "entries and reps must be >= 1");
     int width = -1;
-    for (int i = 0; i < 3; i++)
+    for (int i = 0; i < kNumWidths; i++)
       if (kTileWidths[i] == tile_x) width = i;
     if (width < 0) fail(kErrArg, "tile width must be 8, 16 or 32");
     Solver& s = *h->s;

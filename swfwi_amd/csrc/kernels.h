@@ -112,7 +112,8 @@ struct BwdStepArgs {
 // ---- launchers (all asynchronous on `stream`) -------------------------------
 // Tile widths the step kernels are instantiated for, and the tile count along
 // x of a padded grid for one of them.
-constexpr int kTileWidths[3] = {8, 16, 32};
+constexpr int kNumWidths = 3;
+constexpr int kTileWidths[kNumWidths] = {8, 16, 32};
 int tile_count_x(int nx, int tile_x);
 void launch_fwd_step(const FwdStepArgs& a, int nentries, bool damped, int precision, hipStream_t stream);
 void launch_bwd_step(const BwdStepArgs& a, int nentries, int precision, hipStream_t stream);

@@ -147,7 +147,7 @@ Propagator::Propagator(const Geometry& g, int precision) : g_(g), prec_(precisio
     gzp[i] = g_.gz[i] + g_.bz0;
     gi[i] = (long long)gxp[i] * g_.ld + gzp[i];
   }
-  for (int w = 0; w < 3; w++) {
+  for (int w = 0; w < kNumWidths; w++) {
     const int ntx = tile_count_x(g_.nxpad, kTileWidths[w]);
     build_points(si, g_.ld, g_.ntz, ntx, kTileWidths[w], 0, src_[w], s);
     build_points(gi, g_.ld, g_.ntz, ntx, kTileWidths[w], 0, rec_[w], s);
@@ -190,7 +190,7 @@ bool Propagator::bwd_split(int E) const {
 }
 
 void Propagator::set_tile_x(int tx) {
-  if (tx != 0 && tx != 8 && tx != 16 && tx != 32) fail(kErrArg, "tile width must be 0 (tuned), 8, 16 or 32");
+  if (tx != 0 && tx != 8 && tx != 16 && tx != 32) fail(kErrArg, "tile width must be 0 (by rule), 8, 16 or 32");
   tile_override_ = tx;
 }
 
@@ -261,7 +261,7 @@ float Propagator::time_steps(int kind, int E, const float* rv, long long rvstrid
 int Propagator::tile_choice(int kind, int E, const float* rv, long long rvstride, int rvdiv, Fields& w,
                             bool may_time) {
   if (tile_override_) {
-    for (int i = 0; i < 3; i++)
+    for (int i = 0; i < kNumWidths; i++)
       if (kTileWidths[i] == tile_override_) return i;
   }
   const long long key = ((long long)kind << 32) + E;
@@ -284,14 +284,14 @@ int Propagator::tile_choice(int kind, int E, const float* rv, long long rvstride
     // this point: forward() tunes before it zeroes and runs); three
     // interleaved rounds (the first warms clocks and caches), the best of
     // the last two per width
-    float best_of[3] = {0, 0, 0};
+    float best_of[kNumWidths] = {};
     for (int round = 0; round < 3; round++)
-      for (int i = 0; i < 3; i++) {
+      for (int i = 0; i < kNumWidths; i++) {
         const float us = time_steps(kind, E, rv, rvstride, rvdiv, w, i, 8, round == 0 && i == 0);
         if (round == 1 || (round == 2 && us < best_of[i])) best_of[i] = us;
       }
     best = 0;
-    for (int i = 1; i < 3; i++)
+    for (int i = 1; i < kNumWidths; i++)
       if (best_of[i] < best_of[best]) best = i;
   }
   tile_cache_.push_back({key, best});

@@ -201,7 +201,7 @@ class Propagator {
   int prec_;
   Stream stream_;
   DampTables damp_;
-  DevPoints src_[3], rec_[3];    // points per tile
+  DevPoints src_[kNumWidths], rec_[kNumWidths];  // points per tile, per tile width
   int tile_override_ = 0;                 // 0 = tune, else a width from kTileWidths
   bool tune_ = false;
   int bwd_split_ = -1;

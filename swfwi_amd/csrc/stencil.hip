@@ -12,8 +12,8 @@
 //            same tile, subSource, the zero-lag imaging condition, and the
 //            band restore / receiver injection that the next iteration needs.
 //
-// A tile is kTZ x TX output points (TX = 8, 16 or 32, picked per workload by
-// the propagator's tile tuner).  Its `cur` plus a 6-cell halo is staged in
+// A tile is kTZ x TX output points (TX = 8, 16 or 32, picked per batch
+// size by the propagator's measured rule).  Its `cur` plus a 6-cell halo is staged in
 // LDS with every global load of the block in flight at once, the unscaled
 // 10th-order Laplacian u2 is formed in LDS for the tile plus a 1-cell ring,
 // and each thread finishes TX/4 points of one z-row.  prev, the model and

@@ -202,6 +202,33 @@ def test_coincident_sources_accumulate_in_id_order(oracle):
         assert np.array_equal(bits(s.forward_modeling(wlt, shot=shot)), bits(g.forward_modeling(vt, wlt, shot=shot)))
 
 
+@pytest.mark.parametrize("split", [0, 1])
+def test_crowded_tile_falls_back_to_point_lists(oracle, split):
+    """A receiver spread on several rows puts more than 64 receivers (and
+    coincident sources) into one tile: the kernels take the global point-list
+    path there and stay bit-identical, forward and backward."""
+    from swfwi_amd.workloads import Case
+    nx, nz = 40, 30
+    gx = np.repeat(np.arange(2, 30, dtype=np.int32), 10)
+    gz = np.tile(np.arange(0, 10, dtype=np.int32), 28)
+    sx = np.array([12] * 70 + [20], np.int32)
+    case = Case("crowded", nz, nx, 10.0, 8, 0.001, 330, 25.0, sx, np.ones_like(sx), gx, gz, 1500.0, 3500.0, seed=4)
+    s, g, vt, vi, wlt = setup_case(oracle, case)
+    s.bind_velocity(vt)
+    s.set_tile_x(8)
+    s.set_bwd_split(split)
+    try:
+        code = capi.random_codes(6, case.ns)
+        enc = oracle.encode_source(code, wlt)
+        assert np.array_equal(bits(s.forward_modeling(enc, shot=-1)), bits(g.forward_modeling(vt, enc, shot=-1)))
+        rng = np.random.default_rng(8)
+        vsrc = rng.standard_normal((case.nt, case.ng)).astype(np.float32)
+        assert np.array_equal(bits(s.calgradient(enc, vsrc, shot=-1)), bits(g.calgradient(vt, enc, vsrc, shot=-1)))
+    finally:
+        s.set_tile_x(0)
+        s.set_bwd_split(-1)
+
+
 def test_fp32_mode_within_budget(oracle, small):
     case, s, g, vt, vi, wlt = small
     s.set_precision(capi.PREC_FP32)
