@@ -27,7 +27,7 @@ def trace(path, out):
     rows = list(csv.DictReader(open(path)))
     d = collections.defaultdict(list)
     for r in rows:
-        if "_step" not in r["Kernel_Name"]:
+        if not re.search(r"k_(fwd|bwd)_", r["Kernel_Name"]):
             continue
         blocks = int(r["Grid_Size_X"]) // int(r["Workgroup_Size_X"])
         d[(variant(r["Kernel_Name"]), blocks)].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
@@ -55,8 +55,8 @@ def hbm(a, out):
               f"{copy_bytes / 2**20:.0f} MiB written) reads FETCH_SIZE {next(iter(cf.values())):.1f} KiB and "
               f"WRITE_SIZE {next(iter(cw.values())):.1f} KiB per dispatch: bytes = FETCH_SIZE KiB x 1024 x "
               f"{f_scale:.3f}, WRITE_SIZE KiB x 1024 x {w_scale:.3f}.\n\n")
-    fe, n = counter(a.fetch, lambda s: "_step" in s)
-    wr, _ = counter(a.write, lambda s: "_step" in s)
+    fe, n = counter(a.fetch, lambda s: re.search(r"k_(fwd|bwd)_", s) is not None)
+    wr, _ = counter(a.write, lambda s: re.search(r"k_(fwd|bwd)_", s) is not None)
     out.write("| kernel | blocks | dispatches | FETCH_SIZE KiB | WRITE_SIZE KiB | HBM read MB | HBM write MB |\n"
               "|---|---|---|---|---|---|---|\n")
     for k in sorted(fe):
