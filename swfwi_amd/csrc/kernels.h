@@ -35,7 +35,10 @@ struct PointList {
   // The same lists as fixed-size slots, read with one level of loads: set
   // when no tile holds more than kTilePts points, else null (CSR only).
   const int* count = nullptr;     // [ntiles] points per tile
-  const int2* slots = nullptr;    // [ntiles][kTilePts] (grid index, point id); grid index -1 past count
+  // [ntiles][kTilePts] (grid index, point id, position in the tile
+  // (x - x0) * 64 + (z - z0), 1 if another point of the tile shares the grid
+  // index); grid index -1 past count
+  const int4* slots = nullptr;
   // points sharing a grid index, chained in id order (-1 ends a chain)
   const int* prev_same = nullptr; // [npoints]
   const int* next_same = nullptr; // [npoints]

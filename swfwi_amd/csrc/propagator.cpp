@@ -103,14 +103,21 @@ void build_points(const std::vector<long long>& idx, int ld, int ntz, int ntx, i
   out.max_per_tile = 0;
   for (int t = 0; t < ntiles; t++) out.max_per_tile = std::max(out.max_per_tile, ptr[t + 1] - ptr[t]);
   std::vector<int> count(ntiles);
-  std::vector<int2> slots;
-  if (out.max_per_tile <= kTilePts) {
-    slots.assign((size_t)ntiles * kTilePts, make_int2(-1, 0));
+  std::vector<int4> slots;
+  if (halo == 0 && out.max_per_tile <= kTilePts) {
+    // (grid index, id, position inside the tile, shares its grid index with
+    // another point of the tile)
+    slots.assign((size_t)ntiles * kTilePts, make_int4(-1, 0, 0, 0));
     for (int t = 0; t < ntiles; t++) {
       count[t] = ptr[t + 1] - ptr[t];
+      const int x0 = kHalo + (t / ntz) * tile_x, z0 = kHalo + (t % ntz) * kTZ;
       for (int j = ptr[t]; j < ptr[t + 1]; j++) {
-        const int pid = items[j];
-        slots[(size_t)t * kTilePts + (j - ptr[t])] = make_int2(idx32[pid], pid);
+        const int pid = items[j], q = idx32[pid];
+        int shared = 0;
+        for (int k = ptr[t]; k < ptr[t + 1]; k++)
+          if (k != j && idx32[items[k]] == q) shared = 1;
+        const int loc = (q / ld - x0) * kTZ + (q % ld - z0);
+        slots[(size_t)t * kTilePts + (j - ptr[t])] = make_int4(q, pid, loc, shared);
       }
     }
     out.count.upload(count, s);

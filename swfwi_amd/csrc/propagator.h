@@ -38,7 +38,7 @@ struct Geometry {
 // Per-tile CSR of a point set on the device.
 struct DevPoints {
   DBuf<int> tile_ptr, items, idx, count, prev_same, next_same;
-  DBuf<int2> slots;
+  DBuf<int4> slots;
   int n = 0;
   int max_per_tile = 0;
   PointList view() const {
@@ -48,7 +48,7 @@ struct DevPoints {
     p.idx = idx.get();
     p.prev_same = prev_same.get();
     p.next_same = next_same.get();
-    if (max_per_tile <= kTilePts) {
+    if (max_per_tile <= kTilePts && slots.get()) {
       p.count = count.get();
       p.slots = slots.get();
     }

@@ -224,24 +224,31 @@ struct TilePts {
 };
 
 // One thread's share of a tile's point list: threads 0..n-1 load one item
-// each before the staging, and store it into LDS with the staged tile.  n is
-// the tile's item count (uniform over the block).
+// each before the staging (n is the tile's item count, uniform over the
+// block): grid index (-1 when inactive for this entry), id, amplitude,
+// position in the tile and whether another point of the tile shares the grid
+// index.
 struct PtsLoad {
-  int n, q, pid;
+  int n, q, pid, loc, shared;
   float amp;
 };
-__device__ __forceinline__ PtsLoad load_points(const PointList& pts, const Inject* in, int tile, int e) {
+__device__ __forceinline__ PtsLoad load_points(const PointList& pts, const Inject* in, int tile, int e, int x0, int z0,
+                                               int ld) {
   PtsLoad r;
   r.q = -1;
   r.pid = 0;
+  r.loc = 0;
+  r.shared = 1;
   r.amp = 0.f;
   if (pts.slots) {
     // count and slot in parallel (slots past the count hold -1)
     r.n = pts.count[tile];
     if ((int)threadIdx.x < kTilePts) {
-      const int2 sl = pts.slots[(long long)tile * kTilePts + threadIdx.x];
+      const int4 sl = pts.slots[(long long)tile * kTilePts + threadIdx.x];
       r.q = sl.x;
       r.pid = sl.y;
+      r.loc = sl.z;
+      r.shared = sl.w;
     }
   } else {
     const int b = pts.tile_ptr[tile];
@@ -249,6 +256,8 @@ __device__ __forceinline__ PtsLoad load_points(const PointList& pts, const Injec
     if (r.n <= kTilePts && (int)threadIdx.x < r.n) {
       r.pid = pts.items[b + threadIdx.x];
       r.q = pts.idx[r.pid];
+      const int ix = r.q / ld;
+      r.loc = (ix - x0) * kTZ + (r.q - ix * ld - z0);
     }
   }
   if (in && r.n <= kTilePts && (int)threadIdx.x < r.n) {
@@ -259,48 +268,52 @@ __device__ __forceinline__ PtsLoad load_points(const PointList& pts, const Injec
   }
   return r;
 }
+// Grid index and id go to LDS before the first barrier; the amplitude (the
+// second level of loads) after it.
 __device__ __forceinline__ void store_points(const PtsLoad& r, TilePts& tp) {
   if (r.n <= kTilePts && (int)threadIdx.x < r.n) {
     tp.q[threadIdx.x] = r.q;
     tp.pid[threadIdx.x] = r.pid;
-    tp.amp[threadIdx.x] = r.amp;
   }
 }
+__device__ __forceinline__ void store_amp(const PtsLoad& r, TilePts& tp) {
+  if (r.n <= kTilePts && (int)threadIdx.x < r.n) tp.amp[threadIdx.x] = r.amp;
+}
 // Per-tile map of staged injection points: for each output point of the
-// tile, the first staged slot whose point lands there (-1: none), so an
-// output point scans the slot list only when something is injected into it.
-// mark_clear: all threads, before the barrier that publishes the staged
-// slots; mark_slots: slot threads, after it.
+// tile, the first active slot whose point lands there, with bit 6 set when
+// later slots land there too (-1: none).  mark_clear: all threads, before the
+// barrier that publishes the staged slots; mark_slots: slot threads, after it.
 template <int TX>
 __device__ __forceinline__ void mark_clear(signed char* map) {
   for (int i = threadIdx.x; i < kTZ * TX / 4; i += kThreads) reinterpret_cast<int*>(map)[i] = -1;
 }
-template <int TX>
-__device__ __forceinline__ void mark_slots(const TilePts& tp, int n, const Grid& g, int x0, int z0, signed char* map) {
+__device__ __forceinline__ void mark_slots(const PtsLoad& r, const TilePts& tp, signed char* map) {
   const int j = threadIdx.x;
-  if (j >= n) return;
-  const int q = tp.q[j];
-  if (q < 0) return;
+  if (j >= r.n || r.q < 0) return;
+  if (!r.shared) {  // the only point of the tile at its grid index
+    map[r.loc] = (signed char)j;
+    return;
+  }
   for (int k = 0; k < j; k++)
-    if (tp.q[k] == q) return;  // an earlier slot heads this point's chain
-  const int ix = q / g.ld, iz = q - ix * g.ld;
-  map[(ix - x0) * kTZ + (iz - z0)] = (signed char)j;
+    if (tp.q[k] == r.q) return;  // an earlier active slot heads this point's chain
+  int more = 0;
+  for (int k = j + 1; k < r.n; k++)
+    if (tp.q[k] == r.q) more = 0x40;
+  map[r.loc] = (signed char)(j | more);
 }
 // Add or subtract, in slot (point id) order, the staged amplitudes landing on
-// q, starting at the slot the map names.
-__device__ __forceinline__ float apply_mapped(const TilePts& tp, int n, int head, int sign, int q, float v) {
-  for (int j = head; j < n; j++)
+// q, given the map entry m of its output point.
+__device__ __forceinline__ float apply_mapped(const TilePts& tp, int n, int m, int sign, int q, float v) {
+  const int h = m & 0x3F;
+  if (!(m & 0x40)) return sign > 0 ? v + tp.amp[h] : v - tp.amp[h];
+  for (int j = h; j < n; j++)
     if (tp.q[j] == q) v = sign > 0 ? v + tp.amp[j] : v - tp.amp[j];
   return v;
 }
 // Record the staged receivers of the tile from the staged field (slot threads).
-__device__ __forceinline__ void record_slots(const TilePts& tp, int n, const Grid& g, int x0, int z0,
-                                             const float* tile, int ldz, float* out) {
-  const int j = threadIdx.x;
-  if (j >= n) return;
-  const int q = tp.q[j];
-  const int ix = q / g.ld, iz = q - ix * g.ld;
-  out[tp.pid[j]] = tile[(ix - x0 + kHalo) * ldz + (iz - z0 + kHalo)];
+__device__ __forceinline__ void record_slots(const PtsLoad& r, const float* tile, int ldz, float* out) {
+  if ((int)threadIdx.x >= r.n) return;
+  out[r.pid] = tile[((r.loc >> 6) + kHalo) * ldz + (r.loc & (kTZ - 1)) + kHalo];
 }
 
 // Apply all active point amplitudes of this tile that land on grid index q,
@@ -329,9 +342,9 @@ __device__ __forceinline__ void fwd_tile(const FwdStepArgs& a, int tile, int e, 
   const int lz = threadIdx.x % kTZ, row = threadIdx.x / kTZ;
   const int iz = z0 + lz;
   // every global load of the block is issued here, before the first barrier
-  PtsLoad li{0, -1, 0, 0.f}, lr{0, -1, 0, 0.f};
-  if (a.inj.t >= 0) li = load_points(a.inj.pts, &a.inj, tile, e);
-  if (a.rec.out != nullptr) lr = load_points(a.rec.pts, nullptr, tile, e);
+  PtsLoad li{0, -1, 0, 0, 1, 0.f}, lr{0, -1, 0, 0, 1, 0.f};
+  if (a.inj.t >= 0) li = load_points(a.inj.pts, &a.inj, tile, e, x0, z0, g.ld);
+  if (a.rec.out != nullptr) lr = load_points(a.rec.pts, nullptr, tile, e, x0, z0, g.ld);
   const bool sponge = DAMPED && touches_sponge(g, x0, x0 + TX, z0, z0 + kTZ);
   const bool band_tile = a.band != nullptr && touches_band<TX>(g, x0, z0);
   float pv[T::PER], rr[T::PER], dl[T::PER];
@@ -350,9 +363,12 @@ __device__ __forceinline__ void fwd_tile(const FwdStepArgs& a, int tile, int e, 
   const bool mapped = li.n > 0 && li.n <= kTilePts;
   if (mapped) mark_clear<TX>(imap);
   __syncthreads();
-  if (mapped) mark_slots<TX>(s_inj, li.n, g, x0, z0, imap);
+  if (mapped) {
+    store_amp(li, s_inj);
+    mark_slots(li, s_inj, imap);
+  }
   if (lr.n > 0 && lr.n <= kTilePts)
-    record_slots(s_rec, lr.n, g, x0, z0, s_cur, T::LZ, a.rec.out + (long long)e * a.rec.estride);
+    record_slots(lr, s_cur, T::LZ, a.rec.out + (long long)e * a.rec.estride);
   form_u2<P, TX, FULL>(g, s_cur, s_u2, x0, z0);
   __syncthreads();
 
@@ -431,9 +447,9 @@ __device__ __forceinline__ void bwd_tile(const BwdStepArgs& a, int tile, int e, 
   const int lz = threadIdx.x % kTZ, row = threadIdx.x / kTZ;
   const int iz = z0 + lz;
   // every global load of the block is issued here, before the first barrier
-  PtsLoad ls{0, -1, 0, 0.f}, li{0, -1, 0, 0.f};
-  if (a.s_sub.t >= 0) ls = load_points(a.s_sub.pts, &a.s_sub, tile, e);
-  if (a.g_inj.t >= 0) li = load_points(a.g_inj.pts, &a.g_inj, tile, e);
+  PtsLoad ls{0, -1, 0, 0, 1, 0.f}, li{0, -1, 0, 0, 1, 0.f};
+  if (a.s_sub.t >= 0) ls = load_points(a.s_sub.pts, &a.s_sub, tile, e, x0, z0, g.ld);
+  if (a.g_inj.t >= 0) li = load_points(a.g_inj.pts, &a.g_inj, tile, e, x0, z0, g.ld);
   const bool sponge = touches_sponge(g, x0, x0 + TX, z0, z0 + kTZ);
   const bool band_tile = a.band != nullptr && touches_band<TX>(g, x0, z0);
   float sp[T::PER], gp[T::PER], rr[T::PER], gr[T::PER], dl[T::PER], bv[T::PER];
@@ -460,8 +476,14 @@ __device__ __forceinline__ void bwd_tile(const BwdStepArgs& a, int tile, int e, 
   if (smapped) mark_clear<TX>(smap);
   if (gmapped) mark_clear<TX>(gmap);
   __syncthreads();
-  if (smapped) mark_slots<TX>(s_sub, ls.n, g, x0, z0, smap);
-  if (gmapped) mark_slots<TX>(s_inj, li.n, g, x0, z0, gmap);
+  if (smapped) {
+    store_amp(ls, s_sub);
+    mark_slots(ls, s_sub, smap);
+  }
+  if (gmapped) {
+    store_amp(li, s_inj);
+    mark_slots(li, s_inj, gmap);
+  }
   form_u2<P, TX, FULL>(g, s_sc, s_su, x0, z0);
   form_u2<P, TX, FULL>(g, s_gc, s_gu, x0, z0);
   __syncthreads();
@@ -520,8 +542,8 @@ __device__ __forceinline__ void bwd_src_tile(const BwdStepArgs& a, int tile, int
   float* __restrict__ grad = a.grad + (long long)e * a.gstride;
   const int lz = threadIdx.x % kTZ, row = threadIdx.x / kTZ;
   const int iz = z0 + lz;
-  PtsLoad ls{0, -1, 0, 0.f};
-  if (a.s_sub.t >= 0) ls = load_points(a.s_sub.pts, &a.s_sub, tile, e);
+  PtsLoad ls{0, -1, 0, 0, 1, 0.f};
+  if (a.s_sub.t >= 0) ls = load_points(a.s_sub.pts, &a.s_sub, tile, e, x0, z0, g.ld);
   const bool band_tile = a.band != nullptr && touches_band<TX>(g, x0, z0);
   float sp[T::PER], rr[T::PER], gr[T::PER], gc[T::PER], bv[T::PER];
   unsigned inband = 0;
@@ -543,7 +565,10 @@ __device__ __forceinline__ void bwd_src_tile(const BwdStepArgs& a, int tile, int
   const bool mapped = ls.n > 0 && ls.n <= kTilePts;
   if (mapped) mark_clear<TX>(map);
   __syncthreads();
-  if (mapped) mark_slots<TX>(tp, ls.n, g, x0, z0, map);
+  if (mapped) {
+    store_amp(ls, tp);
+    mark_slots(ls, tp, map);
+  }
   form_u2<P, TX, FULL>(g, s_c, s_u, x0, z0);
   __syncthreads();
 #pragma unroll
@@ -576,8 +601,8 @@ __device__ __forceinline__ void bwd_rcv_tile(const BwdStepArgs& a, int tile, int
   float* __restrict__ g_prev = a.g_prev + fo;
   const int lz = threadIdx.x % kTZ, row = threadIdx.x / kTZ;
   const int iz = z0 + lz;
-  PtsLoad li{0, -1, 0, 0.f};
-  if (a.g_inj.t >= 0) li = load_points(a.g_inj.pts, &a.g_inj, tile, e);
+  PtsLoad li{0, -1, 0, 0, 1, 0.f};
+  if (a.g_inj.t >= 0) li = load_points(a.g_inj.pts, &a.g_inj, tile, e, x0, z0, g.ld);
   const bool sponge = touches_sponge(g, x0, x0 + TX, z0, z0 + kTZ);
   float gp[T::PER], rr[T::PER], dl[T::PER];
 #pragma unroll
@@ -594,7 +619,10 @@ __device__ __forceinline__ void bwd_rcv_tile(const BwdStepArgs& a, int tile, int
   const bool mapped = li.n > 0 && li.n <= kTilePts;
   if (mapped) mark_clear<TX>(map);
   __syncthreads();
-  if (mapped) mark_slots<TX>(tp, li.n, g, x0, z0, map);
+  if (mapped) {
+    store_amp(li, tp);
+    mark_slots(li, tp, map);
+  }
   form_u2<P, TX, FULL>(g, s_c, s_u, x0, z0);
   __syncthreads();
 #pragma unroll
