@@ -20,36 +20,46 @@ def rel_l2(a, b):
 def setup():
     if capi.device_count() < 1:
         pytest.skip("no HIP device")
-    from scipy.ndimage import gaussian_filter
     case = workloads.small_case(nx=48, nz=32, nb=8, nt=360, ns=3)
     vtrue, v0 = workloads.models(case)
-    rng = np.random.default_rng(21)
-    N = 3
-    mem = []
-    for _ in range(N):
-        f = gaussian_filter(rng.standard_normal(v0.shape), 3.0)
-        f = f / np.abs(f).max() * 0.03
-        mem.append(capi.expand_domain((v0 * (1 + f)).astype(np.float32), case.nb, case.dx, case.dt))
     wlt = capi.ricker(case.nt, case.fm, case.dt, 1.0)
     s = capi.Solver(case.nx, case.nz, case.nb, case.dt, case.dx, case.fm, case.nt, case.sx, case.sz, case.gx,
                     case.gz)
     s.bind_velocity(capi.expand_domain(vtrue, case.nb, case.dx, case.dt))
     dobs = s.forward_shots(wlt)
-    return case, s, np.stack(mem), wlt, dobs
+    return case, s, v0, wlt, dobs
 
 
-def test_ensemble_epochs_match_per_member_oracle(oracle, setup):
+def make_members(case, v0, scales):
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(21)
+    mem = []
+    for sc in scales:
+        f = gaussian_filter(rng.standard_normal(v0.shape), 3.0)
+        f = f / np.abs(f).max() * sc
+        mem.append(capi.expand_domain((v0 * (1 + f)).astype(np.float32), case.nb, case.dx, case.dt))
+    return np.stack(mem)
+
+
+@pytest.mark.parametrize("maxdv,scales", [(200.0, (0.03, 0.03, 0.03)), (1000.0, (0.0, 0.04, 0.12))])
+def test_ensemble_epochs_match_per_member_oracle(oracle, setup, maxdv, scales):
+    """Members' line searches of 2 and 6 trials (maxdv=200), and of uneven
+    lengths within one epoch (5, 5 and 6 trials at maxdv=1000 with members
+    0%, 4% and 12% away from the initial model): the lockstep rounds then
+    carry a different number of pending trials per member."""
     from swfwi_amd.enkf import Enkf, EnsembleFwi
-    case, s, members, wlt, dobs = setup
+    case, s, v0, wlt, dobs = setup
+    members = make_members(case, v0, scales)
     N = members.shape[0]
     e = Enkf(s, wlt, dobs, 0.2, N)
     e.set_members(members)
-    fwi = EnsembleFwi(e, 1400.0, 3600.0, 3, 200.0)
+    fwi = EnsembleFwi(e, 1400.0, 3600.0, 3, maxdv)
     g = oracle.Geom(case.nx, case.nz, case.nb, case.dt, case.dx, case.fm, case.nt, case.sx, case.sz, case.gx,
                     case.gz)
     oracle.set_sum_mode(1)  # FP64 accumulation, as the GPU reductions
     try:
-        ref = [oracle.Ess(g, members[i].copy(), wlt, dobs, 1400.0, 3600.0, 3, 200.0) for i in range(N)]
+        ref = [oracle.Ess(g, members[i].copy(), wlt, dobs, 1400.0, 3600.0, 3, maxdv) for i in range(N)]
+        rounds = []
         for it in range(2):
             print(f"ensemble epoch {it} N={N}", flush=True)
             fwi.epoch(it)
@@ -66,6 +76,8 @@ def test_ensemble_epochs_match_per_member_oracle(oracle, setup):
             # one batched trial run per lockstep round; each member's trials counted once
             assert st["trials"] == sum(int(m["ntrial"]) for m in st["members"])
             assert st["rounds"] >= 1
+            rounds.append(st["rounds"])
+        print(f"maxdv={maxdv}: lockstep rounds per epoch {rounds}", flush=True)
         # the members took different steps (they are different models)
         assert len({round(m["steplen"], 12) for m in st["members"]}) > 1
 

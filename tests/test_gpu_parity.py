@@ -229,9 +229,11 @@ def test_crowded_tile_falls_back_to_point_lists(oracle, split):
         s.set_bwd_split(-1)
 
 
-def test_fp32_mode_within_budget(oracle, small):
+@pytest.mark.parametrize("split", [0, 1])
+def test_fp32_mode_within_budget(oracle, small, split):
     case, s, g, vt, vi, wlt = small
     s.set_precision(capi.PREC_FP32)
+    s.set_bwd_split(split)
     try:
         d_gpu = s.forward_modeling(wlt, shot=1)
         d_cpu = g.forward_modeling(vt, wlt, shot=1)
@@ -244,7 +246,8 @@ def test_fp32_mode_within_budget(oracle, small):
         e2 = rel_l2(gg[sl], gc[sl])
     finally:
         s.set_precision(capi.PREC_EXACT)
-    print(f"fp32 seismogram relL2={e1:.3e} gradient relL2={e2:.3e}", flush=True)
+        s.set_bwd_split(-1)
+    print(f"fp32 split={split} seismogram relL2={e1:.3e} gradient relL2={e2:.3e}", flush=True)
     assert e1 <= SEIS_TOL
     assert e2 <= GRAD_TOL
 
