@@ -87,6 +87,11 @@ int swfwi_solver_set_precision(swfwi_solver h, int precision);
 /* Damp4t10d::bindVelocity (damp4t10d.cpp:159-161): padded transformed model */
 int swfwi_solver_bind_velocity(swfwi_solver h, const float *vel_t);
 int swfwi_solver_get_velocity(swfwi_solver h, float *vel_t);
+/* Damp4t10d::writeBndry / readBndry for one step (damp4t10d.cpp:478-536):
+ * the 6-wide boundary band of a padded field [nxpad][nzpad] to bndr[dims[4]]
+ * and back (read_bndry updates only the band points of `field`). */
+int swfwi_write_bndry(swfwi_solver h, const float *field, float *bndr);
+int swfwi_read_bndry(swfwi_solver h, const float *bndr, float *field);
 /* FwiForwardModeling (shot >= 0; encsrc[nt], damp4t10d.cpp:272-289) or
  * EssForwardModeling (shot < 0; encsrc[nt][ns], damp4t10d.cpp:291-307). */
 int swfwi_forward_modeling(swfwi_solver h, const float *encsrc, int shot, float *dcal);

@@ -40,6 +40,8 @@ SIGNATURES = {
     "swfwi_solver_bind_velocity": (_I, [_VP, _f32]),
     "swfwi_solver_get_velocity": (_I, [_VP, _f32]),
     "swfwi_forward_modeling": (_I, [_VP, _f32, _I, _f32]),
+    "swfwi_write_bndry": (_I, [_VP, _f32, _f32]),
+    "swfwi_read_bndry": (_I, [_VP, _f32, _f32]),
     "swfwi_calgradient": (_I, [_VP, _f32, _I, _f32, _f32]),
     "swfwi_forward_shots": (_I, [_VP, _f32, _f32]),
     "swfwi_solver_counters": (_I, [_VP, _f64, np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")]),
@@ -198,6 +200,18 @@ class Solver:
         v = np.zeros(self.shape, np.float32)
         check(load().swfwi_solver_get_velocity(self.h, v))
         return v
+
+    def write_bndry(self, field):
+        """Damp4t10d::writeBndry of one step: the field's boundary band."""
+        b = np.zeros(self.bndr_size, np.float32)
+        check(load().swfwi_write_bndry(self.h, _f(field), b))
+        return b
+
+    def read_bndry(self, bndr, field):
+        """Damp4t10d::readBndry: `field` with its band replaced by `bndr`."""
+        out = _f(field).copy()
+        check(load().swfwi_read_bndry(self.h, _f(bndr), out))
+        return out
 
     def forward_modeling(self, encsrc, shot=-1):
         dcal = np.zeros((self.nt, self.ng), np.float32)

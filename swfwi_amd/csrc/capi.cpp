@@ -277,6 +277,36 @@ int swfwi_solver_get_velocity(swfwi_solver h, float* vel_t) {
   });
 }
 
+int swfwi_write_bndry(swfwi_solver h, const float* field, float* bndr) {
+  return guarded([&] {
+    NEED(h);
+    NEED(field);
+    NEED(bndr);
+    Solver& s = *h->s;
+    const Geometry& g = s.geo();
+    s.tmp.upload(field, (size_t)s.npts(), s.stream());
+    s.band.resize((size_t)g.bsize);
+    op_band_pack(s.tmp.get(), s.band.get(), g.bsize, g.gridgeom(), s.stream());
+    s.band.download(bndr, (size_t)g.bsize, s.stream());
+    s.prop().sync();
+  });
+}
+
+int swfwi_read_bndry(swfwi_solver h, const float* bndr, float* field) {
+  return guarded([&] {
+    NEED(h);
+    NEED(field);
+    NEED(bndr);
+    Solver& s = *h->s;
+    const Geometry& g = s.geo();
+    s.tmp.upload(field, (size_t)s.npts(), s.stream());
+    s.band.upload(bndr, (size_t)g.bsize, s.stream());
+    op_band_unpack(s.band.get(), s.tmp.get(), g.bsize, g.gridgeom(), s.stream());
+    s.tmp.download(field, (size_t)s.npts(), s.stream());
+    s.prop().sync();
+  });
+}
+
 int swfwi_forward_modeling(swfwi_solver h, const float* encsrc, int shot, float* dcal) {
   return guarded([&] {
     NEED(h);

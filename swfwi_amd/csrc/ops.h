@@ -64,6 +64,11 @@ void op_update_vel(const float* v, const float* dir, const float* steps, int E, 
 // refillBoundary (damp4t10d.cpp:228-252), in place, E models.
 void op_refill(float* a, long long stride, int E, const GridGeom& g, hipStream_t s);
 
+// writeBndry / readBndry of one step (damp4t10d.cpp:478-536): the 6-wide
+// band of a padded field to band[bsize] and back.
+void op_band_pack(const float* field, float* band, int bsize, const GridGeom& g, hipStream_t s);
+void op_band_unpack(const float* band, float* field, int bsize, const GridGeom& g, hipStream_t s);
+
 // rv = 1/vel for E models.
 void op_reciprocal(const float* vel, float* rv, long long n, hipStream_t s);
 

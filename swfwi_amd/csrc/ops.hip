@@ -277,6 +277,33 @@ __global__ void k_refill(float* a, long long stride, int E, GridGeom g) {
   }
 }
 
+// The 6-wide boundary band of writeBndry/readBndry (damp4t10d.cpp:478-536):
+// band offset i -> grid point, bottom rows first (iz fastest), then the left
+// and the right columns (iz fastest).
+__device__ __forceinline__ long long band_point(const GridGeom& g, int i) {
+  const int w = 6;
+  const int nxb = g.nx - (g.bx0 - w + g.bxn - w), nzb = g.nz - g.bz0 - g.bzn;
+  int ix, iz;
+  if (i < w * nxb) {
+    ix = g.bx0 - w + i / w;
+    iz = g.nz - g.bzn + i % w;
+  } else {
+    const int k = i - w * nxb;
+    const int col = k / nzb;
+    iz = g.bz0 + k % nzb;
+    ix = col < w ? g.bx0 - w + col : g.nx - g.bxn + (col - w);
+  }
+  return (long long)ix * g.ld + iz;
+}
+__global__ void k_band_pack(const float* p, float* band, int bsize, GridGeom g) {
+  const int i = blockIdx.x * kB + threadIdx.x;
+  if (i < bsize) band[i] = p[band_point(g, i)];
+}
+__global__ void k_band_unpack(const float* band, float* p, int bsize, GridGeom g) {
+  const int i = blockIdx.x * kB + threadIdx.x;
+  if (i < bsize) p[band_point(g, i)] = band[i];
+}
+
 __global__ void k_reciprocal2(const float* vel, float* rv, long long n) {
   for (long long i = blockIdx.x * (long long)kB + threadIdx.x; i < n; i += (long long)gridDim.x * kB)
     rv[i] = 1.0f / vel[i];
@@ -412,6 +439,12 @@ void op_update_vel(const float* v, const float* dir, const float* steps, int E, 
 }
 void op_refill(float* a, long long stride, int E, const GridGeom& g, hipStream_t s) {
   hipLaunchKernelGGL(k_refill, dim3(nblocks((long long)g.nx * g.ld * E)), dim3(kB), 0, s, a, stride, E, g);
+}
+void op_band_pack(const float* field, float* band, int bsize, const GridGeom& g, hipStream_t s) {
+  if (bsize > 0) hipLaunchKernelGGL(k_band_pack, dim3((bsize + kB - 1) / kB), dim3(kB), 0, s, field, band, bsize, g);
+}
+void op_band_unpack(const float* band, float* field, int bsize, const GridGeom& g, hipStream_t s) {
+  if (bsize > 0) hipLaunchKernelGGL(k_band_unpack, dim3((bsize + kB - 1) / kB), dim3(kB), 0, s, band, field, bsize, g);
 }
 void op_reciprocal(const float* vel, float* rv, long long n, hipStream_t s) {
   hipLaunchKernelGGL(k_reciprocal2, dim3(nblocks(n)), dim3(kB), 0, s, vel, rv, n);
