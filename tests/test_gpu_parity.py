@@ -319,8 +319,8 @@ def test_fwi_epoch_matches_oracle(oracle, small):
 
 
 def test_fwi_201x301_shot_full_size(oracle):
-    """One config-0 shot at full size (201x301, nb=20, nt=2000): seismogram and
-    raw gradient bit-identical to the oracle."""
+    """One shot of the 201x301 workload at full size (nb=20, nt=2000):
+    seismogram and raw gradient bit-identical to the oracle."""
     case = workloads.fwi_201x301()
     s, g, vt, vi, wlt = setup_case(oracle, case)
     s.bind_velocity(vi)
@@ -334,3 +334,34 @@ def test_fwi_201x301_shot_full_size(oracle):
     g_gpu = s.calgradient(wlt, vsrc, shot=5)
     g_cpu = g.calgradient(vi, wlt, vsrc, shot=5)
     assert np.array_equal(bits(g_gpu), bits(g_cpu))
+
+
+def test_ess_301x921_batched_shots_and_epoch(oracle):
+    """The 301x921 grid (nb=30, 60 shots, 921 receivers) at full width, nt cut
+    to 800 so the oracle finishes in seconds: the 60-shot batched forward
+    (the 64x32 tile path) bit-identical for the first and last shot, and one
+    ESS epoch (E=1 forward with band save, split backward, E=2 trials) against
+    the FP64-sum oracle."""
+    case = workloads.ess_301x921(nt=800)
+    s, g, vt, vi, wlt = setup_case(oracle, case)
+    s.bind_velocity(vt)
+    print("ess_301x921 shots", flush=True)
+    dobs = s.forward_shots(wlt)
+    for shot in (0, case.ns - 1):
+        assert np.array_equal(bits(dobs[shot]), bits(g.forward_modeling(vt, wlt, shot=shot))), shot
+    oracle.set_sum_mode(1)
+    try:
+        s.bind_velocity(vi)
+        inv = capi.Inversion(s, wlt, dobs, case.vmin, case.vmax, 5, 200.0)
+        e = oracle.Ess(g, vi.copy(), wlt, dobs, case.vmin, case.vmax, 5, 200.0)
+        print("ess_301x921 epoch", flush=True)
+        inv.ess_epoch(0)
+        e.epoch(0)
+        a, b = inv.stats(), e.scalars()
+        for k in ("obj1", "steplen", "alpha2", "alpha3", "obj2", "obj3", "updateobj", "ntrial", "parabolic"):
+            assert abs(a[k] - b[k]) <= 1e-6 * max(abs(b[k]), 1e-30), (k, a[k], b[k])
+        assert rel_l2(inv.gradient(), e.grad()) < 1e-6
+        assert rel_l2(s.velocity(), e.vel) < 1e-6
+        inv.close()
+    finally:
+        oracle.set_sum_mode(0)
