@@ -131,8 +131,20 @@ __device__ __forceinline__ int band_index(const Grid& g, int ix, int iz) {
   return -1;
 }
 
-__device__ __forceinline__ float damping_delta(const Grid& g, int ix, int iz) {
-  return iz >= g.zdamp ? g.delta_z[iz] : g.delta_x[ix];
+// The damping delta of a point, in halves: the z table entry of the
+// thread's row (one per thread), the x table entry of a column (uniform over
+// a wavefront, whose 64 lanes share ix: a scalar load through the constant
+// address space, the tables being read-only while the kernels run), and the
+// pick at the use, so nothing waits on the loads while others are issued.
+// Indices are clamped into the tables: a point past the grid edge of a
+// partial tile reads an entry it never uses.
+__device__ __forceinline__ float delta_z_load(const Grid& g, int iz) { return g.delta_z[min(iz, g.nz - 1)]; }
+__device__ __forceinline__ float delta_x_load(const Grid& g, int ix) {
+  using ConstFloat = __attribute__((address_space(4))) const float;
+  return ((ConstFloat*)g.delta_x)[min(ix, g.nx - 1)];
+}
+__device__ __forceinline__ float delta_pick(const Grid& g, int iz, float dz, float dx) {
+  return iz >= g.zdamp ? dz : dx;
 }
 
 // Whether any point of [x0, x1) x [z0, z1) lies in the sponge (delta != 0:
@@ -153,26 +165,37 @@ __device__ __forceinline__ void block_work(int ntiles, int nentries, int& tile, 
   e = l - tile * nentries;
 }
 
-// Stage cur (tile + halo) into LDS, zero outside the grid; all loads of the
-// thread are issued before the first LDS store.  FULL: the staged region lies
-// inside the grid (every tile but the last row and column), no checks.
+// Stage cur (tile + halo) into LDS, zero outside the grid, in two halves so
+// that other loads can be issued between them: stage_load issues every load
+// of the thread (no branch: a slot past the staged region or outside the grid
+// reads element 0 and drops it), stage_store writes them to LDS.  FULL: the
+// staged region lies inside the grid (every tile but the last row and
+// column), no checks.
+template <int TX>
+struct Staged {
+  float v[Tile<TX>::NSTAGE];
+};
 template <int TX, bool FULL>
-__device__ __forceinline__ void stage_tile(const Grid& g, const float* __restrict__ src, float* __restrict__ lds,
-                                           int x0, int z0) {
+__device__ __forceinline__ void stage_load(const Grid& g, const float* __restrict__ src, int x0, int z0,
+                                           Staged<TX>& st) {
   using T = Tile<TX>;
-  float v[T::NSTAGE];
 #pragma unroll
   for (int k = 0; k < T::NSTAGE; k++) {
     const int i = threadIdx.x + k * kThreads;
     const int lx = i / T::LZ, lz = i - lx * T::LZ;
     const int gx = x0 - kHalo + lx, gz = z0 - kHalo + lz;
     const bool ok = i < T::LX * T::LZ && (FULL || (gx >= 0 && gx < g.nx && gz >= 0 && gz < g.nz));
-    v[k] = ok ? src[(unsigned)(gx * g.ld + gz)] : 0.f;
+    const float v = src[ok ? (unsigned)(gx * g.ld + gz) : 0u];
+    st.v[k] = ok ? v : 0.f;
   }
+}
+template <int TX>
+__device__ __forceinline__ void stage_store(const Staged<TX>& st, float* __restrict__ lds) {
+  using T = Tile<TX>;
 #pragma unroll
   for (int k = 0; k < T::NSTAGE; k++) {
     const int i = threadIdx.x + k * kThreads;
-    if (i < T::LX * T::LZ) lds[i] = v[k];
+    if (i < T::LX * T::LZ) lds[i] = st.v[k];
   }
 }
 
@@ -223,42 +246,65 @@ struct TilePts {
   float amp[kTilePts];
 };
 
-// One thread's share of a tile's point list: threads 0..n-1 load one item
-// each before the staging (n is the tile's item count, uniform over the
-// block): grid index (-1 when inactive for this entry), id, amplitude,
-// position in the tile and whether another point of the tile shares the grid
-// index.
+// One thread's share of a tile's point list: threads 0..n-1 hold one item
+// each (n is the tile's item count, uniform over the block): grid index (-1
+// when inactive for this entry), id, amplitude, position in the tile and
+// whether another point of the tile shares the grid index.
 struct PtsLoad {
   int n, q, pid, loc, shared;
   float amp;
 };
+constexpr PtsLoad kNoPts{0, -1, 0, 0, 1, 0.f};
+
+// A slot list is read in two levels so that its latency hides behind the
+// field loads.  fetch_slots, before them: the tile's count and one slot per
+// thread (threads past kTilePts re-read slot threadIdx % kTilePts and drop
+// it; empty slots hold grid index -1 and id 0), and the entry's id range.
+// finish_slots, once the field loads are in flight: the staged item and, for
+// an injection, its amplitude at this step (only when the tile has points,
+// so the amplitude table exists; an empty slot reads point 0's and drops it).
+struct SlotFetch {
+  int n, lo, hi;
+  int4 sl;
+};
+__device__ __forceinline__ SlotFetch fetch_slots(const PointList& pts, const Inject* in, int tile, int e) {
+  SlotFetch f;
+  f.n = pts.count[tile];
+  f.sl = pts.slots[(long long)tile * kTilePts + (threadIdx.x & (kTilePts - 1))];
+  f.lo = 0;
+  f.hi = 0x7fffffff;
+  if (in && in->lo) {
+    f.lo = in->lo[e];
+    f.hi = in->hi[e];
+  }
+  return f;
+}
+__device__ __forceinline__ PtsLoad finish_slots(const SlotFetch& f, const Inject* in, int e) {
+  PtsLoad r;
+  r.n = f.n;
+  r.pid = f.sl.y;
+  r.loc = f.sl.z;
+  r.shared = f.sl.w;
+  const bool mine = (int)threadIdx.x < f.n;
+  r.q = mine && f.sl.y >= f.lo && f.sl.y < f.hi ? f.sl.x : -1;
+  r.amp = 0.f;
+  if (in && f.n > 0) r.amp = in->amp[e * in->estride + (long long)in->t * in->ld + f.sl.y];
+  return r;
+}
+
+// A tile's items from the CSR list (tiles of more than kTilePts items make
+// the whole list use it): offsets, then ids, then grid indices and
+// amplitudes, each level waiting on the one before.
 __device__ __forceinline__ PtsLoad load_points(const PointList& pts, const Inject* in, int tile, int e, int x0, int z0,
                                                int ld) {
-  PtsLoad r;
-  r.q = -1;
-  r.pid = 0;
-  r.loc = 0;
-  r.shared = 1;
-  r.amp = 0.f;
-  if (pts.slots) {
-    // count and slot in parallel (slots past the count hold -1)
-    r.n = pts.count[tile];
-    if ((int)threadIdx.x < kTilePts) {
-      const int4 sl = pts.slots[(long long)tile * kTilePts + threadIdx.x];
-      r.q = sl.x;
-      r.pid = sl.y;
-      r.loc = sl.z;
-      r.shared = sl.w;
-    }
-  } else {
-    const int b = pts.tile_ptr[tile];
-    r.n = pts.tile_ptr[tile + 1] - b;
-    if (r.n <= kTilePts && (int)threadIdx.x < r.n) {
-      r.pid = pts.items[b + threadIdx.x];
-      r.q = pts.idx[r.pid];
-      const int ix = r.q / ld;
-      r.loc = (ix - x0) * kTZ + (r.q - ix * ld - z0);
-    }
+  PtsLoad r = kNoPts;
+  const int b = pts.tile_ptr[tile];
+  r.n = pts.tile_ptr[tile + 1] - b;
+  if (r.n <= kTilePts && (int)threadIdx.x < r.n) {
+    r.pid = pts.items[b + threadIdx.x];
+    r.q = pts.idx[r.pid];
+    const int ix = r.q / ld;
+    r.loc = (ix - x0) * kTZ + (r.q - ix * ld - z0);
   }
   if (in && r.n <= kTilePts && (int)threadIdx.x < r.n) {
     if (entry_active(*in, e, r.pid))
@@ -268,6 +314,32 @@ __device__ __forceinline__ PtsLoad load_points(const PointList& pts, const Injec
   }
   return r;
 }
+
+// A tile's point list for one launch: slots when the list has them (two
+// levels around the field loads), else the CSR scan (at the start).  LEAN:
+// the launch guarantees that every active list has slots (no CSR code).
+struct TileList {
+  bool on, slots;
+  SlotFetch f;
+  PtsLoad r;
+};
+template <bool LEAN>
+__device__ __forceinline__ TileList list_begin(const PointList& pts, const Inject* in, bool on, int tile, int e,
+                                               int x0, int z0, int ld) {
+  TileList l;
+  l.on = on;
+  l.slots = on && (LEAN || pts.slots != nullptr);
+  l.r = kNoPts;
+  if (l.slots)
+    l.f = fetch_slots(pts, in, tile, e);
+  else if (!LEAN && on)
+    l.r = load_points(pts, in, tile, e, x0, z0, ld);
+  return l;
+}
+__device__ __forceinline__ PtsLoad list_end(const TileList& l, const Inject* in, int e) {
+  return l.slots ? finish_slots(l.f, in, e) : l.r;
+}
+
 // Grid index and id go to LDS before the first barrier; the amplitude (the
 // second level of loads) after it.
 __device__ __forceinline__ void store_points(const PtsLoad& r, TilePts& tp) {
@@ -330,7 +402,7 @@ __device__ __forceinline__ float apply_points(const Inject& in, int tile, int e,
   return v;
 }
 
-template <bool DAMPED, int P, int TX, bool FULL>
+template <bool DAMPED, int P, int TX, bool FULL, bool LEAN>
 __device__ __forceinline__ void fwd_tile(const FwdStepArgs& a, int tile, int e, int x0, int z0, float* s_cur,
                                          float* s_u2, TilePts& s_inj, TilePts& s_rec, signed char* imap) {
   using T = Tile<TX>;
@@ -339,28 +411,36 @@ __device__ __forceinline__ void fwd_tile(const FwdStepArgs& a, int tile, int e, 
   float* __restrict__ prev = a.prev + fo;
   const float* __restrict__ rv = a.rv + (long long)(e / a.rvdiv) * a.rvstride;
 
-  const int lz = threadIdx.x % kTZ, row = threadIdx.x / kTZ;
+  const int lz = threadIdx.x % kTZ, row = __builtin_amdgcn_readfirstlane(threadIdx.x / kTZ);
   const int iz = z0 + lz;
-  // every global load of the block is issued here, before the first barrier
-  PtsLoad li{0, -1, 0, 0, 1, 0.f}, lr{0, -1, 0, 0, 1, 0.f};
-  if (a.inj.t >= 0) li = load_points(a.inj.pts, &a.inj, tile, e, x0, z0, g.ld);
-  if (a.rec.out != nullptr) lr = load_points(a.rec.pts, nullptr, tile, e, x0, z0, g.ld);
+  // every global load of the block is issued here, before the first barrier,
+  // and no branch waits on one: the point lists' first level, the thread's
+  // own points (a point past the grid edge of a partial tile reads element 0
+  // and drops it), the staged tile, then the lists' second level
+  const TileList ti = list_begin<LEAN>(a.inj.pts, &a.inj, a.inj.t >= 0, tile, e, x0, z0, g.ld);
+  const TileList tr = list_begin<LEAN>(a.rec.pts, nullptr, a.rec.out != nullptr, tile, e, x0, z0, g.ld);
   const bool sponge = DAMPED && touches_sponge(g, x0, x0 + TX, z0, z0 + kTZ);
   const bool band_tile = a.band != nullptr && touches_band<TX>(g, x0, z0);
-  float pv[T::PER], rr[T::PER], dl[T::PER];
+  // (values of points past the grid edge are loaded from element 0 and
+  // never used: the update loop skips those points)
+  float pv[T::PER], rr[T::PER], dx[T::PER] = {}, dz = 0.f;
+  if (sponge) dz = delta_z_load(g, iz);
 #pragma unroll
   for (int k = 0; k < T::PER; k++) {
     const int ix = x0 + row + kRows * k;
     const bool ok = FULL || (ix < g.nx - kHalo && iz < g.nz - kHalo);
-    const unsigned q = (unsigned)(ix * g.ld + iz);
-    pv[k] = ok ? prev[q] : 0.f;
-    rr[k] = ok ? rv[q] : 1.f;
-    dl[k] = (sponge && ok) ? damping_delta(g, ix, iz) : 0.f;
+    const unsigned q = ok ? (unsigned)(ix * g.ld + iz) : 0u;
+    pv[k] = prev[q];
+    rr[k] = rv[q];
+    if (sponge) dx[k] = delta_x_load(g, ix);
   }
-  stage_tile<TX, FULL>(g, a.cur + fo, s_cur, x0, z0);
+  Staged<TX> st;
+  stage_load<TX, FULL>(g, a.cur + fo, x0, z0, st);
+  const PtsLoad li = list_end(ti, &a.inj, e), lr = list_end(tr, nullptr, e);
+  stage_store<TX>(st, s_cur);
   store_points(li, s_inj);
   store_points(lr, s_rec);
-  const bool mapped = li.n > 0 && li.n <= kTilePts;
+  const bool mapped = li.n > 0 && (LEAN || li.n <= kTilePts);
   if (mapped) mark_clear<TX>(imap);
   __syncthreads();
   if (mapped) {
@@ -380,13 +460,13 @@ __device__ __forceinline__ void fwd_tile(const FwdStepArgs& a, int tile, int e, 
     const unsigned q = (unsigned)(ix * g.ld + iz);
     const float c = s_cur[(lx + kHalo) * T::LZ + lz + kHalo];
     const float* u = s_u2 + (lx + 1) * T::UZ + lz + 1;
-    if (a.u2out) a.u2out[fo + q] = u[0];
-    float nxt = sponge ? update_at<true, P, T::UZ>(c, pv[k], rr[k], u, dl[k])
+    if (!LEAN && a.u2out) a.u2out[fo + q] = u[0];
+    float nxt = sponge ? update_at<true, P, T::UZ>(c, pv[k], rr[k], u, delta_pick(g, iz, dz, dx[k]))
                        : update_at<false, P, T::UZ>(c, pv[k], rr[k], u, 0.f);
     if (mapped) {
       const int h = imap[lx * kTZ + lz];
       if (h >= 0) nxt = apply_mapped(s_inj, li.n, h, a.inj.sign, q, nxt);
-    } else if (li.n > 0) {
+    } else if (!LEAN && li.n > 0) {
       nxt = apply_points(a.inj, tile, e, q, nxt);
     }
     prev[q] = nxt;
@@ -394,7 +474,7 @@ __device__ __forceinline__ void fwd_tile(const FwdStepArgs& a, int tile, int e, 
       const int bi = band_index(g, ix, iz);
       if (bi >= 0) a.band[(long long)e * a.band_estride + bi] = c;
     }
-    if (lr.n > kTilePts) {
+    if (!LEAN && lr.n > kTilePts) {
       const int b = a.rec.pts.tile_ptr[tile], eidx = a.rec.pts.tile_ptr[tile + 1];
       for (int j = b; j < eidx; j++) {
         const int pid = a.rec.pts.items[j];
@@ -404,7 +484,7 @@ __device__ __forceinline__ void fwd_tile(const FwdStepArgs& a, int tile, int e, 
   }
   // u2 of the ring [5,6) and [n-6,n-5) that no output point owns: the
   // reference writes it too, so the optional u2 export covers it.
-  if (a.u2out) {
+  if (!LEAN && a.u2out) {
     for (int i = threadIdx.x; i < T::UX * T::UZ; i += kThreads) {
       const int ux = i / T::UZ, uz = i - ux * T::UZ;
       const int gx = x0 - 1 + ux, gz = z0 - 1 + uz;
@@ -416,7 +496,7 @@ __device__ __forceinline__ void fwd_tile(const FwdStepArgs& a, int tile, int e, 
   }
 }
 
-template <bool DAMPED, int P, int TX>
+template <bool DAMPED, int P, int TX, bool LEAN>
 __global__ __launch_bounds__(kThreads) void k_fwd_step(FwdStepArgs a, int nentries) {
   using T = Tile<TX>;
   __shared__ float s_cur[T::LX * T::LZ];
@@ -427,12 +507,12 @@ __global__ __launch_bounds__(kThreads) void k_fwd_step(FwdStepArgs a, int nentri
   block_work(a.g.ntz * a.g.ntx, nentries, tile, e);
   const int z0 = kHalo + (tile % a.g.ntz) * kTZ, x0 = kHalo + (tile / a.g.ntz) * TX;
   if (tile_is_full<TX>(a.g, x0, z0))
-    fwd_tile<DAMPED, P, TX, true>(a, tile, e, x0, z0, s_cur, s_u2, s_inj, s_rec, imap);
+    fwd_tile<DAMPED, P, TX, true, LEAN>(a, tile, e, x0, z0, s_cur, s_u2, s_inj, s_rec, imap);
   else
-    fwd_tile<DAMPED, P, TX, false>(a, tile, e, x0, z0, s_cur, s_u2, s_inj, s_rec, imap);
+    fwd_tile<DAMPED, P, TX, false, LEAN>(a, tile, e, x0, z0, s_cur, s_u2, s_inj, s_rec, imap);
 }
 
-template <int P, int TX, bool FULL>
+template <int P, int TX, bool FULL, bool LEAN>
 __device__ __forceinline__ void bwd_tile(const BwdStepArgs& a, int tile, int e, int x0, int z0, float* s_sc,
                                          float* s_gc, float* s_su, float* s_gu, TilePts& s_sub, TilePts& s_inj,
                                          signed char* smap, signed char* gmap) {
@@ -444,35 +524,44 @@ __device__ __forceinline__ void bwd_tile(const BwdStepArgs& a, int tile, int e, 
   float* __restrict__ g_prev = a.g_prev + fo;
   float* __restrict__ grad = a.grad + (long long)e * a.gstride;
 
-  const int lz = threadIdx.x % kTZ, row = threadIdx.x / kTZ;
+  const int lz = threadIdx.x % kTZ, row = __builtin_amdgcn_readfirstlane(threadIdx.x / kTZ);
   const int iz = z0 + lz;
-  // every global load of the block is issued here, before the first barrier
-  PtsLoad ls{0, -1, 0, 0, 1, 0.f}, li{0, -1, 0, 0, 1, 0.f};
-  if (a.s_sub.t >= 0) ls = load_points(a.s_sub.pts, &a.s_sub, tile, e, x0, z0, g.ld);
-  if (a.g_inj.t >= 0) li = load_points(a.g_inj.pts, &a.g_inj, tile, e, x0, z0, g.ld);
+  // every global load of the block is issued here, before the first barrier,
+  // and no branch waits on one (as in the forward tile)
+  const TileList ts = list_begin<LEAN>(a.s_sub.pts, &a.s_sub, a.s_sub.t >= 0, tile, e, x0, z0, g.ld);
+  const TileList tg = list_begin<LEAN>(a.g_inj.pts, &a.g_inj, a.g_inj.t >= 0, tile, e, x0, z0, g.ld);
   const bool sponge = touches_sponge(g, x0, x0 + TX, z0, z0 + kTZ);
   const bool band_tile = a.band != nullptr && touches_band<TX>(g, x0, z0);
-  float sp[T::PER], gp[T::PER], rr[T::PER], gr[T::PER], dl[T::PER], bv[T::PER];
+  // (values of points past the grid edge are loaded from element 0 and
+  // never used: the update loop skips those points; bv is used only in band)
+  float sp[T::PER], gp[T::PER], rr[T::PER], gr[T::PER] = {}, dx[T::PER] = {}, bv[T::PER] = {}, dz = 0.f;
   unsigned inband = 0;
+  if (sponge) dz = delta_z_load(g, iz);
 #pragma unroll
   for (int k = 0; k < T::PER; k++) {
     const int ix = x0 + row + kRows * k;
     const bool ok = FULL || (ix < g.nx - kHalo && iz < g.nz - kHalo);
-    const unsigned q = (unsigned)(ix * g.ld + iz);
-    sp[k] = ok ? s_prev[q] : 0.f;
-    gp[k] = ok ? g_prev[q] : 0.f;
-    rr[k] = ok ? rv[q] : 1.f;
-    gr[k] = (ok && a.do_xcorr) ? grad[q] : 0.f;
-    dl[k] = (sponge && ok) ? damping_delta(g, ix, iz) : 0.f;
-    const int bi = (ok && band_tile) ? band_index(g, ix, iz) : -1;
-    bv[k] = bi >= 0 ? a.band[(long long)e * a.band_estride + bi] : 0.f;
-    if (bi >= 0) inband |= 1u << k;
+    const unsigned q = ok ? (unsigned)(ix * g.ld + iz) : 0u;
+    sp[k] = s_prev[q];
+    gp[k] = g_prev[q];
+    rr[k] = rv[q];
+    if (a.do_xcorr) gr[k] = grad[q];
+    if (sponge) dx[k] = delta_x_load(g, ix);
+    if (band_tile) {
+      const int bi = ok ? band_index(g, ix, iz) : -1;
+      bv[k] = a.band[(long long)e * a.band_estride + (bi >= 0 ? bi : 0)];
+      if (bi >= 0) inband |= 1u << k;
+    }
   }
-  stage_tile<TX, FULL>(g, a.s_cur + fo, s_sc, x0, z0);
-  stage_tile<TX, FULL>(g, a.g_cur + fo, s_gc, x0, z0);
+  Staged<TX> st_s, st_g;
+  stage_load<TX, FULL>(g, a.s_cur + fo, x0, z0, st_s);
+  stage_load<TX, FULL>(g, a.g_cur + fo, x0, z0, st_g);
+  const PtsLoad ls = list_end(ts, &a.s_sub, e), li = list_end(tg, &a.g_inj, e);
+  stage_store<TX>(st_s, s_sc);
+  stage_store<TX>(st_g, s_gc);
   store_points(ls, s_sub);
   store_points(li, s_inj);
-  const bool smapped = ls.n > 0 && ls.n <= kTilePts, gmapped = li.n > 0 && li.n <= kTilePts;
+  const bool smapped = ls.n > 0 && (LEAN || ls.n <= kTilePts), gmapped = li.n > 0 && (LEAN || li.n <= kTilePts);
   if (smapped) mark_clear<TX>(smap);
   if (gmapped) mark_clear<TX>(gmap);
   __syncthreads();
@@ -501,12 +590,12 @@ __device__ __forceinline__ void bwd_tile(const BwdStepArgs& a, int tile, int e, 
     if (smapped) {
       const int h = smap[lx * kTZ + lz];
       if (h >= 0) sn = apply_mapped(s_sub, ls.n, h, a.s_sub.sign, q, sn);
-    } else if (ls.n > 0) {
+    } else if (!LEAN && ls.n > 0) {
       sn = apply_points(a.s_sub, tile, e, q, sn);
     }
     // receiver field: stepForward (damped) of the already injected cur
     const float gc = s_gc[l];
-    float gn = sponge ? update_at<true, P, T::UZ>(gc, gp[k], rr[k], s_gu + lu, dl[k])
+    float gn = sponge ? update_at<true, P, T::UZ>(gc, gp[k], rr[k], s_gu + lu, delta_pick(g, iz, dz, dx[k]))
                       : update_at<false, P, T::UZ>(gc, gp[k], rr[k], s_gu + lu, 0.f);
     // zero-lag imaging condition with the new source field and the injected
     // receiver field (cross_correlation, fwiframework.cpp:107-112)
@@ -517,7 +606,7 @@ __device__ __forceinline__ void bwd_tile(const BwdStepArgs& a, int tile, int e, 
     if (gmapped) {
       const int h = gmap[lx * kTZ + lz];
       if (h >= 0) gn = apply_mapped(s_inj, li.n, h, a.g_inj.sign, q, gn);
-    } else if (li.n > 0) {
+    } else if (!LEAN && li.n > 0) {
       gn = apply_points(a.g_inj, tile, e, q, gn);
     }
     s_prev[q] = sn;
@@ -530,7 +619,7 @@ __device__ __forceinline__ void bwd_tile(const BwdStepArgs& a, int tile, int e, 
 // half (undamped step, subSource, imaging condition with the receiver field
 // read at the point, band restore) and the receiver half (damped step,
 // receiver injection).  Each half stages one field.
-template <int P, int TX, bool FULL>
+template <int P, int TX, bool FULL, bool LEAN>
 __device__ __forceinline__ void bwd_src_tile(const BwdStepArgs& a, int tile, int e, int x0, int z0, float* s_c,
                                              float* s_u, TilePts& tp, signed char* map) {
   using T = Tile<TX>;
@@ -540,29 +629,38 @@ __device__ __forceinline__ void bwd_src_tile(const BwdStepArgs& a, int tile, int
   const float* __restrict__ g_cur = a.g_cur + fo;
   float* __restrict__ s_prev = a.s_prev + fo;
   float* __restrict__ grad = a.grad + (long long)e * a.gstride;
-  const int lz = threadIdx.x % kTZ, row = threadIdx.x / kTZ;
+  const int lz = threadIdx.x % kTZ, row = __builtin_amdgcn_readfirstlane(threadIdx.x / kTZ);
   const int iz = z0 + lz;
-  PtsLoad ls{0, -1, 0, 0, 1, 0.f};
-  if (a.s_sub.t >= 0) ls = load_points(a.s_sub.pts, &a.s_sub, tile, e, x0, z0, g.ld);
+  // loads issued as in the forward tile: no branch waits on one
+  const TileList ts = list_begin<LEAN>(a.s_sub.pts, &a.s_sub, a.s_sub.t >= 0, tile, e, x0, z0, g.ld);
   const bool band_tile = a.band != nullptr && touches_band<TX>(g, x0, z0);
-  float sp[T::PER], rr[T::PER], gr[T::PER], gc[T::PER], bv[T::PER];
+  // (values of points past the grid edge are loaded from element 0 and
+  // never used: the update loop skips those points; bv is used only in band)
+  float sp[T::PER], rr[T::PER], gr[T::PER] = {}, gc[T::PER] = {}, bv[T::PER] = {};
   unsigned inband = 0;
 #pragma unroll
   for (int k = 0; k < T::PER; k++) {
     const int ix = x0 + row + kRows * k;
     const bool ok = FULL || (ix < g.nx - kHalo && iz < g.nz - kHalo);
-    const unsigned q = (unsigned)(ix * g.ld + iz);
-    sp[k] = ok ? s_prev[q] : 0.f;
-    rr[k] = ok ? rv[q] : 1.f;
-    gr[k] = (ok && a.do_xcorr) ? grad[q] : 0.f;
-    gc[k] = (ok && a.do_xcorr) ? g_cur[q] : 0.f;
-    const int bi = (ok && band_tile) ? band_index(g, ix, iz) : -1;
-    bv[k] = bi >= 0 ? a.band[(long long)e * a.band_estride + bi] : 0.f;
-    if (bi >= 0) inband |= 1u << k;
+    const unsigned q = ok ? (unsigned)(ix * g.ld + iz) : 0u;
+    sp[k] = s_prev[q];
+    rr[k] = rv[q];
+    if (a.do_xcorr) {
+      gr[k] = grad[q];
+      gc[k] = g_cur[q];
+    }
+    if (band_tile) {
+      const int bi = ok ? band_index(g, ix, iz) : -1;
+      bv[k] = a.band[(long long)e * a.band_estride + (bi >= 0 ? bi : 0)];
+      if (bi >= 0) inband |= 1u << k;
+    }
   }
-  stage_tile<TX, FULL>(g, a.s_cur + fo, s_c, x0, z0);
+  Staged<TX> st;
+  stage_load<TX, FULL>(g, a.s_cur + fo, x0, z0, st);
+  const PtsLoad ls = list_end(ts, &a.s_sub, e);
+  stage_store<TX>(st, s_c);
   store_points(ls, tp);
-  const bool mapped = ls.n > 0 && ls.n <= kTilePts;
+  const bool mapped = ls.n > 0 && (LEAN || ls.n <= kTilePts);
   if (mapped) mark_clear<TX>(map);
   __syncthreads();
   if (mapped) {
@@ -582,7 +680,7 @@ __device__ __forceinline__ void bwd_src_tile(const BwdStepArgs& a, int tile, int
     if (mapped) {
       const int h = map[lx * kTZ + lz];
       if (h >= 0) sn = apply_mapped(tp, ls.n, h, a.s_sub.sign, q, sn);
-    } else if (ls.n > 0) {
+    } else if (!LEAN && ls.n > 0) {
       sn = apply_points(a.s_sub, tile, e, q, sn);
     }
     if (a.do_xcorr) grad[q] = gr[k] - sn * gc[k] * a.weight;
@@ -591,7 +689,7 @@ __device__ __forceinline__ void bwd_src_tile(const BwdStepArgs& a, int tile, int
   }
 }
 
-template <int P, int TX, bool FULL>
+template <int P, int TX, bool FULL, bool LEAN>
 __device__ __forceinline__ void bwd_rcv_tile(const BwdStepArgs& a, int tile, int e, int x0, int z0, float* s_c,
                                              float* s_u, TilePts& tp, signed char* map) {
   using T = Tile<TX>;
@@ -599,24 +697,30 @@ __device__ __forceinline__ void bwd_rcv_tile(const BwdStepArgs& a, int tile, int
   const long long fo = (long long)e * a.fstride;
   const float* __restrict__ rv = a.rv + (long long)(e / a.rvdiv) * a.rvstride;
   float* __restrict__ g_prev = a.g_prev + fo;
-  const int lz = threadIdx.x % kTZ, row = threadIdx.x / kTZ;
+  const int lz = threadIdx.x % kTZ, row = __builtin_amdgcn_readfirstlane(threadIdx.x / kTZ);
   const int iz = z0 + lz;
-  PtsLoad li{0, -1, 0, 0, 1, 0.f};
-  if (a.g_inj.t >= 0) li = load_points(a.g_inj.pts, &a.g_inj, tile, e, x0, z0, g.ld);
+  // loads issued as in the forward tile: no branch waits on one
+  const TileList tg = list_begin<LEAN>(a.g_inj.pts, &a.g_inj, a.g_inj.t >= 0, tile, e, x0, z0, g.ld);
   const bool sponge = touches_sponge(g, x0, x0 + TX, z0, z0 + kTZ);
-  float gp[T::PER], rr[T::PER], dl[T::PER];
+  // (values of points past the grid edge are loaded from element 0 and
+  // never used: the update loop skips those points)
+  float gp[T::PER], rr[T::PER], dx[T::PER] = {}, dz = 0.f;
+  if (sponge) dz = delta_z_load(g, iz);
 #pragma unroll
   for (int k = 0; k < T::PER; k++) {
     const int ix = x0 + row + kRows * k;
     const bool ok = FULL || (ix < g.nx - kHalo && iz < g.nz - kHalo);
-    const unsigned q = (unsigned)(ix * g.ld + iz);
-    gp[k] = ok ? g_prev[q] : 0.f;
-    rr[k] = ok ? rv[q] : 1.f;
-    dl[k] = (sponge && ok) ? damping_delta(g, ix, iz) : 0.f;
+    const unsigned q = ok ? (unsigned)(ix * g.ld + iz) : 0u;
+    gp[k] = g_prev[q];
+    rr[k] = rv[q];
+    if (sponge) dx[k] = delta_x_load(g, ix);
   }
-  stage_tile<TX, FULL>(g, a.g_cur + fo, s_c, x0, z0);
+  Staged<TX> st;
+  stage_load<TX, FULL>(g, a.g_cur + fo, x0, z0, st);
+  const PtsLoad li = list_end(tg, &a.g_inj, e);
+  stage_store<TX>(st, s_c);
   store_points(li, tp);
-  const bool mapped = li.n > 0 && li.n <= kTilePts;
+  const bool mapped = li.n > 0 && (LEAN || li.n <= kTilePts);
   if (mapped) mark_clear<TX>(map);
   __syncthreads();
   if (mapped) {
@@ -633,19 +737,19 @@ __device__ __forceinline__ void bwd_rcv_tile(const BwdStepArgs& a, int tile, int
     const unsigned q = (unsigned)(ix * g.ld + iz);
     const float c = s_c[(lx + kHalo) * T::LZ + lz + kHalo];
     const float* u = s_u + (lx + 1) * T::UZ + lz + 1;
-    float gn = sponge ? update_at<true, P, T::UZ>(c, gp[k], rr[k], u, dl[k])
+    float gn = sponge ? update_at<true, P, T::UZ>(c, gp[k], rr[k], u, delta_pick(g, iz, dz, dx[k]))
                       : update_at<false, P, T::UZ>(c, gp[k], rr[k], u, 0.f);
     if (mapped) {
       const int h = map[lx * kTZ + lz];
       if (h >= 0) gn = apply_mapped(tp, li.n, h, a.g_inj.sign, q, gn);
-    } else if (li.n > 0) {
+    } else if (!LEAN && li.n > 0) {
       gn = apply_points(a.g_inj, tile, e, q, gn);
     }
     g_prev[q] = gn;
   }
 }
 
-template <int P, int TX>
+template <int P, int TX, bool LEAN>
 __global__ __launch_bounds__(kThreads) void k_bwd_split(BwdStepArgs a, int nentries) {
   using T = Tile<TX>;
   __shared__ float s_c[T::LX * T::LZ];
@@ -659,18 +763,18 @@ __global__ __launch_bounds__(kThreads) void k_bwd_split(BwdStepArgs a, int nentr
   const bool full = tile_is_full<TX>(a.g, x0, z0);
   if (r & 1) {
     if (full)
-      bwd_rcv_tile<P, TX, true>(a, tile, e, x0, z0, s_c, s_u, tp, map);
+      bwd_rcv_tile<P, TX, true, LEAN>(a, tile, e, x0, z0, s_c, s_u, tp, map);
     else
-      bwd_rcv_tile<P, TX, false>(a, tile, e, x0, z0, s_c, s_u, tp, map);
+      bwd_rcv_tile<P, TX, false, LEAN>(a, tile, e, x0, z0, s_c, s_u, tp, map);
   } else {
     if (full)
-      bwd_src_tile<P, TX, true>(a, tile, e, x0, z0, s_c, s_u, tp, map);
+      bwd_src_tile<P, TX, true, LEAN>(a, tile, e, x0, z0, s_c, s_u, tp, map);
     else
-      bwd_src_tile<P, TX, false>(a, tile, e, x0, z0, s_c, s_u, tp, map);
+      bwd_src_tile<P, TX, false, LEAN>(a, tile, e, x0, z0, s_c, s_u, tp, map);
   }
 }
 
-template <int P, int TX>
+template <int P, int TX, bool LEAN>
 __global__ __launch_bounds__(kThreads) void k_bwd_step(BwdStepArgs a, int nentries) {
   using T = Tile<TX>;
   __shared__ float s_sc[T::LX * T::LZ];
@@ -684,9 +788,9 @@ __global__ __launch_bounds__(kThreads) void k_bwd_step(BwdStepArgs a, int nentri
   block_work(a.g.ntz * a.g.ntx, nentries, tile, e);
   const int z0 = kHalo + (tile % a.g.ntz) * kTZ, x0 = kHalo + (tile / a.g.ntz) * TX;
   if (tile_is_full<TX>(a.g, x0, z0))
-    bwd_tile<P, TX, true>(a, tile, e, x0, z0, s_sc, s_gc, s_su, s_gu, s_sub, s_inj, smap, gmap);
+    bwd_tile<P, TX, true, LEAN>(a, tile, e, x0, z0, s_sc, s_gc, s_su, s_gu, s_sub, s_inj, smap, gmap);
   else
-    bwd_tile<P, TX, false>(a, tile, e, x0, z0, s_sc, s_gc, s_su, s_gu, s_sub, s_inj, smap, gmap);
+    bwd_tile<P, TX, false, LEAN>(a, tile, e, x0, z0, s_sc, s_gc, s_su, s_gu, s_sub, s_inj, smap, gmap);
 }
 
 __global__ void k_fill(float* p, long long n, float v) {
@@ -726,37 +830,61 @@ inline int grid_for(long long n) {
   return (int)(b > 8192 ? 8192 : (b < 1 ? 1 : b));
 }
 
-template <int TX>
+template <int TX, bool LEAN>
 void fwd_dispatch(const FwdStepArgs& a, int E, bool damped, int precision, hipStream_t s) {
   const dim3 grid(a.g.ntz * a.g.ntx * E);
   if (damped) {
     if (precision == kExact)
-      hipLaunchKernelGGL((k_fwd_step<true, kExact, TX>), grid, dim3(kThreads), 0, s, a, E);
+      hipLaunchKernelGGL((k_fwd_step<true, kExact, TX, LEAN>), grid, dim3(kThreads), 0, s, a, E);
     else
-      hipLaunchKernelGGL((k_fwd_step<true, kFp32, TX>), grid, dim3(kThreads), 0, s, a, E);
+      hipLaunchKernelGGL((k_fwd_step<true, kFp32, TX, LEAN>), grid, dim3(kThreads), 0, s, a, E);
   } else {
     if (precision == kExact)
-      hipLaunchKernelGGL((k_fwd_step<false, kExact, TX>), grid, dim3(kThreads), 0, s, a, E);
+      hipLaunchKernelGGL((k_fwd_step<false, kExact, TX, LEAN>), grid, dim3(kThreads), 0, s, a, E);
     else
-      hipLaunchKernelGGL((k_fwd_step<false, kFp32, TX>), grid, dim3(kThreads), 0, s, a, E);
+      hipLaunchKernelGGL((k_fwd_step<false, kFp32, TX, LEAN>), grid, dim3(kThreads), 0, s, a, E);
   }
 }
 
-template <int TX>
+template <int TX, bool LEAN>
 void bwd_dispatch(const BwdStepArgs& a, int E, int precision, hipStream_t s) {
   if (a.split) {
     const dim3 grid(a.g.ntz * a.g.ntx * E * 2);
     if (precision == kExact)
-      hipLaunchKernelGGL((k_bwd_split<kExact, TX>), grid, dim3(kThreads), 0, s, a, E);
+      hipLaunchKernelGGL((k_bwd_split<kExact, TX, LEAN>), grid, dim3(kThreads), 0, s, a, E);
     else
-      hipLaunchKernelGGL((k_bwd_split<kFp32, TX>), grid, dim3(kThreads), 0, s, a, E);
+      hipLaunchKernelGGL((k_bwd_split<kFp32, TX, LEAN>), grid, dim3(kThreads), 0, s, a, E);
     return;
   }
   const dim3 grid(a.g.ntz * a.g.ntx * E);
   if (precision == kExact)
-    hipLaunchKernelGGL((k_bwd_step<kExact, TX>), grid, dim3(kThreads), 0, s, a, E);
+    hipLaunchKernelGGL((k_bwd_step<kExact, TX, LEAN>), grid, dim3(kThreads), 0, s, a, E);
   else
-    hipLaunchKernelGGL((k_bwd_step<kFp32, TX>), grid, dim3(kThreads), 0, s, a, E);
+    hipLaunchKernelGGL((k_bwd_step<kFp32, TX, LEAN>), grid, dim3(kThreads), 0, s, a, E);
+}
+
+// The lean instantiation (no CSR point-list code, no u2 export) serves every
+// launch whose active point lists all have slots.
+inline bool fwd_lean(const FwdStepArgs& a) {
+  return (a.inj.t < 0 || a.inj.pts.slots) && (a.rec.out == nullptr || a.rec.pts.slots) && a.u2out == nullptr;
+}
+inline bool bwd_lean(const BwdStepArgs& a) {
+  return (a.s_sub.t < 0 || a.s_sub.pts.slots) && (a.g_inj.t < 0 || a.g_inj.pts.slots);
+}
+
+template <int TX>
+void fwd_dispatch(const FwdStepArgs& a, int E, bool damped, int precision, hipStream_t s) {
+  if (fwd_lean(a))
+    fwd_dispatch<TX, true>(a, E, damped, precision, s);
+  else
+    fwd_dispatch<TX, false>(a, E, damped, precision, s);
+}
+template <int TX>
+void bwd_dispatch(const BwdStepArgs& a, int E, int precision, hipStream_t s) {
+  if (bwd_lean(a))
+    bwd_dispatch<TX, true>(a, E, precision, s);
+  else
+    bwd_dispatch<TX, false>(a, E, precision, s);
 }
 
 }  // namespace
