@@ -191,9 +191,10 @@ void Propagator::set_bwd_split(int mode) {
 }
 
 bool Propagator::bwd_split(int E) const {
-  if (bwd_split_ >= 0) return bwd_split_ == 1;
-  const long long tiles16 = (long long)g_.ntz * tile_count_x(g_.nxpad, 16) * E;
-  return tiles16 < kBwdSplitBelow;
+  // measured on the 301x921 grid: the split form is the faster one at every
+  // batch size from 1 to 100 entries once each form has its best tile width
+  (void)E;
+  return bwd_split_ != 0;
 }
 
 void Propagator::set_tile_x(int tx) {
@@ -201,9 +202,13 @@ void Propagator::set_tile_x(int tx) {
   tile_override_ = tx;
 }
 
+long long Propagator::tile_key(int kind, int E) const {
+  return ((long long)kind << 40) + ((long long)(kind == 1 && bwd_split(E)) << 32) + E;
+}
+
 int Propagator::tile_choice_cached(int kind, int E) const {
   if (tile_override_) return tile_override_;
-  const long long key = ((long long)kind << 32) + E;
+  const long long key = tile_key(kind, E);
   for (const auto& kv : tile_cache_)
     if (kv.first == key) return kTileWidths[kv.second];
   return 0;
@@ -271,21 +276,21 @@ int Propagator::tile_choice(int kind, int E, const float* rv, long long rvstride
     for (int i = 0; i < kNumWidths; i++)
       if (kTileWidths[i] == tile_override_) return i;
   }
-  const long long key = ((long long)kind << 32) + E;
+  const long long key = tile_key(kind, E);
   for (const auto& kv : tile_cache_)
     if (kv.first == key) return kv.second;
   int best = 1;
   if (!tune_ || !may_time) {
-    // measured on the 301x921 grid: forward, the narrow tile wins while the
-    // batch does not fill the chip (more, shorter blocks), the wide one once
-    // it is bandwidth-bound (less halo traffic per output); backward, 64x16
-    // at every batch size (split into source and receiver blocks while the
-    // batch is small, fused beyond)
+    // measured on the 301x921 grid (372 tiles of 64x16 per entry): the
+    // narrow tile wins while the batch does not fill the chip (more, shorter
+    // blocks), the wide one once it is bandwidth-bound (less halo traffic per
+    // output).  Forward: 64x8 at one entry, 64x16 up to 6, 64x32 beyond;
+    // backward split: 64x16 up to 4 entries, 64x32 beyond; fused: 64x16.
     const long long tiles16 = (long long)g_.ntz * tile_count_x(g_.nxpad, 16) * E;
     if (kind == 1)
-      best = 1;
+      best = bwd_split(E) && tiles16 >= 1500 ? 2 : 1;
     else
-      best = tiles16 < 600 ? 0 : (tiles16 < 2000 ? 1 : 2);
+      best = tiles16 < 600 ? 0 : (tiles16 < 2500 ? 1 : 2);
   } else {
     // timed trial on scratch wavefields (the caller's workspace is free at
     // this point: forward() tunes before it zeroes and runs); three

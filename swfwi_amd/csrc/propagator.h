@@ -138,6 +138,7 @@ class Propagator {
   void set_bwd_split(int mode);
   bool bwd_split(int E) const;
   int tile_choice_cached(int kind, int E) const;
+  long long tile_key(int kind, int E) const;
   // Microseconds per launch of `reps` back-to-back step kernels (kind 0
   // forward, 1 backward as backward() launches it) of tile width kTileWidths[width] on E scratch
   // wavefields of model rv (entry e uses rv + (e / rvdiv) * rvstride).
@@ -175,7 +176,6 @@ class Propagator {
   // trial of every width on scratch wavefields (the tile tuner).
   int tile_choice(int kind, int E, const float* rv, long long rvstride, int rvdiv, Fields& w, bool may_time);
   static constexpr int kGraphMaxEntries = 8;
-  static constexpr long long kBwdSplitBelow = 10000;  // 64x16 tiles over the batch
   // Run a time loop (`launches` step launches of kind 0 forward / 1
   // backward): captured into a HIP graph on first use and replayed when the
   // same buffers come back, or launched directly.
