@@ -117,8 +117,9 @@ int swfwi_solver_set_tile_x(swfwi_solver h, int tile_x);
  * size, 0 when that batch size has not run yet. */
 int swfwi_solver_get_tile_x(swfwi_solver h, int kind, int entries, int *tile_x);
 /* Backward iterations as one fused block per tile (0), as separate source- and
- * receiver-field blocks (1), or by the measured rule (-1, the default: split,
- * the faster form at every batch size measured).  Same results. */
+ * receiver-field blocks (1), or by the measured rule (-1, the default: split
+ * while the batch's wavefields fit the last-level cache, fused beyond).
+ * Same results. */
 int swfwi_solver_set_bwd_split(swfwi_solver h, int mode);
 /* Record a HIP event pair around every forward / backward time loop on the
  * handle's stream (graph replays included); swfwi_solver_kernel_times waits

@@ -191,10 +191,13 @@ void Propagator::set_bwd_split(int mode) {
 }
 
 bool Propagator::bwd_split(int E) const {
-  // measured on the 301x921 grid: the split form is the faster one at every
-  // batch size from 1 to 100 entries once each form has its best tile width
-  (void)E;
-  return bwd_split_ != 0;
+  if (bwd_split_ >= 0) return bwd_split_ == 1;
+  // measured on the 301x921 grid with the imaging condition: split (64x16
+  // then 64x32 tiles) up to 30 entries, whose wavefields stay in the 256 MB
+  // last-level cache; fused 64x16 from 45 entries on, where the split form's
+  // second read of the receiver field goes to HBM
+  const long long tiles16 = (long long)g_.ntz * tile_count_x(g_.nxpad, 16) * E;
+  return tiles16 < kBwdSplitBelow;
 }
 
 void Propagator::set_tile_x(int tx) {
@@ -217,10 +220,17 @@ int Propagator::tile_choice_cached(int kind, int E) const {
 float Propagator::time_steps(int kind, int E, const float* rv, long long rvstride, int rvdiv, Fields& w, int width,
                              int reps, bool clear) {
   // `reps` back-to-back launches of one step kernel on scratch wavefields
-  // (values discarded), timed with one event pair; microseconds per launch
+  // (values discarded), timed with one event pair; microseconds per launch.
+  // The backward iteration includes the imaging condition (into a scratch
+  // gradient), which decides between its fused and split forms.
   hipStream_t s = stream();
   const long long np = g_.npts();
   w.ensure(E, np, true);
+  DBuf<float> gscratch;
+  if (kind == 1) {
+    gscratch.resize((size_t)E * (size_t)np);
+    launch_fill(gscratch.get(), (long long)E * np, 0.f, s);
+  }
   if (clear) {
     launch_fill(w.a.get(), (long long)E * np, 0.f, s);
     launch_fill(w.b.get(), (long long)E * np, 0.f, s);
@@ -253,9 +263,10 @@ float Propagator::time_steps(int kind, int E, const float* rv, long long rvstrid
       a.rv = rv;
       a.rvstride = rvstride;
       a.rvdiv = rvdiv;
-      a.grad = w.c.get();  // scratch only
+      a.grad = gscratch.get();
       a.gstride = np;
-      a.do_xcorr = 0;
+      a.weight = 1.f;
+      a.do_xcorr = 1;
       a.split = bwd_split(E);
       launch_bwd_step(a, E, prec_, s);
     }

@@ -176,6 +176,7 @@ class Propagator {
   // trial of every width on scratch wavefields (the tile tuner).
   int tile_choice(int kind, int E, const float* rv, long long rvstride, int rvdiv, Fields& w, bool may_time);
   static constexpr int kGraphMaxEntries = 8;
+  static constexpr long long kBwdSplitBelow = 13000;  // 64x16 tiles over the batch
   // Run a time loop (`launches` step launches of kind 0 forward / 1
   // backward): captured into a HIP graph on first use and replayed when the
   // same buffers come back, or launched directly.

@@ -38,7 +38,9 @@ def run_shots(nt):
 
 def run_sweep(entries, reps, precision):
     """Per-launch time of every step kernel width over batch sizes on the
-    301x921 grid (back-to-back launches on scratch wavefields)."""
+    301x921 grid (back-to-back launches on scratch wavefields; the backward
+    iteration includes the imaging condition into a scratch gradient; no
+    sources, receivers or band)."""
     case = workloads.ess_301x921(nt=8)
     vtrue, _ = workloads.models(case)
     s = capi.Solver(case.nx, case.nz, case.nb, case.dt, case.dx, case.fm, case.nt, case.sx, case.sz, case.gx,
