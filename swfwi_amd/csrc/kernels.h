@@ -104,6 +104,7 @@ struct BwdStepArgs {
   Inject s_sub;                   // subSource into the new source field (t = it)
   Inject g_inj;                   // receiver injection for the next iteration (t = it-1)
   const float* band = nullptr;    // boundary band restored into the new source field (step it-1)
+  const float* band_next = nullptr;  // the band the next launch restores (read ahead into the caches)
   long long band_estride = 0;
   float* grad = nullptr;          // imaging condition accumulator (+ e*gstride)
   long long gstride = 0;

@@ -615,6 +615,7 @@ void Propagator::backward_body(int E, const float* rv, long long rvstride, const
     const bool more = it - 1 >= it_min;
     a.g_inj.t = more ? it - 1 : -1;
     a.band = more ? band + (long long)(it - 1) * g_.bsize : nullptr;
+    a.band_next = it - 2 >= it_min ? band + (long long)(it - 2) * g_.bsize : nullptr;
     a.weight = wts[it];
     launch_bwd_step(a, E, prec_, s);
     std::swap(s_cur, s_prv);

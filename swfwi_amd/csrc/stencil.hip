@@ -534,7 +534,7 @@ __device__ __forceinline__ void bwd_tile(const BwdStepArgs& a, int tile, int e, 
   const bool band_tile = a.band != nullptr && touches_band<TX>(g, x0, z0);
   // (values of points past the grid edge are loaded from element 0 and
   // never used: the update loop skips those points; bv is used only in band)
-  float sp[T::PER], gp[T::PER], rr[T::PER], gr[T::PER] = {}, dx[T::PER] = {}, bv[T::PER] = {}, dz = 0.f;
+  float sp[T::PER], gp[T::PER], rr[T::PER], gr[T::PER] = {}, dx[T::PER] = {}, bv[T::PER] = {}, dz = 0.f, pf[T::PER] = {};
   unsigned inband = 0;
   if (sponge) dz = delta_z_load(g, iz);
 #pragma unroll
@@ -550,6 +550,9 @@ __device__ __forceinline__ void bwd_tile(const BwdStepArgs& a, int tile, int e, 
     if (band_tile) {
       const int bi = ok ? band_index(g, ix, iz) : -1;
       bv[k] = a.band[(long long)e * a.band_estride + (bi >= 0 ? bi : 0)];
+      // the next launch's band, read ahead so that it comes from the
+      // memory-side cache rather than HBM (the values are not used)
+      if (a.band_next) pf[k] = a.band_next[(long long)e * a.band_estride + (bi >= 0 ? bi : 0)];
       if (bi >= 0) inband |= 1u << k;
     }
   }
@@ -612,6 +615,10 @@ __device__ __forceinline__ void bwd_tile(const BwdStepArgs& a, int tile, int e, 
     s_prev[q] = sn;
     g_prev[q] = gn;
   }
+  if (band_tile && a.band_next) {  // keep the read-ahead loads (nothing reads their values)
+#pragma unroll
+    for (int k = 0; k < T::PER; k++) asm volatile("" ::"v"(pf[k]));
+  }
 }
 
 // The same backward iteration as two independent halves of one launch, for
@@ -636,7 +643,7 @@ __device__ __forceinline__ void bwd_src_tile(const BwdStepArgs& a, int tile, int
   const bool band_tile = a.band != nullptr && touches_band<TX>(g, x0, z0);
   // (values of points past the grid edge are loaded from element 0 and
   // never used: the update loop skips those points; bv is used only in band)
-  float sp[T::PER], rr[T::PER], gr[T::PER] = {}, gc[T::PER] = {}, bv[T::PER] = {};
+  float sp[T::PER], rr[T::PER], gr[T::PER] = {}, gc[T::PER] = {}, bv[T::PER] = {}, pf[T::PER] = {};
   unsigned inband = 0;
 #pragma unroll
   for (int k = 0; k < T::PER; k++) {
@@ -652,6 +659,9 @@ __device__ __forceinline__ void bwd_src_tile(const BwdStepArgs& a, int tile, int
     if (band_tile) {
       const int bi = ok ? band_index(g, ix, iz) : -1;
       bv[k] = a.band[(long long)e * a.band_estride + (bi >= 0 ? bi : 0)];
+      // the next launch's band, read ahead so that it comes from the
+      // memory-side cache rather than HBM (the values are not used)
+      if (a.band_next) pf[k] = a.band_next[(long long)e * a.band_estride + (bi >= 0 ? bi : 0)];
       if (bi >= 0) inband |= 1u << k;
     }
   }
@@ -686,6 +696,10 @@ __device__ __forceinline__ void bwd_src_tile(const BwdStepArgs& a, int tile, int
     if (a.do_xcorr) grad[q] = gr[k] - sn * gc[k] * a.weight;
     if (inband & (1u << k)) sn = bv[k];
     s_prev[q] = sn;
+  }
+  if (band_tile && a.band_next) {  // keep the read-ahead loads (nothing reads their values)
+#pragma unroll
+    for (int k = 0; k < T::PER; k++) asm volatile("" ::"v"(pf[k]));
   }
 }
 
