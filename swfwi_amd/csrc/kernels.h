@@ -48,6 +48,7 @@ struct Grid {
   int nx = 0, nz = 0, ld = 0;     // padded dims and leading dimension
   int ntz = 0, ntx = 0;           // tiles along z and x
   int tile_x = 16;                // tile width along x: 8, 16 or 32
+  int tile_major = 0;             // block order: 0 entries fastest (shared model), 1 tiles fastest
   int nb = 0;                     // damping width handed to the damped kernel (bx0)
   int bx0 = 0, bxn = 0, bz0 = 0, bzn = 0;
   const float* delta_x = nullptr; // damping delta per ix (x sponge), 0 inside

@@ -381,6 +381,10 @@ void Propagator::forward_body(int E, const float* rv, long long rvstride, const 
 
   FwdStepArgs a;
   a.g = grid(w);
+  // one model per entry: tiles fastest, unless the batch is too small to be
+  // bandwidth-bound (measured: 2 entries 0.4% slower tile-major, 100 entries
+  // 7% faster)
+  a.g.tile_major = rvstride != 0 && rvdiv == 1 && E >= 4;
   a.fstride = np;
   a.rv = rv;
   a.rvstride = rvstride;
@@ -589,6 +593,10 @@ void Propagator::backward_body(int E, const float* rv, long long rvstride, const
 
   BwdStepArgs a;
   a.g = grid(w);
+  // one model per entry: tiles fastest, unless the batch is too small to be
+  // bandwidth-bound (measured: 2 entries 0.4% slower tile-major, 100 entries
+  // 7% faster)
+  a.g.tile_major = rvstride != 0 && rvdiv == 1 && E >= 4;
   a.fstride = np;
   a.rv = rv;
   a.rvstride = rvstride;

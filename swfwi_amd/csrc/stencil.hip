@@ -154,15 +154,24 @@ __device__ __forceinline__ bool touches_sponge(const Grid& g, int x0, int x1, in
   return x0 < g.nb || x1 > g.nx - g.nb || z1 > g.zdamp;
 }
 
-// Block -> (tile, entry): XCD-contiguous runs, entries fastest.
-__device__ __forceinline__ void block_work(int ntiles, int nentries, int& tile, int& e) {
+// Block -> (tile, entry): XCD-contiguous runs.  Entries fastest when the
+// entries share a model (the tile's model is re-read from the XCD's L2 by
+// every entry); tiles fastest when each entry has its own model (a tile's
+// x-halo, the neighbouring tiles' columns of the same entry, is then re-read
+// from L2 instead of HBM).
+__device__ __forceinline__ void block_work(int ntiles, int nentries, int tile_major, int& tile, int& e) {
   const int n = ntiles * nentries;
   const int b = blockIdx.x;
   const int per = n / 8, rem = n % 8;
   const int xcd = b % 8, local = b / 8;
   const int l = xcd * per + (xcd < rem ? xcd : rem) + local;
-  tile = l / nentries;
-  e = l - tile * nentries;
+  if (tile_major) {
+    e = l / ntiles;
+    tile = l - e * ntiles;
+  } else {
+    tile = l / nentries;
+    e = l - tile * nentries;
+  }
 }
 
 // Stage cur (tile + halo) into LDS, zero outside the grid, in two halves so
@@ -504,7 +513,7 @@ __global__ __launch_bounds__(kThreads) void k_fwd_step(FwdStepArgs a, int nentri
   __shared__ TilePts s_inj, s_rec;
   __shared__ __align__(4) signed char imap[kTZ * TX];
   int tile, e;
-  block_work(a.g.ntz * a.g.ntx, nentries, tile, e);
+  block_work(a.g.ntz * a.g.ntx, nentries, a.g.tile_major, tile, e);
   const int z0 = kHalo + (tile % a.g.ntz) * kTZ, x0 = kHalo + (tile / a.g.ntz) * TX;
   if (tile_is_full<TX>(a.g, x0, z0))
     fwd_tile<DAMPED, P, TX, true, LEAN>(a, tile, e, x0, z0, s_cur, s_u2, s_inj, s_rec, imap);
@@ -771,7 +780,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd_split(BwdStepArgs a, int nentr
   __shared__ TilePts tp;
   __shared__ __align__(4) signed char map[kTZ * TX];
   int tile, r;
-  block_work(a.g.ntz * a.g.ntx, 2 * nentries, tile, r);  // both halves of an entry adjacent
+  block_work(a.g.ntz * a.g.ntx, 2 * nentries, a.g.tile_major, tile, r);  // r: entry x 2 halves
   const int e = r >> 1;
   const int z0 = kHalo + (tile % a.g.ntz) * kTZ, x0 = kHalo + (tile / a.g.ntz) * TX;
   const bool full = tile_is_full<TX>(a.g, x0, z0);
@@ -799,7 +808,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd_step(BwdStepArgs a, int nentri
   __shared__ __align__(4) signed char smap[kTZ * TX];
   __shared__ __align__(4) signed char gmap[kTZ * TX];
   int tile, e;
-  block_work(a.g.ntz * a.g.ntx, nentries, tile, e);
+  block_work(a.g.ntz * a.g.ntx, nentries, a.g.tile_major, tile, e);
   const int z0 = kHalo + (tile % a.g.ntz) * kTZ, x0 = kHalo + (tile / a.g.ntz) * TX;
   if (tile_is_full<TX>(a.g, x0, z0))
     bwd_tile<P, TX, true, LEAN>(a, tile, e, x0, z0, s_sc, s_gc, s_su, s_gu, s_sub, s_inj, smap, gmap);
