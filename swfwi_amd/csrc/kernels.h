@@ -48,7 +48,6 @@ struct Grid {
   int nx = 0, nz = 0, ld = 0;     // padded dims and leading dimension
   int ntz = 0, ntx = 0;           // tiles along z and x
   int tile_x = 16;                // tile width along x: 8, 16 or 32
-  int tile_major = 0;             // block order: 0 entries fastest (shared model), 1 tiles fastest
   int nb = 0;                     // damping width handed to the damped kernel (bx0)
   int bx0 = 0, bxn = 0, bz0 = 0, bzn = 0;
   const float* delta_x = nullptr; // damping delta per ix (x sponge), 0 inside
@@ -120,8 +119,10 @@ struct BwdStepArgs {
 constexpr int kNumWidths = 3;
 constexpr int kTileWidths[kNumWidths] = {8, 16, 32};
 int tile_count_x(int nx, int tile_x);
-void launch_fwd_step(const FwdStepArgs& a, int nentries, bool damped, int precision, hipStream_t stream);
-void launch_bwd_step(const BwdStepArgs& a, int nentries, int precision, hipStream_t stream);
+// chunk: block order, 0 = entries fastest, else runs of `chunk` tiles per entry
+void launch_fwd_step(const FwdStepArgs& a, int nentries, bool damped, int precision, hipStream_t stream,
+                     int chunk = 0);
+void launch_bwd_step(const BwdStepArgs& a, int nentries, int precision, hipStream_t stream, int chunk = 0);
 
 void launch_fill(float* p, long long n, float v, hipStream_t s);
 void launch_fill_d(double* p, long long n, double v, hipStream_t s);

@@ -166,6 +166,20 @@ Propagator::Propagator(const Geometry& g, int precision) : g_(g), prec_(precisio
   stream_.sync();
 }
 
+int Propagator::block_chunk(int kind, int E, long long rvstride, int rvdiv) const {
+  // Measured on the 301x921 grid: below 4 entries the order does not matter
+  // (latency-bound).  A forward step whose entries share a model is fastest
+  // entries first (the model, a quarter of its traffic, read once: 60
+  // entries 65 against 70 us chunked); with one model per entry it is
+  // fastest in chunks of 4 tile columns (ensemble epoch of 100 members 3.05
+  // -> 2.82 s).  The fused backward iteration (four per-entry fields) runs
+  // chunked (100 entries 286 -> 262 us), the split one entries first (30
+  // entries 73 against 79 us).
+  if (E < 4) return 0;
+  const bool chunked = kind == 0 ? (rvstride != 0 && rvdiv == 1) : !bwd_split(E);
+  return chunked ? 4 * g_.ntz : 0;
+}
+
 Grid Propagator::grid(int w) const {
   Grid gr;
   gr.nx = g_.nxpad;
@@ -193,9 +207,9 @@ void Propagator::set_bwd_split(int mode) {
 bool Propagator::bwd_split(int E) const {
   if (bwd_split_ >= 0) return bwd_split_ == 1;
   // measured on the 301x921 grid with the imaging condition: split (64x16
-  // then 64x32 tiles) up to 30 entries, whose wavefields stay in the 256 MB
-  // last-level cache; fused 64x16 from 45 entries on, where the split form's
-  // second read of the receiver field goes to HBM
+  // then 64x32 tiles, entries fastest) up to 16 entries; fused (64x16, in
+  // chunks of tile columns) from 30 entries on, where the split form's second
+  // read of the receiver field costs more than the fused form's occupancy
   const long long tiles16 = (long long)g_.ntz * tile_count_x(g_.nxpad, 16) * E;
   return tiles16 < kBwdSplitBelow;
 }
@@ -251,7 +265,7 @@ float Propagator::time_steps(int kind, int E, const float* rv, long long rvstrid
       a.rv = rv;
       a.rvstride = rvstride;
       a.rvdiv = rvdiv;
-      launch_fwd_step(a, E, true, prec_, s);
+      launch_fwd_step(a, E, true, prec_, s, block_chunk(0, E, rvstride, rvdiv));
     } else {
       BwdStepArgs a;
       a.g = grid(width);
@@ -268,7 +282,7 @@ float Propagator::time_steps(int kind, int E, const float* rv, long long rvstrid
       a.weight = 1.f;
       a.do_xcorr = 1;
       a.split = bwd_split(E);
-      launch_bwd_step(a, E, prec_, s);
+      launch_bwd_step(a, E, prec_, s, block_chunk(1, E, rvstride, rvdiv));
     }
   }
   SWFWI_HIP(hipEventRecord(e1, s));
@@ -381,10 +395,6 @@ void Propagator::forward_body(int E, const float* rv, long long rvstride, const 
 
   FwdStepArgs a;
   a.g = grid(w);
-  // one model per entry: tiles fastest, unless the batch is too small to be
-  // bandwidth-bound (measured: 2 entries 0.4% slower tile-major, 100 entries
-  // 7% faster)
-  a.g.tile_major = rvstride != 0 && rvdiv == 1 && E >= 4;
   a.fstride = np;
   a.rv = rv;
   a.rvstride = rvstride;
@@ -400,7 +410,7 @@ void Propagator::forward_body(int E, const float* rv, long long rvstride, const 
     a.inj.t = it + 1 < nt ? it + 1 : -1;
     a.rec.out = dcal ? dcal + (long long)it * ng : nullptr;
     a.band = band ? band + (long long)it * g_.bsize : nullptr;
-    launch_fwd_step(a, E, true, prec_, s);
+    launch_fwd_step(a, E, true, prec_, s, block_chunk(0, E, rvstride, rvdiv));
   }
   SWFWI_HIP(hipGetLastError());
 }
@@ -593,10 +603,6 @@ void Propagator::backward_body(int E, const float* rv, long long rvstride, const
 
   BwdStepArgs a;
   a.g = grid(w);
-  // one model per entry: tiles fastest, unless the batch is too small to be
-  // bandwidth-bound (measured: 2 entries 0.4% slower tile-major, 100 entries
-  // 7% faster)
-  a.g.tile_major = rvstride != 0 && rvdiv == 1 && E >= 4;
   a.fstride = np;
   a.rv = rv;
   a.rvstride = rvstride;
@@ -625,7 +631,7 @@ void Propagator::backward_body(int E, const float* rv, long long rvstride, const
     a.band = more ? band + (long long)(it - 1) * g_.bsize : nullptr;
     a.band_next = it - 2 >= it_min ? band + (long long)(it - 2) * g_.bsize : nullptr;
     a.weight = wts[it];
-    launch_bwd_step(a, E, prec_, s);
+    launch_bwd_step(a, E, prec_, s, block_chunk(1, E, rvstride, rvdiv));
     std::swap(s_cur, s_prv);
     std::swap(g_cur, g_prv);
   }

@@ -139,6 +139,9 @@ class Propagator {
   bool bwd_split(int E) const;
   int tile_choice_cached(int kind, int E) const;
   long long tile_key(int kind, int E) const;
+  // block order of a step launch (kind 0 forward, 1 backward): 0 = entries
+  // fastest, else runs of that many tiles per entry
+  int block_chunk(int kind, int E, long long rvstride, int rvdiv) const;
   // Microseconds per launch of `reps` back-to-back step kernels (kind 0
   // forward, 1 backward as backward() launches it) of tile width kTileWidths[width] on E scratch
   // wavefields of model rv (entry e uses rv + (e / rvdiv) * rvstride).
@@ -176,7 +179,7 @@ class Propagator {
   // trial of every width on scratch wavefields (the tile tuner).
   int tile_choice(int kind, int E, const float* rv, long long rvstride, int rvdiv, Fields& w, bool may_time);
   static constexpr int kGraphMaxEntries = 8;
-  static constexpr long long kBwdSplitBelow = 13000;  // 64x16 tiles over the batch
+  static constexpr long long kBwdSplitBelow = 6000;  // 64x16 tiles over the batch
   // Run a time loop (`launches` step launches of kind 0 forward / 1
   // backward): captured into a HIP graph on first use and replayed when the
   // same buffers come back, or launched directly.

@@ -154,20 +154,25 @@ __device__ __forceinline__ bool touches_sponge(const Grid& g, int x0, int x1, in
   return x0 < g.nb || x1 > g.nx - g.nb || z1 > g.zdamp;
 }
 
-// Block -> (tile, entry): XCD-contiguous runs.  Entries fastest when the
-// entries share a model (the tile's model is re-read from the XCD's L2 by
-// every entry); tiles fastest when each entry has its own model (a tile's
-// x-halo, the neighbouring tiles' columns of the same entry, is then re-read
-// from L2 instead of HBM).
-__device__ __forceinline__ void block_work(int ntiles, int nentries, int tile_major, int& tile, int& e) {
+// Block -> (tile, entry): XCD-contiguous runs of the work list.  chunk = 0:
+// entries fastest (small batches).  Otherwise the list walks chunks of
+// `chunk` consecutive tiles (a few tile columns), each chunk entry by entry:
+// a tile's x-halo (the neighbouring columns of the same entry) and, when
+// the entries share a model, the chunk's model are re-read from the XCD's L2
+// instead of HBM.
+template <bool CHUNKED>
+__device__ __forceinline__ void block_work(int ntiles, int nentries, int chunk, int& tile, int& e) {
   const int n = ntiles * nentries;
   const int b = blockIdx.x;
   const int per = n / 8, rem = n % 8;
   const int xcd = b % 8, local = b / 8;
   const int l = xcd * per + (xcd < rem ? xcd : rem) + local;
-  if (tile_major) {
-    e = l / ntiles;
-    tile = l - e * ntiles;
+  if constexpr (CHUNKED) {
+    const int c = l / (chunk * nentries);
+    const int r = l - c * chunk * nentries;
+    const int tc = min(chunk, ntiles - c * chunk);  // the last chunk may be short
+    e = r / tc;
+    tile = c * chunk + (r - e * tc);
   } else {
     tile = l / nentries;
     e = l - tile * nentries;
@@ -505,15 +510,15 @@ __device__ __forceinline__ void fwd_tile(const FwdStepArgs& a, int tile, int e, 
   }
 }
 
-template <bool DAMPED, int P, int TX, bool LEAN>
-__global__ __launch_bounds__(kThreads) void k_fwd_step(FwdStepArgs a, int nentries) {
+template <bool DAMPED, int P, int TX, bool LEAN, bool CHUNKED>
+__global__ __launch_bounds__(kThreads) void k_fwd_step(FwdStepArgs a, int nentries, int chunk) {
   using T = Tile<TX>;
   __shared__ float s_cur[T::LX * T::LZ];
   __shared__ float s_u2[T::UX * T::UZ];
   __shared__ TilePts s_inj, s_rec;
   __shared__ __align__(4) signed char imap[kTZ * TX];
   int tile, e;
-  block_work(a.g.ntz * a.g.ntx, nentries, a.g.tile_major, tile, e);
+  block_work<CHUNKED>(a.g.ntz * a.g.ntx, nentries, chunk, tile, e);
   const int z0 = kHalo + (tile % a.g.ntz) * kTZ, x0 = kHalo + (tile / a.g.ntz) * TX;
   if (tile_is_full<TX>(a.g, x0, z0))
     fwd_tile<DAMPED, P, TX, true, LEAN>(a, tile, e, x0, z0, s_cur, s_u2, s_inj, s_rec, imap);
@@ -772,15 +777,15 @@ __device__ __forceinline__ void bwd_rcv_tile(const BwdStepArgs& a, int tile, int
   }
 }
 
-template <int P, int TX, bool LEAN>
-__global__ __launch_bounds__(kThreads) void k_bwd_split(BwdStepArgs a, int nentries) {
+template <int P, int TX, bool LEAN, bool CHUNKED>
+__global__ __launch_bounds__(kThreads) void k_bwd_split(BwdStepArgs a, int nentries, int chunk) {
   using T = Tile<TX>;
   __shared__ float s_c[T::LX * T::LZ];
   __shared__ float s_u[T::UX * T::UZ];
   __shared__ TilePts tp;
   __shared__ __align__(4) signed char map[kTZ * TX];
   int tile, r;
-  block_work(a.g.ntz * a.g.ntx, 2 * nentries, a.g.tile_major, tile, r);  // r: entry x 2 halves
+  block_work<CHUNKED>(a.g.ntz * a.g.ntx, 2 * nentries, chunk, tile, r);  // r: entry x 2 halves
   const int e = r >> 1;
   const int z0 = kHalo + (tile % a.g.ntz) * kTZ, x0 = kHalo + (tile / a.g.ntz) * TX;
   const bool full = tile_is_full<TX>(a.g, x0, z0);
@@ -797,8 +802,8 @@ __global__ __launch_bounds__(kThreads) void k_bwd_split(BwdStepArgs a, int nentr
   }
 }
 
-template <int P, int TX, bool LEAN>
-__global__ __launch_bounds__(kThreads) void k_bwd_step(BwdStepArgs a, int nentries) {
+template <int P, int TX, bool LEAN, bool CHUNKED>
+__global__ __launch_bounds__(kThreads) void k_bwd_step(BwdStepArgs a, int nentries, int chunk) {
   using T = Tile<TX>;
   __shared__ float s_sc[T::LX * T::LZ];
   __shared__ float s_gc[T::LX * T::LZ];
@@ -808,7 +813,7 @@ __global__ __launch_bounds__(kThreads) void k_bwd_step(BwdStepArgs a, int nentri
   __shared__ __align__(4) signed char smap[kTZ * TX];
   __shared__ __align__(4) signed char gmap[kTZ * TX];
   int tile, e;
-  block_work(a.g.ntz * a.g.ntx, nentries, a.g.tile_major, tile, e);
+  block_work<CHUNKED>(a.g.ntz * a.g.ntx, nentries, chunk, tile, e);
   const int z0 = kHalo + (tile % a.g.ntz) * kTZ, x0 = kHalo + (tile / a.g.ntz) * TX;
   if (tile_is_full<TX>(a.g, x0, z0))
     bwd_tile<P, TX, true, LEAN>(a, tile, e, x0, z0, s_sc, s_gc, s_su, s_gu, s_sub, s_inj, smap, gmap);
@@ -853,41 +858,43 @@ inline int grid_for(long long n) {
   return (int)(b > 8192 ? 8192 : (b < 1 ? 1 : b));
 }
 
-template <int TX, bool LEAN>
-void fwd_dispatch(const FwdStepArgs& a, int E, bool damped, int precision, hipStream_t s) {
+template <int TX, bool LEAN, bool CHUNKED>
+void fwd_dispatch(const FwdStepArgs& a, int E, bool damped, int precision, hipStream_t s, int chunk) {
   const dim3 grid(a.g.ntz * a.g.ntx * E);
   if (damped) {
     if (precision == kExact)
-      hipLaunchKernelGGL((k_fwd_step<true, kExact, TX, LEAN>), grid, dim3(kThreads), 0, s, a, E);
+      hipLaunchKernelGGL((k_fwd_step<true, kExact, TX, LEAN, CHUNKED>), grid, dim3(kThreads), 0, s, a, E, chunk);
     else
-      hipLaunchKernelGGL((k_fwd_step<true, kFp32, TX, LEAN>), grid, dim3(kThreads), 0, s, a, E);
+      hipLaunchKernelGGL((k_fwd_step<true, kFp32, TX, LEAN, CHUNKED>), grid, dim3(kThreads), 0, s, a, E, chunk);
   } else {
     if (precision == kExact)
-      hipLaunchKernelGGL((k_fwd_step<false, kExact, TX, LEAN>), grid, dim3(kThreads), 0, s, a, E);
+      hipLaunchKernelGGL((k_fwd_step<false, kExact, TX, LEAN, CHUNKED>), grid, dim3(kThreads), 0, s, a, E, chunk);
     else
-      hipLaunchKernelGGL((k_fwd_step<false, kFp32, TX, LEAN>), grid, dim3(kThreads), 0, s, a, E);
+      hipLaunchKernelGGL((k_fwd_step<false, kFp32, TX, LEAN, CHUNKED>), grid, dim3(kThreads), 0, s, a, E, chunk);
   }
 }
 
-template <int TX, bool LEAN>
-void bwd_dispatch(const BwdStepArgs& a, int E, int precision, hipStream_t s) {
+template <int TX, bool LEAN, bool CHUNKED>
+void bwd_dispatch(const BwdStepArgs& a, int E, int precision, hipStream_t s, int chunk) {
   if (a.split) {
     const dim3 grid(a.g.ntz * a.g.ntx * E * 2);
     if (precision == kExact)
-      hipLaunchKernelGGL((k_bwd_split<kExact, TX, LEAN>), grid, dim3(kThreads), 0, s, a, E);
+      hipLaunchKernelGGL((k_bwd_split<kExact, TX, LEAN, CHUNKED>), grid, dim3(kThreads), 0, s, a, E, chunk);
     else
-      hipLaunchKernelGGL((k_bwd_split<kFp32, TX, LEAN>), grid, dim3(kThreads), 0, s, a, E);
+      hipLaunchKernelGGL((k_bwd_split<kFp32, TX, LEAN, CHUNKED>), grid, dim3(kThreads), 0, s, a, E, chunk);
     return;
   }
   const dim3 grid(a.g.ntz * a.g.ntx * E);
   if (precision == kExact)
-    hipLaunchKernelGGL((k_bwd_step<kExact, TX, LEAN>), grid, dim3(kThreads), 0, s, a, E);
+    hipLaunchKernelGGL((k_bwd_step<kExact, TX, LEAN, CHUNKED>), grid, dim3(kThreads), 0, s, a, E, chunk);
   else
-    hipLaunchKernelGGL((k_bwd_step<kFp32, TX, LEAN>), grid, dim3(kThreads), 0, s, a, E);
+    hipLaunchKernelGGL((k_bwd_step<kFp32, TX, LEAN, CHUNKED>), grid, dim3(kThreads), 0, s, a, E, chunk);
 }
 
 // The lean instantiation (no CSR point-list code, no u2 export) serves every
-// launch whose active point lists all have slots.
+// launch whose active point lists all have slots; the chunked block order
+// has lean instantiations only (a general launch walks the
+// entries fastest: a different order, the same results).
 inline bool fwd_lean(const FwdStepArgs& a) {
   return (a.inj.t < 0 || a.inj.pts.slots) && (a.rec.out == nullptr || a.rec.pts.slots) && a.u2out == nullptr;
 }
@@ -896,37 +903,42 @@ inline bool bwd_lean(const BwdStepArgs& a) {
 }
 
 template <int TX>
-void fwd_dispatch(const FwdStepArgs& a, int E, bool damped, int precision, hipStream_t s) {
-  if (fwd_lean(a))
-    fwd_dispatch<TX, true>(a, E, damped, precision, s);
+void fwd_dispatch(const FwdStepArgs& a, int E, bool damped, int precision, hipStream_t s, int chunk) {
+  if (!fwd_lean(a))
+    fwd_dispatch<TX, false, false>(a, E, damped, precision, s, chunk);
+  else if (chunk)
+    fwd_dispatch<TX, true, true>(a, E, damped, precision, s, chunk);
   else
-    fwd_dispatch<TX, false>(a, E, damped, precision, s);
+    fwd_dispatch<TX, true, false>(a, E, damped, precision, s, chunk);
 }
 template <int TX>
-void bwd_dispatch(const BwdStepArgs& a, int E, int precision, hipStream_t s) {
-  if (bwd_lean(a))
-    bwd_dispatch<TX, true>(a, E, precision, s);
+void bwd_dispatch(const BwdStepArgs& a, int E, int precision, hipStream_t s, int chunk) {
+  if (!bwd_lean(a))
+    bwd_dispatch<TX, false, false>(a, E, precision, s, chunk);
+  else if (chunk)
+    bwd_dispatch<TX, true, true>(a, E, precision, s, chunk);
   else
-    bwd_dispatch<TX, false>(a, E, precision, s);
+    bwd_dispatch<TX, true, false>(a, E, precision, s, chunk);
 }
 
 }  // namespace
 
 int tile_count_x(int nx, int tile_x) { return (nx - 2 * kHalo + tile_x - 1) / tile_x; }
 
-void launch_fwd_step(const FwdStepArgs& a, int nentries, bool damped, int precision, hipStream_t stream) {
+void launch_fwd_step(const FwdStepArgs& a, int nentries, bool damped, int precision, hipStream_t stream,
+                     int chunk) {
   switch (a.g.tile_x) {
-    case 8: fwd_dispatch<8>(a, nentries, damped, precision, stream); break;
-    case 32: fwd_dispatch<32>(a, nentries, damped, precision, stream); break;
-    default: fwd_dispatch<16>(a, nentries, damped, precision, stream); break;
+    case 8: fwd_dispatch<8>(a, nentries, damped, precision, stream, chunk); break;
+    case 32: fwd_dispatch<32>(a, nentries, damped, precision, stream, chunk); break;
+    default: fwd_dispatch<16>(a, nentries, damped, precision, stream, chunk); break;
   }
 }
 
-void launch_bwd_step(const BwdStepArgs& a, int nentries, int precision, hipStream_t stream) {
+void launch_bwd_step(const BwdStepArgs& a, int nentries, int precision, hipStream_t stream, int chunk) {
   switch (a.g.tile_x) {
-    case 8: bwd_dispatch<8>(a, nentries, precision, stream); break;
-    case 32: bwd_dispatch<32>(a, nentries, precision, stream); break;
-    default: bwd_dispatch<16>(a, nentries, precision, stream); break;
+    case 8: bwd_dispatch<8>(a, nentries, precision, stream, chunk); break;
+    case 32: bwd_dispatch<32>(a, nentries, precision, stream, chunk); break;
+    default: bwd_dispatch<16>(a, nentries, precision, stream, chunk); break;
   }
 }
 
