@@ -41,14 +41,18 @@ def make_members(case, v0, scales):
     return np.stack(mem)
 
 
-@pytest.mark.parametrize("maxdv,scales", [(200.0, (0.03, 0.03, 0.03)), (1000.0, (0.0, 0.04, 0.12))])
-def test_ensemble_epochs_match_per_member_oracle(oracle, setup, maxdv, scales):
+@pytest.mark.parametrize("maxdv,scales,split", [(200.0, (0.03, 0.03, 0.03), -1), (1000.0, (0.0, 0.04, 0.12), -1),
+                                                (200.0, (0.0, 0.01, 0.02, 0.03, 0.05), 0)])
+def test_ensemble_epochs_match_per_member_oracle(oracle, setup, maxdv, scales, split):
     """Members' line searches of 2 and 6 trials (maxdv=200), and of uneven
     lengths within one epoch (5, 5 and 6 trials at maxdv=1000 with members
     0%, 4% and 12% away from the initial model): the lockstep rounds then
-    carry a different number of pending trials per member."""
+    carry a different number of pending trials per member.  Five members
+    with the fused backward: batches of 4+ entries with one model each run
+    in the chunked block order (forward, fused backward, trial runs)."""
     from swfwi_amd.enkf import Enkf, EnsembleFwi
     case, s, v0, wlt, dobs = setup
+    s.set_bwd_split(split)
     members = make_members(case, v0, scales)
     N = members.shape[0]
     e = Enkf(s, wlt, dobs, 0.2, N)
@@ -95,5 +99,6 @@ def test_ensemble_epochs_match_per_member_oracle(oracle, setup, maxdv, scales):
         assert abs(obj - want) <= 1e-6 * abs(want), (obj, want)
     finally:
         oracle.set_sum_mode(0)
+        s.set_bwd_split(-1)
     fwi.close()
     e.close()
