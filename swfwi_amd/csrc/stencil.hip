@@ -21,9 +21,12 @@
 // registers before the staging so their latency overlaps it.
 //
 // Block order: blocks are spread round-robin over the 8 XCDs, so the block
-// index is remapped to give each XCD one contiguous run of work, and inside a
-// run all entries of a tile come back to back: the tile's model and the
-// halos of neighbouring tiles are then re-read from that XCD's L2.
+// index is remapped to give each XCD one contiguous run of the work list.
+// Inside a run either all entries of a tile come back to back (the tile's
+// shared model is re-read from that XCD's L2), or, for batches with one
+// model per entry and for the fused backward, chunks of tile columns come
+// entry by entry (a tile's x-halo, the neighbouring columns of the same
+// entry, is re-read from L2); the launcher picks the order.
 //
 // Precision: kExact reproduces the reference's expression trees
 // (fd4t10s-damp-zjh.c:44-54,85-88; fd4t10s-zjh.c:28-38,49-52): float ring
