@@ -8,6 +8,8 @@ forward runs), model update.  value = stencil grid-point updates per second
 over the timed epochs.
 
     python bench.py [--steps K] [--warmup W] [--precision exact|fp32]
+    python bench.py --workload fwi_201x301   # plain FWI, 201x301, 11 shots:
+                                             # FwiFramework epochs, gradient per shot
 """
 import argparse
 import json
@@ -25,10 +27,10 @@ FWD_BYTES_PER_PT = 16.0   # read cur, prev, model; write next
 BWD_BYTES_PER_PT = 40.0   # two steps (source + receiver field) + gradient read-modify-write
 
 
-def measured_traffic(kernel):
+def measured_traffic(kernel, workload):
     """HBM bytes per launch of `kernel` in this workload, from the rocprofv3
     FETCH_SIZE / WRITE_SIZE runs summarised under profiles/ (null if absent)."""
-    path = os.path.join(ROOT, "profiles", "k_fwd_step_ess_301x921_traffic.txt")
+    path = os.path.join(ROOT, "profiles", f"k_fwd_step_{workload}_traffic.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -38,11 +40,12 @@ def measured_traffic(kernel):
     return None
 
 
-def cpu_baseline(case, seconds_cap=30.0):
+def cpu_baseline(case, encoded, seconds_cap=30.0):
     """The oracle (CPU restatement of the reference path, bit-identical to it)
-    timed on this host on a bounded sample of the same workload: one ESS
-    gradient of the 301x921 grid (nt=4000: 4000 forward steps, 3700 backward
-    iterations of two steps; ~13 s with 16 threads), reference threading
+    timed on this host on a bounded sample of the same workload: one gradient
+    (ESS: the encoded supershot of the 301x921 grid, nt=4000: 4000 forward
+    steps, 3700 backward iterations of two steps, ~13 s with 16 threads;
+    plain FWI: one shot of the 201x301 grid, nt=2000), reference threading
     (serial u2 pass of the damped kernel) and all-parallel."""
     from oracle import pyoracle as po
     from swfwi_amd import workloads
@@ -55,11 +58,11 @@ def cpu_baseline(case, seconds_cap=30.0):
                     parallel_u2=par)
         vi = po.expand_domain(v0, case.nb, case.dx, case.dt)
         wlt = po.ricker(nt, case.fm, case.dt, case.amp)
-        enc = po.encode_source(po.gen_codes(1, case.ns), wlt)
+        src = po.encode_source(po.gen_codes(1, case.ns), wlt) if encoded else wlt
         rng = np.random.default_rng(0)
         vsrc = (rng.standard_normal((nt, case.ng)) * 1e-3).astype(np.float32)
         t0 = time.perf_counter()
-        g.calgradient(vi, enc, vsrc, shot=-1)
+        g.calgradient(vi, src, vsrc, shot=-1 if encoded else 0)
         dt = time.perf_counter() - t0
         iters = sum(1 for it in range(nt - 1, -1, -1) if np.float32(case.dt) * np.float32(it) > 0.3) + 1
         pts = case.update_points * (nt + 2 * iters)
@@ -77,6 +80,7 @@ def main():
     ap.add_argument("--precision", choices=["exact", "fp32"], default="exact")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--nt", type=int, default=None, help="override nt (testing only)")
+    ap.add_argument("--workload", choices=["ess_301x921", "fwi_201x301"], default="ess_301x921")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch the time loops directly instead of replaying HIP graphs (for kernel tracers)")
     args = ap.parse_args()
@@ -90,7 +94,8 @@ def main():
     if capi.device_count() < 1:
         print("no HIP device", file=sys.stderr)
         sys.exit(1)
-    case = workloads.ess_301x921(nt=args.nt or 4000)
+    ess = args.workload == "ess_301x921"
+    case = workloads.ess_301x921(nt=args.nt or 4000) if ess else workloads.fwi_201x301(nt=args.nt or 2000)
     prec = capi.PREC_EXACT if args.precision == "exact" else capi.PREC_FP32
     vtrue, v0 = workloads.models(case)
     vt = capi.expand_domain(vtrue, case.nb, case.dx, case.dt)
@@ -109,8 +114,9 @@ def main():
     if args.no_graphs:
         s.set_graphs(False)
     objs = []
+    epoch = inv.ess_epoch if ess else inv.fwi_epoch
     for it in range(args.warmup):
-        inv.ess_epoch(it)  # also captures the time loops into HIP graphs
+        epoch(it)  # also captures the time loops into HIP graphs
         objs.append(inv.stats()["obj1"])
         print(f"[bench] warmup epoch {it} obj {objs[-1]:.4e}", file=sys.stderr, flush=True)
     # the timed epochs, with one HIP event pair around every time loop on the
@@ -121,7 +127,7 @@ def main():
     stats = []
     t0 = time.perf_counter()
     for k in range(args.steps):
-        inv.ess_epoch(args.warmup + k)
+        epoch(args.warmup + k)
         stats.append(inv.stats())
     t1 = time.perf_counter()
     pu1 = s.counters()[0]
@@ -148,19 +154,20 @@ def main():
         name, ms, n, byts = "k_fwd_step", fwd_ms, fwd_n, fwd_pu * FWD_BYTES_PER_PT
     achieved = byts / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(name), "kernel": name,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(name, args.workload), "kernel": name,
             "algorithmic_bytes_per_launch": round(byts / max(n, 1)),
             "avg_launch_us": round(1e3 * ms / max(n, 1), 3), "launches": n}
 
     cpu = None
     if not args.no_cpu_baseline:
-        res, threads = cpu_baseline(case)
+        res, threads = cpu_baseline(case, ess)
         ref = res.get("reference_threading")
         par = res.get("all_parallel")
+        what = "ESS calgradient (one supershot) on the 301x921 grid" if ess else \
+            "FWI calgradient (one shot) on the 201x301 grid"
         cpu = {"value": round(ref[0], 4), "unit": "Gpts/s", "cores": threads, "kind": "port",
-               "sample": f"oracle ESS calgradient (one supershot) on the 301x921 grid, nt={ref[2]} "
-                         f"({ref[2]} forward steps + {ref[3]} backward iterations), {ref[1]:.1f} s, "
-                         "serial u2 pass as the reference build",
+               "sample": f"oracle {what}, nt={ref[2]} ({ref[2]} forward steps + {ref[3]} backward iterations), "
+                         f"{ref[1]:.1f} s, serial u2 pass as the reference build",
                "all_parallel_value": round(par[0], 4) if par else None}
 
     line = {
@@ -176,10 +183,13 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded layered model; observed data from the same propagator on the true model)",
-        "config": {"workload": "ESS-FWI 301x921 dx=12.5 nb=30 nt=4000, 60 shots x 921 receivers, one supershot "
-                               "per epoch", "precision_mode": args.precision, "graphs": not args.no_graphs,
-                   "parallelism": "single GPU"},
-        "gradient_ms_per_supershot": round(grad_ms, 3),
+        "config": {"workload": (f"ESS-FWI 301x921 dx=12.5 nb=30 nt={case.nt}, 60 shots x 921 receivers, one "
+                                "supershot per epoch") if ess else
+                               (f"FWI 201x301 dx=10 nb=20 nt={case.nt}, 11 shots x 301 receivers, every shot's "
+                                "gradient per epoch"),
+                   "precision_mode": args.precision, "graphs": not args.no_graphs, "parallelism": "single GPU"},
+        ("gradient_ms_per_supershot" if ess else "gradient_ms_per_shot"): round(grad_ms if ess else
+                                                                             grad_ms / case.ns, 3),
         "linesearch_ms": round(ls_ms, 3),
         "trial_forward_runs_per_epoch": ntrial,
         "misfit": [float(f"{o:.6e}") for o in objs],

@@ -3,6 +3,7 @@
     python -m swfwi_amd.profile_workloads copy      # 256 Mi-float device copy (counter calibration)
     python -m swfwi_amd.profile_workloads shots     # 60-shot batched forward, 301x921, nt=400
     python -m swfwi_amd.profile_workloads ess       # ESS epochs, 301x921, nt=1000 (--no-graphs for counters)
+    python -m swfwi_amd.profile_workloads fwi       # plain FWI epochs, 201x301, 11 shots, nt=2000
     python -m swfwi_amd.profile_workloads enkf      # EnKF analyses, 301x921, N members
     python -m swfwi_amd.profile_workloads enfwi     # ensemble-Kalman FWI epochs, 301x921, N members
     python -m swfwi_amd.profile_workloads sweep     # step-kernel time per tile width and batch size
@@ -81,6 +82,31 @@ def run_ess(nt, epochs, graphs=True):
         st = inv.stats()
         print(f"ess epoch {it}: obj {st['obj1']:.4e} grad {st['grad_ms']:.1f} ms "
               f"line search {st['linesearch_ms']:.1f} ms trials {st['ntrial']:.0f}", flush=True)
+
+
+def run_fwi(nt, epochs):
+    """Plain FWI on the 201x301 grid (nb=20, 11 shots, 301 receivers): the
+    observed data, then `epochs` FwiFramework epochs (every shot's gradient
+    in one batch, the trial forward runs batched); gradient time per shot."""
+    case = workloads.fwi_201x301(nt=nt)
+    vtrue, v0 = workloads.models(case)
+    s = capi.Solver(case.nx, case.nz, case.nb, case.dt, case.dx, case.fm, case.nt, case.sx, case.sz, case.gx,
+                    case.gz)
+    wlt = capi.ricker(case.nt, case.fm, case.dt, case.amp)
+    s.bind_velocity(capi.expand_domain(vtrue, case.nb, case.dx, case.dt))
+    dobs = s.forward_shots(wlt)
+    s.bind_velocity(capi.expand_domain(v0, case.nb, case.dx, case.dt))
+    inv = capi.Inversion(s, wlt, dobs, case.vmin, case.vmax, 5, 200.0)
+    for it in range(epochs):
+        pu0 = sum(s.counters()[0])
+        t0 = time.perf_counter()
+        inv.fwi_epoch(it)
+        dt = time.perf_counter() - t0
+        pu = sum(s.counters()[0]) - pu0
+        st = inv.stats()
+        print(f"fwi epoch {it}: {case.ns} shots, {dt * 1e3:.1f} ms ({pu / dt / 1e9:.1f} Gpts/s), gradient "
+              f"{st['grad_ms']:.1f} ms ({st['grad_ms'] / case.ns:.2f} ms per shot), line search "
+              f"{st['linesearch_ms']:.1f} ms, obj {st['obj1']:.4e}", flush=True)
 
 
 def run_enkf(nt, members):
@@ -166,7 +192,7 @@ def run_enfwi(nt, members, epochs):
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["copy", "shots", "ess", "enkf", "enfwi", "sweep"])
+    ap.add_argument("mode", choices=["copy", "shots", "ess", "fwi", "enkf", "enfwi", "sweep"])
     ap.add_argument("--entries", default="1,2,3,4,8,16,60,100,200")
     ap.add_argument("--reps", type=int, default=100)
     ap.add_argument("--precision", choices=["exact", "fp32"], default="exact")
@@ -190,6 +216,8 @@ def main(argv=None):
         run_enfwi(a.nt or 4000, a.members, a.epochs)
     elif a.mode == "enkf":
         run_enkf(a.nt or 4000, a.members)
+    elif a.mode == "fwi":
+        run_fwi(a.nt or 2000, a.epochs)
     else:
         run_ess(a.nt or 1000, a.epochs, graphs=not a.no_graphs)
     return 0
