@@ -64,7 +64,8 @@ def cpu_baseline(case, encoded, seconds_cap=30.0):
         t0 = time.perf_counter()
         g.calgradient(vi, src, vsrc, shot=-1 if encoded else 0)
         dt = time.perf_counter() - t0
-        iters = sum(1 for it in range(nt - 1, -1, -1) if np.float32(case.dt) * np.float32(it) > 0.3) + 1
+        # the reference's taper test: float dt*it against the double 0.3
+        iters = sum(1 for it in range(nt - 1, -1, -1) if float(np.float32(case.dt) * np.float32(it)) > 0.3) + 1
         pts = case.update_points * (nt + 2 * iters)
         out[label] = (pts / dt / 1e9, dt, nt, iters)
         if dt > seconds_cap:
