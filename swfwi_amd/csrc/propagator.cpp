@@ -207,9 +207,9 @@ void Propagator::set_bwd_split(int mode) {
 bool Propagator::bwd_split(int E) const {
   if (bwd_split_ >= 0) return bwd_split_ == 1;
   // measured on the 301x921 grid with the imaging condition: split (64x16
-  // then 64x32 tiles, entries fastest) up to 16 entries; fused (64x16, in
-  // chunks of tile columns) from 30 entries on, where the split form's second
-  // read of the receiver field costs more than the fused form's occupancy
+  // tiles, entries fastest) up to 16 entries; fused (64x16, in chunks of tile
+  // columns) from 30 entries on, where the split form's second read of the
+  // receiver field costs more than the fused form's occupancy
   const long long tiles16 = (long long)g_.ntz * tile_count_x(g_.nxpad, 16) * E;
   return tiles16 < kBwdSplitBelow;
 }
@@ -310,10 +310,10 @@ int Propagator::tile_choice(int kind, int E, const float* rv, long long rvstride
     // narrow tile wins while the batch does not fill the chip (more, shorter
     // blocks), the wide one once it is bandwidth-bound (less halo traffic per
     // output).  Forward: 64x8 at one entry, 64x16 up to 6, 64x32 beyond;
-    // backward split: 64x16 up to 4 entries, 64x32 beyond; fused: 64x16.
+    // backward: 64x16 (split up to 16 entries, fused beyond)
     const long long tiles16 = (long long)g_.ntz * tile_count_x(g_.nxpad, 16) * E;
     if (kind == 1)
-      best = bwd_split(E) && tiles16 >= 1500 ? 2 : 1;
+      best = 1;
     else
       best = tiles16 < 600 ? 0 : (tiles16 < 2500 ? 1 : 2);
   } else {
