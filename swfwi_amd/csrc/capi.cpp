@@ -229,6 +229,8 @@ int swfwi_solver_create(swfwi_solver* out, int nx, int nz, int nb, float dt, flo
     g.sz.assign(sz, sz + ns);
     g.gx.assign(gx, gx + ng);
     g.gz.assign(gz, gz + ng);
+    g.derive();  // argument errors before any device resource is created
+    if (precision != SWFWI_PREC_EXACT && precision != SWFWI_PREC_FP32) fail(kErrArg, "unknown precision mode");
     std::unique_ptr<swfwi_solver_s> h(new swfwi_solver_s);
     h->s.reset(new Solver(g, precision));
     *out = h.release();

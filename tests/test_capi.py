@@ -63,3 +63,26 @@ def test_errors_are_reported_not_fatal():
     with pytest.raises(capi.SwfwiError) as e:
         capi.expand_domain(np.zeros((0, 3), np.float32), 2, 1.0, 1.0)
     assert e.value.code == 1
+
+
+@pytest.mark.parametrize("kw,code,needle", [
+    (dict(nx=50000, nz=50000), 5, "32-bit"),               # padded grid beyond 2^31 points
+    (dict(nb=4), 1, "nb must be >= 6"),                     # the band must lie in the sponge
+    (dict(sx=[80]), 1, "sources exceeds"),                  # Params::check of the tools
+    (dict(gx=[0], gz=[40]), 1, "geophones exceeds"),
+    (dict(dt=0.0), 1, "positive"),
+    (dict(precision=7), 1, "precision"),
+])
+def test_solver_arguments_are_checked_before_any_device_call(kw, code, needle):
+    """Solver creation validates its geometry on the host first, so a bad
+    argument reports SWFWI_ERR_ARG / SWFWI_ERR_RANGE (with the reference's
+    messages) rather than a device error, with or without a GPU."""
+    a = dict(nx=60, nz=40, nb=8, dt=0.001, dx=10.0, fm=15.0, nt=100, sx=[5], sz=[1], gx=[0, 1, 2], gz=[0, 0, 0],
+             precision=0)
+    a.update(kw)
+    i32 = lambda v: np.array(v, np.int32)  # noqa: E731
+    with pytest.raises(capi.SwfwiError) as e:
+        capi.Solver(a["nx"], a["nz"], a["nb"], a["dt"], a["dx"], a["fm"], a["nt"], i32(a["sx"]), i32(a["sz"]),
+                    i32(a["gx"]), i32(a["gz"]), a["precision"])
+    assert e.value.code == code, e.value
+    assert needle in str(e.value)
