@@ -157,12 +157,12 @@ __device__ __forceinline__ bool touches_sponge(const Grid& g, int x0, int x1, in
   return x0 < g.nb || x1 > g.nx - g.nb || z1 > g.zdamp;
 }
 
-// Block -> (tile, entry): XCD-contiguous runs of the work list.  chunk = 0:
-// entries fastest (small batches).  Otherwise the list walks chunks of
-// `chunk` consecutive tiles (a few tile columns), each chunk entry by entry:
-// a tile's x-halo (the neighbouring columns of the same entry) and, when
-// the entries share a model, the chunk's model are re-read from the XCD's L2
-// instead of HBM.
+// Block -> (tile, entry): XCD-contiguous runs of the work list.  Not
+// CHUNKED: entries fastest (all entries of a tile back to back).  CHUNKED:
+// the list walks chunks of `chunk` consecutive tiles (a few tile columns),
+// each chunk entry by entry, so that a tile's x-halo (the neighbouring
+// columns of the same entry) and, when the entries share a model, the
+// chunk's model are re-read from the XCD's L2 instead of HBM.
 template <bool CHUNKED>
 __device__ __forceinline__ void block_work(int ntiles, int nentries, int chunk, int& tile, int& e) {
   const int n = ntiles * nentries;
