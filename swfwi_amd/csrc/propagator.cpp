@@ -140,6 +140,7 @@ void Fields::ensure(int e, long long npts, bool receiver) {
 Propagator::Propagator(const Geometry& g, int precision) : g_(g), prec_(precision) {
   g_.derive();
   if (precision != kExact && precision != kFp32) fail(kErrArg, "unknown precision mode");
+  for (hipEvent_t& e : lane_events_) SWFWI_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   hipStream_t s = stream_.get();
   build_damping(g_.nxpad, g_.nzpad, g_.bx0, damp_, s);
   std::vector<long long> si(g_.ns()), gi(g_.ng());
@@ -373,12 +374,11 @@ std::vector<Propagator::FwdLaunch> Propagator::plan_forward(Fields& fw) const {
 }
 
 void Propagator::forward_body(int E, const float* rv, long long rvstride, const Sources& src, float* dcal,
-                              float* band, Fields& fw, int rvdiv, int w) {
-  hipStream_t s = stream();
+                              float* band, float* fa, float* fb, int rvdiv, int w, hipStream_t s) {
   const long long np = g_.npts();
   const int nt = g_.nt, ng = g_.ng();
-  launch_fill(fw.a.get(), (long long)E * np, 0.f, s);
-  launch_fill(fw.b.get(), (long long)E * np, 0.f, s);
+  launch_fill(fa, (long long)E * np, 0.f, s);
+  launch_fill(fb, (long long)E * np, 0.f, s);
 
   Inject inj;
   inj.pts = src_[w].view();
@@ -391,7 +391,7 @@ void Propagator::forward_body(int E, const float* rv, long long rvstride, const 
 
   // step 0 injects into the initially zero cur (addSource before stepForward)
   inj.t = 0;
-  launch_point_inject(fw.b.get(), np, inj, g_.ns(), E, s);
+  launch_point_inject(fb, np, inj, g_.ns(), E, s);
 
   FwdStepArgs a;
   a.g = grid(w);
@@ -403,10 +403,11 @@ void Propagator::forward_body(int E, const float* rv, long long rvstride, const 
   a.rec.pts = rec_[w].view();
   a.rec.estride = (long long)nt * ng;
   a.band_estride = (long long)nt * g_.bsize;
-  for (const FwdLaunch& l : plan_forward(fw)) {
-    const int it = l.it;
-    a.cur = l.cur;
-    a.prev = l.prev;
+  float* cur = fb;
+  float* prv = fa;
+  for (int it = 0; it < nt; it++, std::swap(cur, prv)) {
+    a.cur = cur;
+    a.prev = prv;
     a.inj.t = it + 1 < nt ? it + 1 : -1;
     a.rec.out = dcal ? dcal + (long long)it * ng : nullptr;
     a.band = band ? band + (long long)it * g_.bsize : nullptr;
@@ -441,10 +442,44 @@ void Propagator::forward(int E, const float* rv, long long rvstride, const Sourc
                            (uintptr_t)w.c.get(),
                            (uintptr_t)w.d.get(),
                            (uintptr_t)prec_};
-  run_maybe_graph(std::vector<uintptr_t>(std::begin(key), std::end(key)), E, 0, g_.nt,
-                  [&] { forward_body(E, rv, rvstride, src, dcal, band, w, rvdiv, tw); });
+  const bool lanes = E >= 2 && E <= kLanes && band == nullptr;
+  if (!lanes) {
+    hipStream_t s = stream();
+    run_maybe_graph(
+        std::vector<uintptr_t>(std::begin(key), std::end(key)), E, 0, g_.nt,
+        [&] { forward_body(E, rv, rvstride, src, dcal, band, w.a.get(), w.b.get(), rvdiv, tw, s); }, s);
+  } else {
+    // entry e as a one-entry run on lane e, over its own slice of the batch's
+    // buffers (the same rotation, so the batch's final state is unchanged);
+    // the lanes start after the work queued so far and the main stream
+    // continues after all of them
+    const int tw1 = tile_choice(0, 1, rv, rvstride, rvdiv, w, false);
+    hipStream_t main = stream();
+    SWFWI_HIP(hipEventRecord(lane_events_[0], main));
+    for (int e = 0; e < E; e++) {
+      hipStream_t ls = e == 0 ? main : lane_streams_[e - 1].get();
+      if (e > 0) SWFWI_HIP(hipStreamWaitEvent(ls, lane_events_[0], 0));
+      Sources se = src;
+      se.amp = src.amp + (long long)e * src.estride;
+      se.lo = src.lo ? src.lo + e : nullptr;
+      se.hi = src.hi ? src.hi + e : nullptr;
+      const float* rve = rv + (long long)(e / rvdiv) * rvstride;
+      float* dce = dcal ? dcal + (long long)e * g_.nt * g_.ng() : nullptr;
+      float* fa = w.a.get() + (long long)e * np;
+      float* fb = w.b.get() + (long long)e * np;
+      const uintptr_t lkey[] = {3, (uintptr_t)tw1, (uintptr_t)e, (uintptr_t)ls, (uintptr_t)rve,
+                                (uintptr_t)se.amp, (uintptr_t)src.ld, (uintptr_t)se.lo, (uintptr_t)se.hi,
+                                (uintptr_t)dce, (uintptr_t)fa, (uintptr_t)fb, (uintptr_t)prec_};
+      run_maybe_graph(std::vector<uintptr_t>(std::begin(lkey), std::end(lkey)), 1, 0, g_.nt,
+                      [&] { forward_body(1, rve, 0, se, dce, nullptr, fa, fb, 1, tw1, ls); }, ls);
+    }
+    for (int e = 1; e < E; e++) {
+      SWFWI_HIP(hipEventRecord(lane_events_[e], lane_streams_[e - 1].get()));
+      SWFWI_HIP(hipStreamWaitEvent(main, lane_events_[e], 0));
+    }
+  }
   const std::vector<FwdLaunch> plan = plan_forward(w);  // sets the final state
-  fwd_launches += (long long)plan.size();
+  fwd_launches += (long long)plan.size() * (lanes ? E : 1);
   fwd_point_updates += (double)E * (g_.nxpad - 2 * kHalo) * (g_.nzpad - 2 * kHalo) * g_.nt;
 }
 
@@ -482,8 +517,7 @@ hipEvent_t Propagator::pooled_event() {
 }
 
 void Propagator::run_maybe_graph(const std::vector<uintptr_t>& key, int E, int kind, long long launches,
-                                 const std::function<void()>& body) {
-  hipStream_t s = stream();
+                                 const std::function<void()>& body, hipStream_t s) {
   // timing: one event pair around the whole time loop (outside any graph),
   // so the measured loop is the production one, launch gaps included
   hipEvent_t t0 = nullptr;
@@ -536,6 +570,8 @@ Propagator::~Propagator() {
     (void)hipEventDestroy(t.stop);
   }
   for (hipEvent_t e : event_pool_) (void)hipEventDestroy(e);
+  for (hipEvent_t e : lane_events_)
+    if (e) (void)hipEventDestroy(e);
 }
 
 int Propagator::backward(int E, const float* rv, long long rvstride, const Sources& src, const float* vsrc,
@@ -572,7 +608,7 @@ int Propagator::backward(int E, const float* rv, long long rvstride, const Sourc
                            (uintptr_t)bwd_split(E)};
   run_maybe_graph(std::vector<uintptr_t>(std::begin(key), std::end(key)), E, 1, g_.nt - it_min, [&] {
     backward_body(E, rv, rvstride, src, vsrc, vstride, band, grad, gstride, w, rvdiv, it_min, wts, tw);
-  });
+  }, stream());
   const int iters = g_.nt - it_min;
   bwd_launches += iters;
   bwd_point_updates += 2.0 * E * (g_.nxpad - 2 * kHalo) * (g_.nzpad - 2 * kHalo) * iters;

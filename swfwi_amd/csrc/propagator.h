@@ -169,8 +169,10 @@ class Propagator {
   };
   // the launches of a forward run over fw's buffers; sets fw's final state
   std::vector<FwdLaunch> plan_forward(Fields& fw) const;
+  // one forward run of E entries on stream s: p(-1) in fa, p(0) in fb, the
+  // two buffers then alternate (the rotation plan_forward describes)
   void forward_body(int E, const float* rv, long long rvstride, const Sources& src, float* dcal, float* band,
-                    Fields& fw, int rvdiv, int w);
+                    float* fa, float* fb, int rvdiv, int w, hipStream_t s);
   void backward_body(int E, const float* rv, long long rvstride, const Sources& src, const float* vsrc,
                      long long vstride, const float* band, float* grad, long long gstride, Fields& fw, int rvdiv,
                      int it_min, const std::vector<float>& wts, int w);
@@ -179,12 +181,17 @@ class Propagator {
   // trial of every width on scratch wavefields (the tile tuner).
   int tile_choice(int kind, int E, const float* rv, long long rvstride, int rvdiv, Fields& w, bool may_time);
   static constexpr int kGraphMaxEntries = 8;
+  // A forward batch of 2..kLanes entries without a band (line-search trial
+  // runs) runs as one-entry loops side by side, lane e on its own stream:
+  // measured on the 301x921 grid, two one-entry loops on two streams take
+  // 25 ms where one two-entry loop takes 30 ms.
+  static constexpr int kLanes = 4;
   static constexpr long long kBwdSplitBelow = 6000;  // 64x16 tiles over the batch
   // Run a time loop (`launches` step launches of kind 0 forward / 1
   // backward): captured into a HIP graph on first use and replayed when the
   // same buffers come back, or launched directly.
   void run_maybe_graph(const std::vector<uintptr_t>& key, int E, int kind, long long launches,
-                       const std::function<void()>& body);
+                       const std::function<void()>& body, hipStream_t s);
   hipEvent_t pooled_event();
 
   struct GraphEntry {
@@ -204,6 +211,8 @@ class Propagator {
   Geometry g_;
   int prec_;
   Stream stream_;
+  Stream lane_streams_[kLanes - 1];  // lanes 1.. (lane 0 is stream_)
+  hipEvent_t lane_events_[kLanes] = {};
   DampTables damp_;
   DevPoints src_[kNumWidths], rec_[kNumWidths];  // points per tile, per tile width
   int tile_override_ = 0;                 // 0 = tune, else a width from kTileWidths
