@@ -158,7 +158,8 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(name, args.workload), "kernel": name,
             "algorithmic_bytes_per_launch": round(byts / max(n, 1)),
             "avg_launch_us": round(1e3 * ms / max(n, 1), 3), "launches": n,
-            "note": ("batches of 1-2 entries" if ess else "batches of 11-22 small-grid entries") +
+            "note": ("one-entry launches (the two line-search trials as two loops side by side on two streams, "
+                     "each launch timed on its own stream)" if ess else "batches of 11-22 small-grid entries") +
                     ": the live set fits the 256 MB last-level cache, so achieved is an effective (algorithmic) "
                     "bandwidth of latency-bound launches; traffic is the measured FETCH_SIZE + WRITE_SIZE per "
                     "launch"}
