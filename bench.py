@@ -3,8 +3,8 @@
 
 A step is one EssFwiFramework epoch on the GPU: new +-1 codes, encoded
 forward modelling + boundary save (one fused pass), mute, residual, adjoint
-gradient, PR+ direction, bracketing/parabolic line search (batched trial
-forward runs), model update.  value = stencil grid-point updates per second
+gradient, PR+ direction, bracketing/parabolic line search (the trial
+forward runs side by side on their own streams), model update.  value = stencil grid-point updates per second
 over the timed epochs.
 
     python bench.py [--steps K] [--warmup W] [--precision exact|fp32]
