@@ -47,8 +47,18 @@ struct Tile {
   static constexpr int UZ = kTZ + 2;          // LDS row length of the u2 tile
   static constexpr int UX = TX + 2;
   static constexpr int PER = TX / kRows;      // outputs per thread
-  static constexpr int NSTAGE = (LX * LZ + kThreads - 1) / kThreads;
-  static constexpr int NU2 = (UX * UZ + kThreads - 1) / kThreads;
+  // Staging: every staged row's first 64 z (one per lane of the row's
+  // wavefront, SROWS rows per thread) and its last 2*kHalo z (SREST per thread).
+  static constexpr int ZREST = LZ - kTZ;
+  static constexpr int SROWS = LX / kRows;
+  static constexpr int SREST = (ZREST * LX + kThreads - 1) / kThreads;
+  static constexpr int NSTAGE = SROWS + SREST;
+  // u2: the tile's own 64 z of every u2 row (UROWS rows per thread, the last
+  // pass only in the first UX % kRows wavefronts), and the two ring z of every
+  // row (2*UX points, taken by the wavefronts with one row fewer).
+  static constexpr int UROWS = (UX + kRows - 1) / kRows;
+  static_assert(LX % kRows == 0, "staged rows split evenly over the wavefronts");
+  static_assert(2 * UX <= kThreads - (UX % kRows) * kTZ, "ring points fit the wavefronts with fewer rows");
 };
 
 // Zhang Jinhai coefficients as float, as the reference stores them.
@@ -187,52 +197,78 @@ __device__ __forceinline__ void block_work(int ntiles, int nentries, int chunk, 
 // of the thread (no branch: a slot past the staged region or outside the grid
 // reads element 0 and drops it), stage_store writes them to LDS.  FULL: the
 // staged region lies inside the grid (every tile but the last row and
-// column), no checks.
+// column), no checks.  The wavefront of thread row r loads the first 64 z of
+// staged rows r, r + 4, ... (one z per lane, so every address and LDS offset
+// of the pass is a constant step from the first); the last 2*kHalo z of every
+// staged row are spread over all threads.
 template <int TX>
 struct Staged {
   float v[Tile<TX>::NSTAGE];
+  int rest[Tile<TX>::SREST];  // LDS index of each of the thread's last-z elements (-1: none)
 };
 template <int TX, bool FULL>
 __device__ __forceinline__ void stage_load(const Grid& g, const float* __restrict__ src, int x0, int z0,
                                            Staged<TX>& st) {
   using T = Tile<TX>;
+  const int lane = threadIdx.x % kTZ, row = threadIdx.x / kTZ;
+  const int gz = z0 - kHalo + lane;  // >= 0: z0 >= kHalo
 #pragma unroll
-  for (int k = 0; k < T::NSTAGE; k++) {
-    const int i = threadIdx.x + k * kThreads;
-    const int lx = i / T::LZ, lz = i - lx * T::LZ;
-    const int gx = x0 - kHalo + lx, gz = z0 - kHalo + lz;
-    const bool ok = i < T::LX * T::LZ && (FULL || (gx >= 0 && gx < g.nx && gz >= 0 && gz < g.nz));
+  for (int k = 0; k < T::SROWS; k++) {
+    const int gx = x0 - kHalo + row + kRows * k;  // >= 0: x0 >= kHalo
+    const bool ok = FULL || (gx < g.nx && gz < g.nz);
     const float v = src[ok ? (unsigned)(gx * g.ld + gz) : 0u];
     st.v[k] = ok ? v : 0.f;
+  }
+#pragma unroll
+  for (int k = 0; k < T::SREST; k++) {
+    const int t = threadIdx.x + k * kThreads;
+    const int lx = t / T::ZREST, lz = kTZ + (t - lx * T::ZREST);
+    const int gx = x0 - kHalo + lx, gzr = z0 - kHalo + lz;
+    const bool in = t < T::ZREST * T::LX;
+    const bool ok = in && (FULL || (gx < g.nx && gzr < g.nz));
+    const float v = src[ok ? (unsigned)(gx * g.ld + gzr) : 0u];
+    st.v[T::SROWS + k] = ok ? v : 0.f;
+    st.rest[k] = in ? lx * T::LZ + lz : -1;
   }
 }
 template <int TX>
 __device__ __forceinline__ void stage_store(const Staged<TX>& st, float* __restrict__ lds) {
   using T = Tile<TX>;
+  const int lane = threadIdx.x % kTZ, row = threadIdx.x / kTZ;
 #pragma unroll
-  for (int k = 0; k < T::NSTAGE; k++) {
-    const int i = threadIdx.x + k * kThreads;
-    if (i < T::LX * T::LZ) lds[i] = st.v[k];
-  }
+  for (int k = 0; k < T::SROWS; k++) lds[(row + kRows * k) * T::LZ + lane] = st.v[k];
+#pragma unroll
+  for (int k = 0; k < T::SREST; k++)
+    if (st.rest[k] >= 0) lds[st.rest[k]] = st.v[T::SROWS + k];
 }
 
 // u2 over the tile plus a one-cell ring, only where the reference defines it
 // ([5, n-5) in both directions); elsewhere 0 (never read by a valid output).
+// The wavefront of thread row r forms the tile's own 64 z of u2 rows r, r + 4,
+// ... (one z per lane); the ring z (one below, one above the tile) of every
+// row go to the wavefronts that have one row fewer.
+template <int P, int TX, bool FULL>
+__device__ __forceinline__ void u2_point(const Grid& g, const float* __restrict__ cur, float* __restrict__ u2,
+                                         int x0, int z0, int ux, int uz) {
+  using T = Tile<TX>;
+  const int gx = x0 - 1 + ux, gz = z0 - 1 + uz;
+  float v = 0.f;
+  if (FULL || (gx >= 5 && gx < g.nx - 5 && gz >= 5 && gz < g.nz - 5))
+    v = u2_at<P, T::LZ>(cur + (ux + kHalo - 1) * T::LZ + (uz + kHalo - 1));
+  u2[ux * T::UZ + uz] = v;
+}
 template <int P, int TX, bool FULL>
 __device__ __forceinline__ void form_u2(const Grid& g, const float* __restrict__ cur, float* __restrict__ u2,
                                         int x0, int z0) {
   using T = Tile<TX>;
+  const int lane = threadIdx.x % kTZ, row = threadIdx.x / kTZ;
 #pragma unroll
-  for (int k = 0; k < T::NU2; k++) {
-    const int i = threadIdx.x + k * kThreads;
-    if (i >= T::UX * T::UZ) break;
-    const int ux = i / T::UZ, uz = i - ux * T::UZ;
-    const int gx = x0 - 1 + ux, gz = z0 - 1 + uz;
-    float v = 0.f;
-    if (FULL || (gx >= 5 && gx < g.nx - 5 && gz >= 5 && gz < g.nz - 5))
-      v = u2_at<P, T::LZ>(cur + (ux + kHalo - 1) * T::LZ + (uz + kHalo - 1));
-    u2[i] = v;
+  for (int k = 0; k < T::UROWS; k++) {
+    const int ux = row + kRows * k;
+    if (ux < T::UX) u2_point<P, TX, FULL>(g, cur, u2, x0, z0, ux, lane + 1);  // uniform over the wavefront
   }
+  const int t = (int)threadIdx.x - (T::UX % kRows) * kTZ;
+  if (t >= 0 && t < 2 * T::UX) u2_point<P, TX, FULL>(g, cur, u2, x0, z0, t >> 1, (t & 1) ? kTZ + 1 : 0);
 }
 
 // Whether the tile may hold points of the boundary band (a superset test;
