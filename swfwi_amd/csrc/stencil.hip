@@ -523,16 +523,24 @@ __device__ __forceinline__ void fwd_tile(const FwdStepArgs& a, int tile, int e, 
       nxt = apply_points(a.inj, tile, e, q, nxt);
     }
     prev[q] = nxt;
-    if (band_tile) {
-      const int bi = band_index(g, ix, iz);
-      if (bi >= 0) a.band[(long long)e * a.band_estride + bi] = c;
-    }
     if (!LEAN && lr.n > kTilePts) {
       const int b = a.rec.pts.tile_ptr[tile], eidx = a.rec.pts.tile_ptr[tile + 1];
       for (int j = b; j < eidx; j++) {
         const int pid = a.rec.pts.items[j];
         if (a.rec.pts.idx[pid] == (int)q) a.rec.out[(long long)e * a.rec.estride + pid] = c;
       }
+    }
+  }
+  // the band of cur (writeBndry), as a pass of its own so that no index
+  // arithmetic of it is computed ahead in the tiles that hold no band point
+  if (band_tile) {
+#pragma unroll
+    for (int k = 0; k < T::PER; k++) {
+      const int lx = row + kRows * k;
+      const int ix = x0 + lx;
+      if (!FULL && (ix >= g.nx - kHalo || iz >= g.nz - kHalo)) continue;
+      const int bi = band_index(g, ix, iz);
+      if (bi >= 0) a.band[(long long)e * a.band_estride + bi] = s_cur[(lx + kHalo) * T::LZ + lz + kHalo];
     }
   }
   // u2 of the ring [5,6) and [n-6,n-5) that no output point owns: the
