@@ -70,7 +70,7 @@ int run_step(float* prev, const float* cur, const float* vel, float* u2, int nx,
     NEED(cur);
     NEED(vel);
     if (nx < 2 * kHalo + 1 || nz < 2 * kHalo + 1) fail(kErrArg, "grid smaller than the stencil");
-    if ((long long)nx * nz >= (1LL << 31)) fail(kErrRange, "grid too large: a field is indexed with 32-bit offsets");
+    if ((long long)nx * nz >= (1LL << 30)) fail(kErrRange, "grid too large: a field is indexed with 32-bit byte offsets");
     if (precision != kExact && precision != kFp32) fail(kErrArg, "unknown precision mode");
     StepContext& c = step_ctx();
     std::lock_guard<std::mutex> lock(c.mu);

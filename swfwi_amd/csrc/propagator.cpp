@@ -11,8 +11,8 @@ void Geometry::derive() {
   if (nx <= 0 || nz <= 0 || nt <= 0) fail(kErrArg, "grid and time dimensions must be positive");
   if (nb < 6) fail(kErrArg, "sponge width nb must be >= 6 (the 6-wide boundary band lives inside the sponge)");
   if (!(dt > 0) || !(dx > 0)) fail(kErrArg, "dt and dx must be positive");
-  if ((long long)(nx + 2 * nb + 12) * (nz + nb + 12) >= (1LL << 31))
-    fail(kErrRange, "padded grid too large: a field is indexed with 32-bit offsets");
+  if ((long long)(nx + 2 * nb + 12) * (nz + nb + 12) >= (1LL << 30))
+    fail(kErrRange, "padded grid too large: a field is indexed with 32-bit byte offsets");
   if (sx.size() != sz.size() || gx.size() != gz.size()) fail(kErrArg, "position arrays differ in length");
   if (sx.empty() || gx.empty()) fail(kErrArg, "need at least one source and one receiver");
   // Params::check (main-fm-damp.cpp:183-194): every position inside the model

@@ -35,10 +35,22 @@
 // library is compiled with -ffp-contract=off; kFp32 uses explicit fmaf.
 #include "kernels.h"
 
+#include <type_traits>
+
 namespace swfwi {
 namespace {
 
 constexpr int kRows = kThreads / kTZ;  // thread rows per block (4)
+
+// Element i of p, with the byte offset formed in 32 bits (a field holds fewer
+// than 2^30 points, checked at handle creation): a load or store then takes
+// the block-uniform base from scalar registers and one 32-bit offset per
+// lane, with no 64-bit address arithmetic per access.
+template <class T>
+__device__ __forceinline__ T& elem(T* p, unsigned i) {
+  using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+  return *reinterpret_cast<T*>(reinterpret_cast<B*>(p) + i * (unsigned)sizeof(T));
+}
 
 template <int TX>
 struct Tile {
@@ -216,7 +228,7 @@ __device__ __forceinline__ void stage_load(const Grid& g, const float* __restric
   for (int k = 0; k < T::SROWS; k++) {
     const int gx = x0 - kHalo + row + kRows * k;  // >= 0: x0 >= kHalo
     const bool ok = FULL || (gx < g.nx && gz < g.nz);
-    const float v = src[ok ? (unsigned)(gx * g.ld + gz) : 0u];
+    const float v = elem(src, ok ? (unsigned)(gx * g.ld + gz) : 0u);
     st.v[k] = ok ? v : 0.f;
   }
 #pragma unroll
@@ -226,7 +238,7 @@ __device__ __forceinline__ void stage_load(const Grid& g, const float* __restric
     const int gx = x0 - kHalo + lx, gzr = z0 - kHalo + lz;
     const bool in = t < T::ZREST * T::LX;
     const bool ok = in && (FULL || (gx < g.nx && gzr < g.nz));
-    const float v = src[ok ? (unsigned)(gx * g.ld + gzr) : 0u];
+    const float v = elem(src, ok ? (unsigned)(gx * g.ld + gzr) : 0u);
     st.v[T::SROWS + k] = ok ? v : 0.f;
     st.rest[k] = in ? lx * T::LZ + lz : -1;
   }
@@ -483,8 +495,8 @@ __device__ __forceinline__ void fwd_tile(const FwdStepArgs& a, int tile, int e, 
     const int ix = x0 + row + kRows * k;
     const bool ok = FULL || (ix < g.nx - kHalo && iz < g.nz - kHalo);
     const unsigned q = ok ? (unsigned)(ix * g.ld + iz) : 0u;
-    pv[k] = prev[q];
-    rr[k] = rv[q];
+    pv[k] = elem(prev, q);
+    rr[k] = elem(rv, q);
     if (sponge) dx[k] = delta_x_load(g, ix);
   }
   Staged<TX> st;
@@ -522,7 +534,7 @@ __device__ __forceinline__ void fwd_tile(const FwdStepArgs& a, int tile, int e, 
     } else if (!LEAN && li.n > 0) {
       nxt = apply_points(a.inj, tile, e, q, nxt);
     }
-    prev[q] = nxt;
+    elem(prev, q) = nxt;
     if (!LEAN && lr.n > kTilePts) {
       const int b = a.rec.pts.tile_ptr[tile], eidx = a.rec.pts.tile_ptr[tile + 1];
       for (int j = b; j < eidx; j++) {
@@ -540,7 +552,7 @@ __device__ __forceinline__ void fwd_tile(const FwdStepArgs& a, int tile, int e, 
       const int ix = x0 + lx;
       if (!FULL && (ix >= g.nx - kHalo || iz >= g.nz - kHalo)) continue;
       const int bi = band_index(g, ix, iz);
-      if (bi >= 0) a.band[(long long)e * a.band_estride + bi] = s_cur[(lx + kHalo) * T::LZ + lz + kHalo];
+      if (bi >= 0) elem(a.band + (long long)e * a.band_estride, bi) = s_cur[(lx + kHalo) * T::LZ + lz + kHalo];
     }
   }
   // u2 of the ring [5,6) and [n-6,n-5) that no output point owns: the
@@ -603,17 +615,17 @@ __device__ __forceinline__ void bwd_tile(const BwdStepArgs& a, int tile, int e, 
     const int ix = x0 + row + kRows * k;
     const bool ok = FULL || (ix < g.nx - kHalo && iz < g.nz - kHalo);
     const unsigned q = ok ? (unsigned)(ix * g.ld + iz) : 0u;
-    sp[k] = s_prev[q];
-    gp[k] = g_prev[q];
-    rr[k] = rv[q];
-    if (a.do_xcorr) gr[k] = grad[q];
+    sp[k] = elem(s_prev, q);
+    gp[k] = elem(g_prev, q);
+    rr[k] = elem(rv, q);
+    if (a.do_xcorr) gr[k] = elem(grad, q);
     if (sponge) dx[k] = delta_x_load(g, ix);
     if (band_tile) {
       const int bi = ok ? band_index(g, ix, iz) : -1;
-      bv[k] = a.band[(long long)e * a.band_estride + (bi >= 0 ? bi : 0)];
+      bv[k] = elem(a.band + (long long)e * a.band_estride, bi >= 0 ? (unsigned)bi : 0u);
       // the next launch's band, read ahead so that it comes from the
       // memory-side cache rather than HBM (the values are not used)
-      if (a.band_next) pf[k] = a.band_next[(long long)e * a.band_estride + (bi >= 0 ? bi : 0)];
+      if (a.band_next) pf[k] = elem(a.band_next + (long long)e * a.band_estride, bi >= 0 ? (unsigned)bi : 0u);
       if (bi >= 0) inband |= 1u << k;
     }
   }
@@ -663,7 +675,7 @@ __device__ __forceinline__ void bwd_tile(const BwdStepArgs& a, int tile, int e, 
                       : update_at<false, P, T::UZ>(gc, gp[k], rr[k], s_gu + lu, 0.f);
     // zero-lag imaging condition with the new source field and the injected
     // receiver field (cross_correlation, fwiframework.cpp:107-112)
-    if (a.do_xcorr) grad[q] = gr[k] - sn * gc * a.weight;
+    if (a.do_xcorr) elem(grad, q) = gr[k] - sn * gc * a.weight;
     // what the next iteration (it-1) expects to find: the band of the source
     // field restored (readBndry) and the receivers injected (addSource)
     if (inband & (1u << k)) sn = bv[k];
@@ -673,8 +685,8 @@ __device__ __forceinline__ void bwd_tile(const BwdStepArgs& a, int tile, int e, 
     } else if (!LEAN && li.n > 0) {
       gn = apply_points(a.g_inj, tile, e, q, gn);
     }
-    s_prev[q] = sn;
-    g_prev[q] = gn;
+    elem(s_prev, q) = sn;
+    elem(g_prev, q) = gn;
   }
   if (band_tile && a.band_next) {  // keep the read-ahead loads (nothing reads their values)
 #pragma unroll
@@ -711,18 +723,18 @@ __device__ __forceinline__ void bwd_src_tile(const BwdStepArgs& a, int tile, int
     const int ix = x0 + row + kRows * k;
     const bool ok = FULL || (ix < g.nx - kHalo && iz < g.nz - kHalo);
     const unsigned q = ok ? (unsigned)(ix * g.ld + iz) : 0u;
-    sp[k] = s_prev[q];
-    rr[k] = rv[q];
+    sp[k] = elem(s_prev, q);
+    rr[k] = elem(rv, q);
     if (a.do_xcorr) {
-      gr[k] = grad[q];
-      gc[k] = g_cur[q];
+      gr[k] = elem(grad, q);
+      gc[k] = elem(g_cur, q);
     }
     if (band_tile) {
       const int bi = ok ? band_index(g, ix, iz) : -1;
-      bv[k] = a.band[(long long)e * a.band_estride + (bi >= 0 ? bi : 0)];
+      bv[k] = elem(a.band + (long long)e * a.band_estride, bi >= 0 ? (unsigned)bi : 0u);
       // the next launch's band, read ahead so that it comes from the
       // memory-side cache rather than HBM (the values are not used)
-      if (a.band_next) pf[k] = a.band_next[(long long)e * a.band_estride + (bi >= 0 ? bi : 0)];
+      if (a.band_next) pf[k] = elem(a.band_next + (long long)e * a.band_estride, bi >= 0 ? (unsigned)bi : 0u);
       if (bi >= 0) inband |= 1u << k;
     }
   }
@@ -754,9 +766,9 @@ __device__ __forceinline__ void bwd_src_tile(const BwdStepArgs& a, int tile, int
     } else if (!LEAN && ls.n > 0) {
       sn = apply_points(a.s_sub, tile, e, q, sn);
     }
-    if (a.do_xcorr) grad[q] = gr[k] - sn * gc[k] * a.weight;
+    if (a.do_xcorr) elem(grad, q) = gr[k] - sn * gc[k] * a.weight;
     if (inband & (1u << k)) sn = bv[k];
-    s_prev[q] = sn;
+    elem(s_prev, q) = sn;
   }
   if (band_tile && a.band_next) {  // keep the read-ahead loads (nothing reads their values)
 #pragma unroll
@@ -786,8 +798,8 @@ __device__ __forceinline__ void bwd_rcv_tile(const BwdStepArgs& a, int tile, int
     const int ix = x0 + row + kRows * k;
     const bool ok = FULL || (ix < g.nx - kHalo && iz < g.nz - kHalo);
     const unsigned q = ok ? (unsigned)(ix * g.ld + iz) : 0u;
-    gp[k] = g_prev[q];
-    rr[k] = rv[q];
+    gp[k] = elem(g_prev, q);
+    rr[k] = elem(rv, q);
     if (sponge) dx[k] = delta_x_load(g, ix);
   }
   Staged<TX> st;
@@ -820,7 +832,7 @@ __device__ __forceinline__ void bwd_rcv_tile(const BwdStepArgs& a, int tile, int
     } else if (!LEAN && li.n > 0) {
       gn = apply_points(a.g_inj, tile, e, q, gn);
     }
-    g_prev[q] = gn;
+    elem(g_prev, q) = gn;
   }
 }
 
