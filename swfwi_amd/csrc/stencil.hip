@@ -179,6 +179,27 @@ __device__ __forceinline__ bool touches_sponge(const Grid& g, int x0, int x1, in
   return x0 < g.nb || x1 > g.nx - g.nb || z1 > g.zdamp;
 }
 
+// Divisions of the block-index arithmetic, as a multiply-high by
+// m = ceil(2^32 / d): exact while numerator x d <= 2^32, which the launcher
+// checks for the largest numerator (m = 0 otherwise: plain division), so a
+// block starts with a few scalar multiplies instead of the generic integer
+// division sequences.
+struct Div {
+  int d = 1;
+  unsigned m = 0;
+};
+__device__ __forceinline__ int div_of(int n, const Div& v) {
+  if (v.d == 1) return n;
+  return v.m ? (int)__umulhi((unsigned)n, v.m) : n / v.d;
+}
+// What a block needs to find its tile and entry: the entry count of the work
+// list, the chunk length (block order below), and the divisors by entries,
+// chunk x entries, chunk, tiles along z and entries per model.
+struct BlockMap {
+  int nentries = 1, chunk = 0;
+  Div ent, span, chk, ntz, rvdiv;
+};
+
 // Block -> (tile, entry): XCD-contiguous runs of the work list.  Not
 // CHUNKED: entries fastest (all entries of a tile back to back).  CHUNKED:
 // the list walks chunks of `chunk` consecutive tiles (a few tile columns),
@@ -186,21 +207,21 @@ __device__ __forceinline__ bool touches_sponge(const Grid& g, int x0, int x1, in
 // columns of the same entry) and, when the entries share a model, the
 // chunk's model are re-read from the XCD's L2 instead of HBM.
 template <bool CHUNKED>
-__device__ __forceinline__ void block_work(int ntiles, int nentries, int chunk, int& tile, int& e) {
-  const int n = ntiles * nentries;
+__device__ __forceinline__ void block_work(int ntiles, const BlockMap& m, int& tile, int& e) {
+  const int n = ntiles * m.nentries;
   const int b = blockIdx.x;
   const int per = n / 8, rem = n % 8;
   const int xcd = b % 8, local = b / 8;
   const int l = xcd * per + (xcd < rem ? xcd : rem) + local;
   if constexpr (CHUNKED) {
-    const int c = l / (chunk * nentries);
-    const int r = l - c * chunk * nentries;
-    const int tc = min(chunk, ntiles - c * chunk);  // the last chunk may be short
-    e = r / tc;
-    tile = c * chunk + (r - e * tc);
+    const int c = div_of(l, m.span);
+    const int r = l - c * m.span.d;
+    const int tc = min(m.chunk, ntiles - c * m.chunk);  // the last chunk may be short
+    e = tc == m.chunk ? div_of(r, m.chk) : r / tc;
+    tile = c * m.chunk + (r - e * tc);
   } else {
-    tile = l / nentries;
-    e = l - tile * nentries;
+    tile = div_of(l, m.ent);
+    e = l - tile * m.nentries;
   }
 }
 
@@ -468,13 +489,13 @@ __device__ __forceinline__ float apply_points(const Inject& in, int tile, int e,
 }
 
 template <bool DAMPED, int P, int TX, bool FULL, bool LEAN>
-__device__ __forceinline__ void fwd_tile(const FwdStepArgs& a, int tile, int e, int x0, int z0, float* s_cur,
+__device__ __forceinline__ void fwd_tile(const FwdStepArgs& a, int tile, int e, int model, int x0, int z0, float* s_cur,
                                          float* s_u2, TilePts& s_inj, TilePts& s_rec, signed char* imap) {
   using T = Tile<TX>;
   const Grid& g = a.g;
   const long long fo = (long long)e * a.fstride;
   float* __restrict__ prev = a.prev + fo;
-  const float* __restrict__ rv = a.rv + (long long)(e / a.rvdiv) * a.rvstride;
+  const float* __restrict__ rv = a.rv + (long long)model * a.rvstride;
 
   const int lz = threadIdx.x % kTZ, row = __builtin_amdgcn_readfirstlane(threadIdx.x / kTZ);
   const int iz = z0 + lz;
@@ -570,29 +591,30 @@ __device__ __forceinline__ void fwd_tile(const FwdStepArgs& a, int tile, int e, 
 }
 
 template <bool DAMPED, int P, int TX, bool LEAN, bool CHUNKED>
-__global__ __launch_bounds__(kThreads) void k_fwd_step(FwdStepArgs a, int nentries, int chunk) {
+__global__ __launch_bounds__(kThreads) void k_fwd_step(FwdStepArgs a, BlockMap m) {
   using T = Tile<TX>;
   __shared__ float s_cur[T::LX * T::LZ];
   __shared__ float s_u2[T::UX * T::UZ];
   __shared__ TilePts s_inj, s_rec;
   __shared__ __align__(4) signed char imap[kTZ * TX];
   int tile, e;
-  block_work<CHUNKED>(a.g.ntz * a.g.ntx, nentries, chunk, tile, e);
-  const int z0 = kHalo + (tile % a.g.ntz) * kTZ, x0 = kHalo + (tile / a.g.ntz) * TX;
+  block_work<CHUNKED>(a.g.ntz * a.g.ntx, m, tile, e);
+  const int tx = div_of(tile, m.ntz), model = div_of(e, m.rvdiv);
+  const int z0 = kHalo + (tile - tx * a.g.ntz) * kTZ, x0 = kHalo + tx * TX;
   if (tile_is_full<TX>(a.g, x0, z0))
-    fwd_tile<DAMPED, P, TX, true, LEAN>(a, tile, e, x0, z0, s_cur, s_u2, s_inj, s_rec, imap);
+    fwd_tile<DAMPED, P, TX, true, LEAN>(a, tile, e, model, x0, z0, s_cur, s_u2, s_inj, s_rec, imap);
   else
-    fwd_tile<DAMPED, P, TX, false, LEAN>(a, tile, e, x0, z0, s_cur, s_u2, s_inj, s_rec, imap);
+    fwd_tile<DAMPED, P, TX, false, LEAN>(a, tile, e, model, x0, z0, s_cur, s_u2, s_inj, s_rec, imap);
 }
 
 template <int P, int TX, bool FULL, bool LEAN>
-__device__ __forceinline__ void bwd_tile(const BwdStepArgs& a, int tile, int e, int x0, int z0, float* s_sc,
+__device__ __forceinline__ void bwd_tile(const BwdStepArgs& a, int tile, int e, int model, int x0, int z0, float* s_sc,
                                          float* s_gc, float* s_su, float* s_gu, TilePts& s_sub, TilePts& s_inj,
                                          signed char* smap, signed char* gmap) {
   using T = Tile<TX>;
   const Grid& g = a.g;
   const long long fo = (long long)e * a.fstride;
-  const float* __restrict__ rv = a.rv + (long long)(e / a.rvdiv) * a.rvstride;
+  const float* __restrict__ rv = a.rv + (long long)model * a.rvstride;
   float* __restrict__ s_prev = a.s_prev + fo;
   float* __restrict__ g_prev = a.g_prev + fo;
   float* __restrict__ grad = a.grad + (long long)e * a.gstride;
@@ -700,12 +722,12 @@ __device__ __forceinline__ void bwd_tile(const BwdStepArgs& a, int tile, int e, 
 // read at the point, band restore) and the receiver half (damped step,
 // receiver injection).  Each half stages one field.
 template <int P, int TX, bool FULL, bool LEAN>
-__device__ __forceinline__ void bwd_src_tile(const BwdStepArgs& a, int tile, int e, int x0, int z0, float* s_c,
+__device__ __forceinline__ void bwd_src_tile(const BwdStepArgs& a, int tile, int e, int model, int x0, int z0, float* s_c,
                                              float* s_u, TilePts& tp, signed char* map) {
   using T = Tile<TX>;
   const Grid& g = a.g;
   const long long fo = (long long)e * a.fstride;
-  const float* __restrict__ rv = a.rv + (long long)(e / a.rvdiv) * a.rvstride;
+  const float* __restrict__ rv = a.rv + (long long)model * a.rvstride;
   const float* __restrict__ g_cur = a.g_cur + fo;
   float* __restrict__ s_prev = a.s_prev + fo;
   float* __restrict__ grad = a.grad + (long long)e * a.gstride;
@@ -777,12 +799,12 @@ __device__ __forceinline__ void bwd_src_tile(const BwdStepArgs& a, int tile, int
 }
 
 template <int P, int TX, bool FULL, bool LEAN>
-__device__ __forceinline__ void bwd_rcv_tile(const BwdStepArgs& a, int tile, int e, int x0, int z0, float* s_c,
+__device__ __forceinline__ void bwd_rcv_tile(const BwdStepArgs& a, int tile, int e, int model, int x0, int z0, float* s_c,
                                              float* s_u, TilePts& tp, signed char* map) {
   using T = Tile<TX>;
   const Grid& g = a.g;
   const long long fo = (long long)e * a.fstride;
-  const float* __restrict__ rv = a.rv + (long long)(e / a.rvdiv) * a.rvstride;
+  const float* __restrict__ rv = a.rv + (long long)model * a.rvstride;
   float* __restrict__ g_prev = a.g_prev + fo;
   const int lz = threadIdx.x % kTZ, row = __builtin_amdgcn_readfirstlane(threadIdx.x / kTZ);
   const int iz = z0 + lz;
@@ -837,32 +859,33 @@ __device__ __forceinline__ void bwd_rcv_tile(const BwdStepArgs& a, int tile, int
 }
 
 template <int P, int TX, bool LEAN, bool CHUNKED>
-__global__ __launch_bounds__(kThreads) void k_bwd_split(BwdStepArgs a, int nentries, int chunk) {
+__global__ __launch_bounds__(kThreads) void k_bwd_split(BwdStepArgs a, BlockMap m) {
   using T = Tile<TX>;
   __shared__ float s_c[T::LX * T::LZ];
   __shared__ float s_u[T::UX * T::UZ];
   __shared__ TilePts tp;
   __shared__ __align__(4) signed char map[kTZ * TX];
   int tile, r;
-  block_work<CHUNKED>(a.g.ntz * a.g.ntx, 2 * nentries, chunk, tile, r);  // r: entry x 2 halves
+  block_work<CHUNKED>(a.g.ntz * a.g.ntx, m, tile, r);  // r: entry x 2 halves
   const int e = r >> 1;
-  const int z0 = kHalo + (tile % a.g.ntz) * kTZ, x0 = kHalo + (tile / a.g.ntz) * TX;
+  const int tx = div_of(tile, m.ntz), model = div_of(e, m.rvdiv);
+  const int z0 = kHalo + (tile - tx * a.g.ntz) * kTZ, x0 = kHalo + tx * TX;
   const bool full = tile_is_full<TX>(a.g, x0, z0);
   if (r & 1) {
     if (full)
-      bwd_rcv_tile<P, TX, true, LEAN>(a, tile, e, x0, z0, s_c, s_u, tp, map);
+      bwd_rcv_tile<P, TX, true, LEAN>(a, tile, e, model, x0, z0, s_c, s_u, tp, map);
     else
-      bwd_rcv_tile<P, TX, false, LEAN>(a, tile, e, x0, z0, s_c, s_u, tp, map);
+      bwd_rcv_tile<P, TX, false, LEAN>(a, tile, e, model, x0, z0, s_c, s_u, tp, map);
   } else {
     if (full)
-      bwd_src_tile<P, TX, true, LEAN>(a, tile, e, x0, z0, s_c, s_u, tp, map);
+      bwd_src_tile<P, TX, true, LEAN>(a, tile, e, model, x0, z0, s_c, s_u, tp, map);
     else
-      bwd_src_tile<P, TX, false, LEAN>(a, tile, e, x0, z0, s_c, s_u, tp, map);
+      bwd_src_tile<P, TX, false, LEAN>(a, tile, e, model, x0, z0, s_c, s_u, tp, map);
   }
 }
 
 template <int P, int TX, bool LEAN, bool CHUNKED>
-__global__ __launch_bounds__(kThreads) void k_bwd_step(BwdStepArgs a, int nentries, int chunk) {
+__global__ __launch_bounds__(kThreads) void k_bwd_step(BwdStepArgs a, BlockMap m) {
   using T = Tile<TX>;
   __shared__ float s_sc[T::LX * T::LZ];
   __shared__ float s_gc[T::LX * T::LZ];
@@ -872,12 +895,13 @@ __global__ __launch_bounds__(kThreads) void k_bwd_step(BwdStepArgs a, int nentri
   __shared__ __align__(4) signed char smap[kTZ * TX];
   __shared__ __align__(4) signed char gmap[kTZ * TX];
   int tile, e;
-  block_work<CHUNKED>(a.g.ntz * a.g.ntx, nentries, chunk, tile, e);
-  const int z0 = kHalo + (tile % a.g.ntz) * kTZ, x0 = kHalo + (tile / a.g.ntz) * TX;
+  block_work<CHUNKED>(a.g.ntz * a.g.ntx, m, tile, e);
+  const int tx = div_of(tile, m.ntz), model = div_of(e, m.rvdiv);
+  const int z0 = kHalo + (tile - tx * a.g.ntz) * kTZ, x0 = kHalo + tx * TX;
   if (tile_is_full<TX>(a.g, x0, z0))
-    bwd_tile<P, TX, true, LEAN>(a, tile, e, x0, z0, s_sc, s_gc, s_su, s_gu, s_sub, s_inj, smap, gmap);
+    bwd_tile<P, TX, true, LEAN>(a, tile, e, model, x0, z0, s_sc, s_gc, s_su, s_gu, s_sub, s_inj, smap, gmap);
   else
-    bwd_tile<P, TX, false, LEAN>(a, tile, e, x0, z0, s_sc, s_gc, s_su, s_gu, s_sub, s_inj, smap, gmap);
+    bwd_tile<P, TX, false, LEAN>(a, tile, e, model, x0, z0, s_sc, s_gc, s_su, s_gu, s_sub, s_inj, smap, gmap);
 }
 
 __global__ void k_fill(float* p, long long n, float v) {
@@ -912,6 +936,29 @@ __global__ void k_reciprocal(const float* vel, float* rv, long long n) {
     rv[i] = 1.0f / vel[i];
 }
 
+// A divisor for numerators below `bound` (see Div).
+inline Div make_div(int d, long long bound) {
+  Div v;
+  v.d = d;
+  if (d > 1 && bound * d <= (1LL << 32)) v.m = (unsigned)(((1ULL << 32) + (unsigned long long)d - 1) / d);
+  return v;
+}
+// entries: entries of the work list (twice the batch for the split backward,
+// whose entries are entry x half); ents: entries of the batch.
+inline BlockMap make_map(const Grid& g, int entries, int chunk, int ents, int rvdiv) {
+  BlockMap m;
+  const int ntiles = g.ntz * g.ntx;
+  const long long nblocks = (long long)ntiles * entries;
+  m.nentries = entries;
+  m.chunk = chunk;
+  m.ent = make_div(entries, nblocks);
+  m.span = make_div(chunk > 0 ? chunk * entries : 1, nblocks);
+  m.chk = make_div(chunk > 0 ? chunk : 1, (long long)chunk * entries);
+  m.ntz = make_div(g.ntz, ntiles);
+  m.rvdiv = make_div(rvdiv > 0 ? rvdiv : 1, ents);
+  return m;
+}
+
 inline int grid_for(long long n) {
   long long b = (n + 255) / 256;
   return (int)(b > 8192 ? 8192 : (b < 1 ? 1 : b));
@@ -920,16 +967,17 @@ inline int grid_for(long long n) {
 template <int TX, bool LEAN, bool CHUNKED>
 void fwd_dispatch(const FwdStepArgs& a, int E, bool damped, int precision, hipStream_t s, int chunk) {
   const dim3 grid(a.g.ntz * a.g.ntx * E);
+  const BlockMap m = make_map(a.g, E, chunk, E, a.rvdiv);
   if (damped) {
     if (precision == kExact)
-      hipLaunchKernelGGL((k_fwd_step<true, kExact, TX, LEAN, CHUNKED>), grid, dim3(kThreads), 0, s, a, E, chunk);
+      hipLaunchKernelGGL((k_fwd_step<true, kExact, TX, LEAN, CHUNKED>), grid, dim3(kThreads), 0, s, a, m);
     else
-      hipLaunchKernelGGL((k_fwd_step<true, kFp32, TX, LEAN, CHUNKED>), grid, dim3(kThreads), 0, s, a, E, chunk);
+      hipLaunchKernelGGL((k_fwd_step<true, kFp32, TX, LEAN, CHUNKED>), grid, dim3(kThreads), 0, s, a, m);
   } else {
     if (precision == kExact)
-      hipLaunchKernelGGL((k_fwd_step<false, kExact, TX, LEAN, CHUNKED>), grid, dim3(kThreads), 0, s, a, E, chunk);
+      hipLaunchKernelGGL((k_fwd_step<false, kExact, TX, LEAN, CHUNKED>), grid, dim3(kThreads), 0, s, a, m);
     else
-      hipLaunchKernelGGL((k_fwd_step<false, kFp32, TX, LEAN, CHUNKED>), grid, dim3(kThreads), 0, s, a, E, chunk);
+      hipLaunchKernelGGL((k_fwd_step<false, kFp32, TX, LEAN, CHUNKED>), grid, dim3(kThreads), 0, s, a, m);
   }
 }
 
@@ -937,17 +985,19 @@ template <int TX, bool LEAN, bool CHUNKED>
 void bwd_dispatch(const BwdStepArgs& a, int E, int precision, hipStream_t s, int chunk) {
   if (a.split) {
     const dim3 grid(a.g.ntz * a.g.ntx * E * 2);
+    const BlockMap m = make_map(a.g, 2 * E, chunk, E, a.rvdiv);
     if (precision == kExact)
-      hipLaunchKernelGGL((k_bwd_split<kExact, TX, LEAN, CHUNKED>), grid, dim3(kThreads), 0, s, a, E, chunk);
+      hipLaunchKernelGGL((k_bwd_split<kExact, TX, LEAN, CHUNKED>), grid, dim3(kThreads), 0, s, a, m);
     else
-      hipLaunchKernelGGL((k_bwd_split<kFp32, TX, LEAN, CHUNKED>), grid, dim3(kThreads), 0, s, a, E, chunk);
+      hipLaunchKernelGGL((k_bwd_split<kFp32, TX, LEAN, CHUNKED>), grid, dim3(kThreads), 0, s, a, m);
     return;
   }
   const dim3 grid(a.g.ntz * a.g.ntx * E);
+  const BlockMap m = make_map(a.g, E, chunk, E, a.rvdiv);
   if (precision == kExact)
-    hipLaunchKernelGGL((k_bwd_step<kExact, TX, LEAN, CHUNKED>), grid, dim3(kThreads), 0, s, a, E, chunk);
+    hipLaunchKernelGGL((k_bwd_step<kExact, TX, LEAN, CHUNKED>), grid, dim3(kThreads), 0, s, a, m);
   else
-    hipLaunchKernelGGL((k_bwd_step<kFp32, TX, LEAN, CHUNKED>), grid, dim3(kThreads), 0, s, a, E, chunk);
+    hipLaunchKernelGGL((k_bwd_step<kFp32, TX, LEAN, CHUNKED>), grid, dim3(kThreads), 0, s, a, m);
 }
 
 // The lean instantiation (no CSR point-list code, no u2 export) serves every
