@@ -65,10 +65,10 @@ struct Tile {
   static constexpr int SROWS = LX / kRows;
   static constexpr int SREST = (ZREST * LX + kThreads - 1) / kThreads;
   static constexpr int NSTAGE = SROWS + SREST;
-  // u2: the tile's own 64 z of every u2 row (UROWS rows per thread, the last
-  // pass only in the first UX % kRows wavefronts), and the two ring z of every
-  // row (2*UX points, taken by the wavefronts with one row fewer).
-  static constexpr int UROWS = (UX + kRows - 1) / kRows;
+  // u2: the tile's own 64 z of every u2 row (a run of consecutive rows per
+  // wavefront, one row more in the first UX % kRows wavefronts), and the two
+  // ring z of every row (2*UX points, taken by the wavefronts with one row
+  // fewer).
   static_assert(LX % kRows == 0, "staged rows split evenly over the wavefronts");
   static_assert(2 * UX <= kThreads - (UX % kRows) * kTZ, "ring points fit the wavefronts with fewer rows");
 };
@@ -79,6 +79,29 @@ constexpr float A3 = 0.07612173f, A4 = -0.01626042f, A5 = 0.00216736f;
 constexpr double M4A0 = -4.0 * (double)A0;
 constexpr float C12 = 1.0f / 12;
 
+// u2 from the centre value and the five ring sums r_k = ((z-k + z+k) + x-k)
+// + x+k, summed as the reference does.
+template <int P>
+__device__ __forceinline__ float u2_sum(float s0, float r1, float r2, float r3, float r4, float r5) {
+  if constexpr (P == kExact) {
+    double acc = M4A0 * (double)s0;
+    acc = acc + (double)(A1 * r1);
+    acc = acc + (double)(A2 * r2);
+    acc = acc + (double)(A3 * r3);
+    acc = acc + (double)(A4 * r4);
+    acc = acc + (double)(A5 * r5);
+    return (float)acc;
+  } else {
+    float acc = (-4.0f * A0) * s0;
+    acc = __builtin_fmaf(A1, r1, acc);
+    acc = __builtin_fmaf(A2, r2, acc);
+    acc = __builtin_fmaf(A3, r3, acc);
+    acc = __builtin_fmaf(A4, r4, acc);
+    acc = __builtin_fmaf(A5, r5, acc);
+    return acc;
+  }
+}
+
 template <int P, int LZ>
 __device__ __forceinline__ float u2_at(const float* __restrict__ s) {
   // s: LDS element; z neighbours at +-k, x neighbours at +-k*LZ
@@ -87,23 +110,18 @@ __device__ __forceinline__ float u2_at(const float* __restrict__ s) {
   const float r3 = ((s[-3] + s[3]) + s[-3 * LZ]) + s[3 * LZ];
   const float r4 = ((s[-4] + s[4]) + s[-4 * LZ]) + s[4 * LZ];
   const float r5 = ((s[-5] + s[5]) + s[-5 * LZ]) + s[5 * LZ];
-  if constexpr (P == kExact) {
-    double acc = M4A0 * (double)s[0];
-    acc = acc + (double)(A1 * r1);
-    acc = acc + (double)(A2 * r2);
-    acc = acc + (double)(A3 * r3);
-    acc = acc + (double)(A4 * r4);
-    acc = acc + (double)(A5 * r5);
-    return (float)acc;
-  } else {
-    float acc = (-4.0f * A0) * s[0];
-    acc = __builtin_fmaf(A1, r1, acc);
-    acc = __builtin_fmaf(A2, r2, acc);
-    acc = __builtin_fmaf(A3, r3, acc);
-    acc = __builtin_fmaf(A4, r4, acc);
-    acc = __builtin_fmaf(A5, r5, acc);
-    return acc;
-  }
+  return u2_sum<P>(s[0], r1, r2, r3, r4, r5);
+}
+
+// The same with the x neighbours taken from w[0..10] = the column at x-5..x+5.
+template <int P>
+__device__ __forceinline__ float u2_window(const float* __restrict__ s, const float (&w)[11]) {
+  const float r1 = ((s[-1] + s[1]) + w[4]) + w[6];
+  const float r2 = ((s[-2] + s[2]) + w[3]) + w[7];
+  const float r3 = ((s[-3] + s[3]) + w[2]) + w[8];
+  const float r4 = ((s[-4] + s[4]) + w[1]) + w[9];
+  const float r5 = ((s[-5] + s[5]) + w[0]) + w[10];
+  return u2_sum<P>(w[5], r1, r2, r3, r4, r5);
 }
 
 // next from cur c, prev p, reciprocal model rv, u2 tile pointer u (at the
@@ -295,12 +313,32 @@ __device__ __forceinline__ void form_u2(const Grid& g, const float* __restrict__
                                         int x0, int z0) {
   using T = Tile<TX>;
   const int lane = threadIdx.x % kTZ, row = threadIdx.x / kTZ;
+  // the wavefront of thread row r takes the consecutive u2 rows [u0, u0 + n),
+  // sliding the column of its lane over x so that each staged value is read
+  // from LDS once per wavefront for the x-direction terms
+  constexpr int BASE = T::UX / kRows, EXTRA = T::UX % kRows;
+  const int n = BASE + (row < EXTRA ? 1 : 0);
+  const int u0 = row * BASE + (row < EXTRA ? row : EXTRA);
+  const float* __restrict__ col = cur + u0 * T::LZ + lane + kHalo;  // staged row u0 (= x of u2 row u0, minus 5)
+  float w[11];
 #pragma unroll
-  for (int k = 0; k < T::UROWS; k++) {
-    const int ux = row + kRows * k;
-    if (ux < T::UX) u2_point<P, TX, FULL>(g, cur, u2, x0, z0, ux, lane + 1);  // uniform over the wavefront
-  }
-  const int t = (int)threadIdx.x - (T::UX % kRows) * kTZ;
+  for (int d = 1; d < 11; d++) w[d] = col[(d - 1) * T::LZ];
+  const int gz = z0 + lane;
+  auto step = [&](int k) {
+#pragma unroll
+    for (int d = 0; d < 10; d++) w[d] = w[d + 1];
+    w[10] = col[(k + 10) * T::LZ];
+    const int ux = u0 + k;
+    const int gx = x0 - 1 + ux;
+    float v = 0.f;
+    if (FULL || (gx >= 5 && gx < g.nx - 5 && gz >= 5 && gz < g.nz - 5))
+      v = u2_window<P>(col + (k + 5) * T::LZ, w);
+    u2[ux * T::UZ + lane + 1] = v;
+  };
+#pragma unroll
+  for (int k = 0; k < BASE; k++) step(k);
+  if (n > BASE) step(BASE);  // uniform over the wavefront
+  const int t = (int)threadIdx.x - EXTRA * kTZ;
   if (t >= 0 && t < 2 * T::UX) u2_point<P, TX, FULL>(g, cur, u2, x0, z0, t >> 1, (t & 1) ? kTZ + 1 : 0);
 }
 
@@ -374,7 +412,7 @@ __device__ __forceinline__ PtsLoad finish_slots(const SlotFetch& f, const Inject
   const bool mine = (int)threadIdx.x < f.n;
   r.q = mine && f.sl.y >= f.lo && f.sl.y < f.hi ? f.sl.x : -1;
   r.amp = 0.f;
-  if (in && f.n > 0) r.amp = in->amp[e * in->estride + (long long)in->t * in->ld + f.sl.y];
+  if (in && f.n > 0) r.amp = elem(in->amp + e * in->estride + (long long)in->t * in->ld, (unsigned)f.sl.y);
   return r;
 }
 
@@ -471,7 +509,7 @@ __device__ __forceinline__ float apply_mapped(const TilePts& tp, int n, int m, i
 // Record the staged receivers of the tile from the staged field (slot threads).
 __device__ __forceinline__ void record_slots(const PtsLoad& r, const float* tile, int ldz, float* out) {
   if ((int)threadIdx.x >= r.n) return;
-  out[r.pid] = tile[((r.loc >> 6) + kHalo) * ldz + (r.loc & (kTZ - 1)) + kHalo];
+  elem(out, (unsigned)r.pid) = tile[((r.loc >> 6) + kHalo) * ldz + (r.loc & (kTZ - 1)) + kHalo];
 }
 
 // Apply all active point amplitudes of this tile that land on grid index q,
