@@ -16,7 +16,8 @@
 // size by the propagator's measured rule).  Its `cur` plus a 6-cell halo is staged in
 // LDS with every global load of the block in flight at once, the unscaled
 // 10th-order Laplacian u2 is formed in LDS for the tile plus a 1-cell ring,
-// and each thread finishes TX/4 points of one z-row.  prev, the model and
+// and each thread finishes TX/4 consecutive points along x at its z (the
+// wavefront of thread row r the rows r*TX/4 ...).  prev, the model and
 // (backward) the gradient of the thread's points are prefetched into
 // registers before the staging so their latency overlaps it.
 //
@@ -124,12 +125,14 @@ __device__ __forceinline__ float u2_window(const float* __restrict__ s, const fl
   return u2_sum<P>(w[5], r1, r2, r3, r4, r5);
 }
 
-// next from cur c, prev p, reciprocal model rv, u2 tile pointer u (at the
-// point), damping delta (0 inside; ignored when !DAMPED).
-template <bool DAMPED, int P, int UZ>
-__device__ __forceinline__ float update_at(float c, float p, float rv, const float* __restrict__ u, float delta) {
-  float lap = ((u[-1] + u[1]) + u[-UZ]) + u[UZ];
-  lap = lap - 4 * u[0];
+// next from cur c, prev p, reciprocal model rv, the u2 value of the point
+// u0 and its z (zm, zp) and x (xm, xp) neighbours, damping delta (0 inside;
+// ignored when !DAMPED).
+template <bool DAMPED, int P>
+__device__ __forceinline__ float update_vals(float c, float p, float rv, float u0, float zm, float zp, float xm,
+                                             float xp, float delta) {
+  float lap = ((zm + zp) + xm) + xp;
+  lap = lap - 4 * u0;
   const float cc = (C12 * rv) * rv;
   if constexpr (P == kExact) {
     double acc;
@@ -142,7 +145,7 @@ __device__ __forceinline__ float update_at(float c, float p, float rv, const flo
       acc = 2. * (double)c;
       acc = acc - (double)p;
     }
-    acc = acc + (double)(rv * u[0]);
+    acc = acc + (double)(rv * u0);
     acc = acc + (double)(cc * lap);
     return (float)acc;
   } else {
@@ -152,9 +155,27 @@ __device__ __forceinline__ float update_at(float c, float p, float rv, const flo
       b = 1 - 2 * delta;
     }
     float acc = __builtin_fmaf(a, c, -(b * p));
-    acc = __builtin_fmaf(rv, u[0], acc);
+    acc = __builtin_fmaf(rv, u0, acc);
     return __builtin_fmaf(cc, lap, acc);
   }
+}
+
+// The u2 column of a thread's outputs: the thread's z, the rows of its PER
+// consecutive outputs and one more on either side, read from the u2 tile
+// once (x neighbours of one output are the centres of the next and previous).
+template <int TX>
+__device__ __forceinline__ void u2_column(const float* __restrict__ u2, int row, int lz,
+                                          float (&col)[Tile<TX>::PER + 2]) {
+  using T = Tile<TX>;
+#pragma unroll
+  for (int j = 0; j < T::PER + 2; j++) col[j] = u2[(row * T::PER + j) * T::UZ + lz + 1];
+}
+// next of output k of the thread: u points at its u2 value (z neighbours at
+// u[-1], u[1]), col is the thread's u2 column.
+template <bool DAMPED, int P, int N>
+__device__ __forceinline__ float update_col(float c, float p, float rv, const float (&col)[N], int k,
+                                            const float* __restrict__ u, float delta) {
+  return update_vals<DAMPED, P>(c, p, rv, col[k + 1], u[-1], u[1], col[k], col[k + 2], delta);
 }
 
 // Offset of (ix, iz) in the reference's boundary band (writeBndry layout,
@@ -551,7 +572,7 @@ __device__ __forceinline__ void fwd_tile(const FwdStepArgs& a, int tile, int e, 
   if (sponge) dz = delta_z_load(g, iz);
 #pragma unroll
   for (int k = 0; k < T::PER; k++) {
-    const int ix = x0 + row + kRows * k;
+    const int ix = x0 + row * T::PER + k;
     const bool ok = FULL || (ix < g.nx - kHalo && iz < g.nz - kHalo);
     const unsigned q = ok ? (unsigned)(ix * g.ld + iz) : 0u;
     pv[k] = elem(prev, q);
@@ -576,17 +597,19 @@ __device__ __forceinline__ void fwd_tile(const FwdStepArgs& a, int tile, int e, 
   form_u2<P, TX, FULL>(g, s_cur, s_u2, x0, z0);
   __syncthreads();
 
+  float ucol[T::PER + 2];
+  u2_column<TX>(s_u2, row, lz, ucol);
 #pragma unroll
   for (int k = 0; k < T::PER; k++) {
-    const int lx = row + kRows * k;
+    const int lx = row * T::PER + k;
     const int ix = x0 + lx;
     if (!FULL && (ix >= g.nx - kHalo || iz >= g.nz - kHalo)) continue;
     const unsigned q = (unsigned)(ix * g.ld + iz);
     const float c = s_cur[(lx + kHalo) * T::LZ + lz + kHalo];
     const float* u = s_u2 + (lx + 1) * T::UZ + lz + 1;
     if (!LEAN && a.u2out) a.u2out[fo + q] = u[0];
-    float nxt = sponge ? update_at<true, P, T::UZ>(c, pv[k], rr[k], u, delta_pick(g, iz, dz, dx[k]))
-                       : update_at<false, P, T::UZ>(c, pv[k], rr[k], u, 0.f);
+    float nxt = sponge ? update_col<true, P>(c, pv[k], rr[k], ucol, k, u, delta_pick(g, iz, dz, dx[k]))
+                       : update_col<false, P>(c, pv[k], rr[k], ucol, k, u, 0.f);
     if (mapped) {
       const int h = imap[lx * kTZ + lz];
       if (h >= 0) nxt = apply_mapped(s_inj, li.n, h, a.inj.sign, q, nxt);
@@ -607,7 +630,7 @@ __device__ __forceinline__ void fwd_tile(const FwdStepArgs& a, int tile, int e, 
   if (band_tile) {
 #pragma unroll
     for (int k = 0; k < T::PER; k++) {
-      const int lx = row + kRows * k;
+      const int lx = row * T::PER + k;
       const int ix = x0 + lx;
       if (!FULL && (ix >= g.nx - kHalo || iz >= g.nz - kHalo)) continue;
       const int bi = band_index(g, ix, iz);
@@ -672,7 +695,7 @@ __device__ __forceinline__ void bwd_tile(const BwdStepArgs& a, int tile, int e, 
   if (sponge) dz = delta_z_load(g, iz);
 #pragma unroll
   for (int k = 0; k < T::PER; k++) {
-    const int ix = x0 + row + kRows * k;
+    const int ix = x0 + row * T::PER + k;
     const bool ok = FULL || (ix < g.nx - kHalo && iz < g.nz - kHalo);
     const unsigned q = ok ? (unsigned)(ix * g.ld + iz) : 0u;
     sp[k] = elem(s_prev, q);
@@ -712,17 +735,20 @@ __device__ __forceinline__ void bwd_tile(const BwdStepArgs& a, int tile, int e, 
   form_u2<P, TX, FULL>(g, s_sc, s_su, x0, z0);
   form_u2<P, TX, FULL>(g, s_gc, s_gu, x0, z0);
   __syncthreads();
+  float scol[T::PER + 2], gcol[T::PER + 2];
+  u2_column<TX>(s_su, row, lz, scol);
+  u2_column<TX>(s_gu, row, lz, gcol);
 
 #pragma unroll
   for (int k = 0; k < T::PER; k++) {
-    const int lx = row + kRows * k;
+    const int lx = row * T::PER + k;
     const int ix = x0 + lx;
     if (!FULL && (ix >= g.nx - kHalo || iz >= g.nz - kHalo)) continue;
     const unsigned q = (unsigned)(ix * g.ld + iz);
     const int l = (lx + kHalo) * T::LZ + lz + kHalo;
     const int lu = (lx + 1) * T::UZ + lz + 1;
     // source field: stepBackward (undamped) then subSource
-    float sn = update_at<false, P, T::UZ>(s_sc[l], sp[k], rr[k], s_su + lu, 0.f);
+    float sn = update_col<false, P>(s_sc[l], sp[k], rr[k], scol, k, s_su + lu, 0.f);
     if (smapped) {
       const int h = smap[lx * kTZ + lz];
       if (h >= 0) sn = apply_mapped(s_sub, ls.n, h, a.s_sub.sign, q, sn);
@@ -731,8 +757,8 @@ __device__ __forceinline__ void bwd_tile(const BwdStepArgs& a, int tile, int e, 
     }
     // receiver field: stepForward (damped) of the already injected cur
     const float gc = s_gc[l];
-    float gn = sponge ? update_at<true, P, T::UZ>(gc, gp[k], rr[k], s_gu + lu, delta_pick(g, iz, dz, dx[k]))
-                      : update_at<false, P, T::UZ>(gc, gp[k], rr[k], s_gu + lu, 0.f);
+    float gn = sponge ? update_col<true, P>(gc, gp[k], rr[k], gcol, k, s_gu + lu, delta_pick(g, iz, dz, dx[k]))
+                      : update_col<false, P>(gc, gp[k], rr[k], gcol, k, s_gu + lu, 0.f);
     // zero-lag imaging condition with the new source field and the injected
     // receiver field (cross_correlation, fwiframework.cpp:107-112)
     if (a.do_xcorr) elem(grad, q) = gr[k] - sn * gc * a.weight;
@@ -780,7 +806,7 @@ __device__ __forceinline__ void bwd_src_tile(const BwdStepArgs& a, int tile, int
   unsigned inband = 0;
 #pragma unroll
   for (int k = 0; k < T::PER; k++) {
-    const int ix = x0 + row + kRows * k;
+    const int ix = x0 + row * T::PER + k;
     const bool ok = FULL || (ix < g.nx - kHalo && iz < g.nz - kHalo);
     const unsigned q = ok ? (unsigned)(ix * g.ld + iz) : 0u;
     sp[k] = elem(s_prev, q);
@@ -812,14 +838,16 @@ __device__ __forceinline__ void bwd_src_tile(const BwdStepArgs& a, int tile, int
   }
   form_u2<P, TX, FULL>(g, s_c, s_u, x0, z0);
   __syncthreads();
+  float ucol[T::PER + 2];
+  u2_column<TX>(s_u, row, lz, ucol);
 #pragma unroll
   for (int k = 0; k < T::PER; k++) {
-    const int lx = row + kRows * k;
+    const int lx = row * T::PER + k;
     const int ix = x0 + lx;
     if (!FULL && (ix >= g.nx - kHalo || iz >= g.nz - kHalo)) continue;
     const unsigned q = (unsigned)(ix * g.ld + iz);
-    float sn = update_at<false, P, T::UZ>(s_c[(lx + kHalo) * T::LZ + lz + kHalo], sp[k], rr[k],
-                                          s_u + (lx + 1) * T::UZ + lz + 1, 0.f);
+    float sn = update_col<false, P>(s_c[(lx + kHalo) * T::LZ + lz + kHalo], sp[k], rr[k], ucol, k,
+                                    s_u + (lx + 1) * T::UZ + lz + 1, 0.f);
     if (mapped) {
       const int h = map[lx * kTZ + lz];
       if (h >= 0) sn = apply_mapped(tp, ls.n, h, a.s_sub.sign, q, sn);
@@ -855,7 +883,7 @@ __device__ __forceinline__ void bwd_rcv_tile(const BwdStepArgs& a, int tile, int
   if (sponge) dz = delta_z_load(g, iz);
 #pragma unroll
   for (int k = 0; k < T::PER; k++) {
-    const int ix = x0 + row + kRows * k;
+    const int ix = x0 + row * T::PER + k;
     const bool ok = FULL || (ix < g.nx - kHalo && iz < g.nz - kHalo);
     const unsigned q = ok ? (unsigned)(ix * g.ld + iz) : 0u;
     gp[k] = elem(g_prev, q);
@@ -876,16 +904,18 @@ __device__ __forceinline__ void bwd_rcv_tile(const BwdStepArgs& a, int tile, int
   }
   form_u2<P, TX, FULL>(g, s_c, s_u, x0, z0);
   __syncthreads();
+  float ucol[T::PER + 2];
+  u2_column<TX>(s_u, row, lz, ucol);
 #pragma unroll
   for (int k = 0; k < T::PER; k++) {
-    const int lx = row + kRows * k;
+    const int lx = row * T::PER + k;
     const int ix = x0 + lx;
     if (!FULL && (ix >= g.nx - kHalo || iz >= g.nz - kHalo)) continue;
     const unsigned q = (unsigned)(ix * g.ld + iz);
     const float c = s_c[(lx + kHalo) * T::LZ + lz + kHalo];
     const float* u = s_u + (lx + 1) * T::UZ + lz + 1;
-    float gn = sponge ? update_at<true, P, T::UZ>(c, gp[k], rr[k], u, delta_pick(g, iz, dz, dx[k]))
-                      : update_at<false, P, T::UZ>(c, gp[k], rr[k], u, 0.f);
+    float gn = sponge ? update_col<true, P>(c, gp[k], rr[k], ucol, k, u, delta_pick(g, iz, dz, dx[k]))
+                      : update_col<false, P>(c, gp[k], rr[k], ucol, k, u, 0.f);
     if (mapped) {
       const int h = map[lx * kTZ + lz];
       if (h >= 0) gn = apply_mapped(tp, li.n, h, a.g_inj.sign, q, gn);
