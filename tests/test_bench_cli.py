@@ -22,6 +22,8 @@ def test_measured_traffic_reads_the_committed_profile():
     import bench
     t = bench.measured_traffic("k_fwd_step", "ess_301x921")
     assert isinstance(t, int) and t > 0
+    tb = bench.measured_traffic("k_fwd_step_batched", "ess_301x921")
+    assert isinstance(tb, int) and tb > t
     assert bench.measured_traffic("k_fwd_step", "fwi_201x301") is None
     assert bench.measured_traffic("k_bwd_step", "ess_301x921") is None
 
