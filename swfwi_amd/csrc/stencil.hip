@@ -351,9 +351,13 @@ __device__ __forceinline__ void form_u2(const Grid& g, const float* __restrict__
     w[10] = col[(k + 10) * T::LZ];
     const int ux = u0 + k;
     const int gx = x0 - 1 + ux;
+    // the row's offset passes through an empty asm so that its z reads are
+    // addressed from one register with small immediate offsets (paired LDS
+    // reads), not re-based from the thread's first row for every pair
+    int row_off = (k + 5) * T::LZ;
+    asm("" : "+v"(row_off));
     float v = 0.f;
-    if (FULL || (gx >= 5 && gx < g.nx - 5 && gz >= 5 && gz < g.nz - 5))
-      v = u2_window<P>(col + (k + 5) * T::LZ, w);
+    if (FULL || (gx >= 5 && gx < g.nx - 5 && gz >= 5 && gz < g.nz - 5)) v = u2_window<P>(col + row_off, w);
     u2[ux * T::UZ + lane + 1] = v;
   };
 #pragma unroll
