@@ -562,10 +562,13 @@ __device__ __forceinline__ void fwd_tile(const FwdStepArgs& a, int tile, int e, 
 
   const int lz = threadIdx.x % kTZ, row = __builtin_amdgcn_readfirstlane(threadIdx.x / kTZ);
   const int iz = z0 + lz;
-  // every global load of the block is issued here, before the first barrier,
-  // and no branch waits on one: the point lists' first level, the thread's
-  // own points (a point past the grid edge of a partial tile reads element 0
-  // and drops it), the staged tile, then the lists' second level
+  // every global load of the block is issued here, before the first barrier:
+  // the point lists' first level, the thread's own points (a point past the
+  // grid edge of a partial tile reads element 0 and drops it), the staged
+  // tile, then the lists' second level.  (The compiler copies the receiver
+  // list's loaded slot where the branch that skips an inactive list joins,
+  // and that copy waits for the slot load; a branch-free form measured no
+  // faster.)
   const TileList ti = list_begin<LEAN>(a.inj.pts, &a.inj, a.inj.t >= 0, tile, e, x0, z0, g.ld);
   const TileList tr = list_begin<LEAN>(a.rec.pts, nullptr, a.rec.out != nullptr, tile, e, x0, z0, g.ld);
   const bool sponge = DAMPED && touches_sponge(g, x0, x0 + TX, z0, z0 + kTZ);
