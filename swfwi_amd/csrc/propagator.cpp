@@ -311,10 +311,11 @@ int Propagator::tile_choice(int kind, int E, const float* rv, long long rvstride
     // narrow tile wins while the batch does not fill the chip (more, shorter
     // blocks), the wide one once it is bandwidth-bound (less halo traffic per
     // output).  Forward: 64x8 at one entry, 64x16 up to 6, 64x32 beyond;
-    // backward: 64x16 (split up to 16 entries, fused beyond)
+    // backward: 64x8 at one entry, 64x16 beyond (split up to 16 entries,
+    // fused beyond)
     const long long tiles16 = (long long)g_.ntz * tile_count_x(g_.nxpad, 16) * E;
     if (kind == 1)
-      best = 1;
+      best = tiles16 < 600 ? 0 : 1;
     else
       best = tiles16 < 600 ? 0 : (tiles16 < 2500 ? 1 : 2);
   } else {
