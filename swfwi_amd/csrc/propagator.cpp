@@ -480,6 +480,7 @@ void Propagator::forward(int E, const float* rv, long long rvstride, const Sourc
     }
   }
   const std::vector<FwdLaunch> plan = plan_forward(w);  // sets the final state
+  flush_timing_if_full();
   fwd_launches += (long long)plan.size() * (lanes ? E : 1);
   fwd_point_updates += (double)E * (g_.nxpad - 2 * kHalo) * (g_.nzpad - 2 * kHalo) * g_.nt;
 }
@@ -492,11 +493,13 @@ void Propagator::set_timing(bool on) {
 }
 
 void Propagator::collect_timing() {
-  // timed loops still in flight: wait for them and add their durations
+  // timed loops still in flight: wait for each loop's own stop event (a loop
+  // may run on a lane stream that has not joined the main stream yet) and
+  // add their durations
   if (pending_.empty()) return;
-  SWFWI_HIP(hipStreamSynchronize(stream()));
   for (const TimedLoop& t : pending_) {
     float ms = 0;
+    SWFWI_HIP(hipEventSynchronize(t.stop));
     SWFWI_HIP(hipEventElapsedTime(&ms, t.start, t.stop));
     kernel_ms[t.kind] += ms;
     kernel_n[t.kind] += t.launches;
@@ -560,8 +563,13 @@ void Propagator::run_maybe_graph(const std::vector<uintptr_t>& key, int E, int k
     hipEvent_t t1 = pooled_event();
     SWFWI_HIP(hipEventRecord(t1, s));
     pending_.push_back(TimedLoop{t0, t1, kind, launches});
-    if (pending_.size() >= 64) collect_timing();
   }
+}
+
+void Propagator::flush_timing_if_full() {
+  // called once a forward()/backward() has joined its lanes, so every pending
+  // loop is ordered before work already queued on the main stream
+  if (pending_.size() >= (size_t)timing_flush_) collect_timing();
 }
 
 Propagator::~Propagator() {
@@ -610,6 +618,7 @@ int Propagator::backward(int E, const float* rv, long long rvstride, const Sourc
   run_maybe_graph(std::vector<uintptr_t>(std::begin(key), std::end(key)), E, 1, g_.nt - it_min, [&] {
     backward_body(E, rv, rvstride, src, vsrc, vstride, band, grad, gstride, w, rvdiv, it_min, wts, tw);
   }, stream());
+  flush_timing_if_full();
   const int iters = g_.nt - it_min;
   bwd_launches += iters;
   bwd_point_updates += 2.0 * E * (g_.nxpad - 2 * kHalo) * (g_.nzpad - 2 * kHalo) * iters;

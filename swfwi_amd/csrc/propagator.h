@@ -154,6 +154,8 @@ class Propagator {
   // (milliseconds, step launches).
   void set_timing(bool on);
   void collect_timing();
+  // pending timed loops collected (host waits) once this many are in flight
+  void set_timing_flush(int n) { timing_flush_ = n < 1 ? 1 : n; }
   double kernel_ms[2] = {0, 0};
   long long kernel_n[2] = {0, 0};
 
@@ -193,6 +195,8 @@ class Propagator {
   void run_maybe_graph(const std::vector<uintptr_t>& key, int E, int kind, long long launches,
                        const std::function<void()>& body, hipStream_t s);
   hipEvent_t pooled_event();
+  void flush_timing_if_full();
+  int timing_flush_ = 64;
 
   struct GraphEntry {
     std::vector<uintptr_t> key;

@@ -365,3 +365,32 @@ def test_ess_301x921_batched_shots_and_epoch(oracle):
         inv.close()
     finally:
         oracle.set_sum_mode(0)
+
+
+def test_timing_survives_many_epochs(oracle, small):
+    """Timing on across 20 ESS epochs: each epoch queues four timed loops (the
+    gradient's forward and backward, two line-search trial loops on lane
+    streams), so the pending list passes its flush size of 64 while a trial
+    lane may still be running; every loop's stop event must be waited on
+    before its time is read.  Regression for 'hipEventElapsedTime: device not
+    ready' at the 16th timed epoch."""
+    case, s, g, vt, vi, wlt = small
+    dobs = s.forward_shots(wlt)
+    try:
+        s.bind_velocity(vi)
+        inv = capi.Inversion(s, wlt, dobs, 1400.0, 3600.0, 3, 200.0)
+        inv.ess_epoch(0)  # captures the loops into graphs
+        s.set_timing(True)
+        n0 = s.counters()[1]
+        for it in range(1, 21):
+            inv.ess_epoch(it)
+        kt = s.kernel_times()
+        n1 = s.counters()[1]
+        s.set_timing(False)
+        fwd_ms, fwd_n = kt["fwd"]
+        bwd_ms, bwd_n = kt["bwd"]
+        assert fwd_ms > 0 and bwd_ms > 0
+        assert fwd_n == n1[0] - n0[0] and bwd_n == n1[1] - n0[1], (kt, n0, n1)
+        inv.close()
+    finally:
+        s.bind_velocity(vt)
