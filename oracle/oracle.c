@@ -584,10 +584,22 @@ void orc_ess_init(orc_ess *e) {
 
 void orc_ess_free(orc_ess *e) { free(e->g0); free(e->dir); free(e->last_grad); }
 
+/* Scripted objective for the line-search tests: when set, calobjval returns
+ * hook(steplen, ctx) instead of running a trial forward model, so that every
+ * branch of refineAlpha can be driven by a hand-made objective table. */
+typedef float (*orc_objective_hook)(float steplen, void *ctx);
+static orc_objective_hook orc_obj_hook = 0;
+static void *orc_obj_hook_ctx = 0;
+void orc_set_objective_hook(orc_objective_hook f, void *ctx) { orc_obj_hook = f; orc_obj_hook_ctx = ctx; }
+
 /* UpdateSteplenOp::calobjval (updatesteplenop.cpp:59-86): the mute uses the
  * trial model; shot < 0 = ESS (mute by source 0), else FWI shot. */
 static float orc_calobjval(orc_ess *e, const float *dir, float steplen, const float *encsrc, const float *encobs,
                            int shot) {
+  if (orc_obj_hook) {
+    e->n_trial_runs++;
+    return orc_obj_hook(steplen, orc_obj_hook_ctx);
+  }
   const orc_geom *g = &e->geom;
   size_t n = (size_t)g->nx * g->nz, m = (size_t)g->nt * g->ng;
   float *nv = (float *)malloc(sizeof(float) * n);
@@ -670,12 +682,8 @@ static void orc_init_alpha23(orc_ess *e, float maxAlpha3, float *a2, float *a3) 
   *a2 = *a3 * 0.5;
 }
 
-/* calsteplen (updatesteplenop.cpp:216-265) */
-static void orc_calsteplen(orc_ess *e, float obj_val1, float *steplen, float *objval) {
-  const orc_geom *g = &e->geom;
-  size_t n = (size_t)g->nx * g->nz;
-  float max_alpha2, max_alpha3;
-  orc_cal_max_alpha23(e->vel, e->dir, n, g->dt, g->dx, e->maxdv, &max_alpha2, &max_alpha3);
+/* calsteplen after calMaxAlpha2_3 (updatesteplenop.cpp:231-264) */
+static void orc_steplen_search(orc_ess *e, float obj_val1, float max_alpha3, float *steplen, float *objval) {
   float alpha1 = 0, alpha2, alpha3;
   orc_init_alpha23(e, max_alpha3, &alpha2, &alpha3);
   float obj_val2, obj_val3;
@@ -693,6 +701,35 @@ static void orc_calsteplen(orc_ess *e, float obj_val1, float *steplen, float *ob
   *objval = obj_val4;
   e->last_alpha2 = alpha2; e->last_alpha3 = alpha3; e->last_obj2 = obj_val2; e->last_obj3 = obj_val3;
   e->last_parabolic = toParabolic;
+}
+
+/* calsteplen (updatesteplenop.cpp:216-265) */
+static void orc_calsteplen(orc_ess *e, float obj_val1, float *steplen, float *objval) {
+  const orc_geom *g = &e->geom;
+  size_t n = (size_t)g->nx * g->nz;
+  float max_alpha2, max_alpha3;
+  orc_cal_max_alpha23(e->vel, e->dir, n, g->dt, g->dx, e->maxdv, &max_alpha2, &max_alpha3);
+  orc_steplen_search(e, obj_val1, max_alpha3, steplen, objval);
+}
+
+/* One step-length search on a scripted objective (orc_set_objective_hook),
+ * for the line-search tests: state[0] = preservedAlpha.alpha, state[1] =
+ * preservedAlpha.init on entry, both updated on return; out[10] as
+ * orc_ess_scalars (obj1, steplen, alpha2, alpha3, obj2, obj3, objval, -,
+ * trial evaluations, parabolic). */
+void orc_ess_scalars(const orc_ess *e, float *out);
+void orc_steplen_scripted(float obj_val1, float max_alpha3, int nita, float *state, float *out) {
+  orc_ess e;
+  memset(&e, 0, sizeof e);
+  e.nita = nita;
+  e.preserved_alpha = state[0];
+  e.preserved_init = state[1] != 0;
+  float steplen, objval;
+  orc_steplen_search(&e, obj_val1, max_alpha3, &steplen, &objval);
+  state[0] = e.preserved_alpha;
+  state[1] = (float)e.preserved_init;
+  e.last_obj1 = obj_val1; e.last_steplen = steplen; e.updateobj = objval;
+  orc_ess_scalars(&e, out);
 }
 
 /* EssFwiFramework::epoch (essfwiframework.cpp:191-239) */

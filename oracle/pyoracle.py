@@ -19,6 +19,7 @@ f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
 i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 VP = C.c_void_p
 I, F = C.c_int, C.c_float
+OBJ_HOOK = C.CFUNCTYPE(C.c_float, C.c_float, C.c_void_p)
 
 
 def lib():
@@ -73,6 +74,8 @@ def lib():
             "orc_ess_scalars": (None, [VP, f32p]),
             "orc_ess_copy_grad": (None, [VP, f32p]),
             "orc_ess_copy_dir": (None, [VP, f32p]),
+            "orc_set_objective_hook": (None, [OBJ_HOOK, VP]),
+            "orc_steplen_scripted": (None, [F, F, I, f32p, f32p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -144,6 +147,28 @@ def cal_max_alpha23(vel, grad, dt, dx, maxdv):
     a2, a3 = F(), F()
     lib().orc_cal_max_alpha23(vel.ravel(), grad.ravel(), vel.size, dt, dx, maxdv, C.byref(a2), C.byref(a3))
     return a2.value, a3.value
+
+
+def steplen_scripted(objective, obj1, max_alpha3, nita, preserved=(0.0, 0)):
+    """calsteplen's search (initAlpha23 + refineAlpha + parabola) with
+    calobjval replaced by objective(alpha) -> float.  Returns (trial alphas in
+    evaluation order, scalars as Ess.scalars, new preserved state)."""
+    trials = []
+
+    def hook(alpha, _ctx):
+        trials.append(alpha)
+        return float(np.float32(objective(alpha)))
+
+    cb = OBJ_HOOK(hook)
+    state = np.array([preserved[0], preserved[1]], np.float32)
+    out = np.zeros(10, np.float32)
+    lib().orc_set_objective_hook(cb, None)
+    try:
+        lib().orc_steplen_scripted(obj1, max_alpha3, nita, state, out)
+    finally:
+        lib().orc_set_objective_hook(OBJ_HOOK(), None)
+    keys = ["obj1", "steplen", "alpha2", "alpha3", "obj2", "obj3", "updateobj", "initobj", "ntrial", "parabolic"]
+    return trials, dict(zip(keys, out.tolist())), (float(state[0]), int(state[1]))
 
 
 def clip_position(s):
