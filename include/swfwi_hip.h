@@ -95,6 +95,16 @@ int swfwi_read_bndry(swfwi_solver h, const float *bndr, float *field);
 /* FwiForwardModeling (shot >= 0; encsrc[nt], damp4t10d.cpp:272-289) or
  * EssForwardModeling (shot < 0; encsrc[nt][ns], damp4t10d.cpp:291-307). */
 int swfwi_forward_modeling(swfwi_solver h, const float *encsrc, int shot, float *dcal);
+/* swfwi_forward_modeling without any device allocation.  No counterpart in
+ * the reference: for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments and results as swfwi_forward_modeling; the wavefields, the
+ * device seismogram and the source table live in a fixed per-GPU workspace
+ * of 64 MiB held until the process ends: a geometry needing more than
+ * 2*nxpad*nzpad + nt*ng + nt*ns floats (each region rounded up to 64 floats)
+ * beyond that returns SWFWI_ERR_RANGE.  Every call on one GPU shares that
+ * GPU's workspace: calls that overlap on one GPU race. */
+int swfwi_forward_modeling_noalloc(swfwi_solver h, const float *encsrc, int shot, float *dcal);
 /* calgradient (fwiframework.cpp:114-177 for shot >= 0,
  * essfwiframework.cpp:114-173 for shot < 0): vsrc [nt][ng] adjoint source,
  * grad [nxpad][nzpad] receives the raw imaging condition (before

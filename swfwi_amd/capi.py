@@ -13,6 +13,9 @@ LIB_PATH = os.path.join(HERE, "libswfwi_hip.so")
 PREC_EXACT = 0
 PREC_FP32 = 1
 
+# status codes (include/swfwi_hip.h)
+OK, ERR_ARG, ERR_STATE, ERR_HIP, ERR_NUMERIC, ERR_RANGE = 0, 1, 2, 3, 4, 5
+
 _f32 = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
 _i32 = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
 _f64 = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
@@ -40,6 +43,7 @@ SIGNATURES = {
     "swfwi_solver_bind_velocity": (_I, [_VP, _f32]),
     "swfwi_solver_get_velocity": (_I, [_VP, _f32]),
     "swfwi_forward_modeling": (_I, [_VP, _f32, _I, _f32]),
+    "swfwi_forward_modeling_noalloc": (_I, [_VP, _f32, _I, _f32]),
     "swfwi_write_bndry": (_I, [_VP, _f32, _f32]),
     "swfwi_read_bndry": (_I, [_VP, _f32, _f32]),
     "swfwi_calgradient": (_I, [_VP, _f32, _I, _f32, _f32]),
@@ -216,6 +220,13 @@ class Solver:
     def forward_modeling(self, encsrc, shot=-1):
         dcal = np.zeros((self.nt, self.ng), np.float32)
         check(load().swfwi_forward_modeling(self.h, _f(encsrc), shot, dcal))
+        return dcal
+
+    def forward_modeling_noalloc(self, encsrc, shot=-1):
+        """forward_modeling over the library's fixed per-GPU workspace (no
+        device allocation); raises SwfwiError (code 5) past its size."""
+        dcal = np.zeros((self.nt, self.ng), np.float32)
+        check(load().swfwi_forward_modeling_noalloc(self.h, _f(encsrc), shot, dcal))
         return dcal
 
     def calgradient(self, encsrc, vsrc, shot=-1):

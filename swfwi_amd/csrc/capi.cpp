@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "hostmath.h"
+#include "noalloc.h"
 #include "solver.h"
 
 using namespace swfwi;
@@ -315,6 +316,17 @@ int swfwi_forward_modeling(swfwi_solver h, const float* encsrc, int shot, float*
     NEED(encsrc);
     NEED(dcal);
     h->s->forward_modeling(encsrc, shot, dcal);
+  });
+}
+
+int swfwi_forward_modeling_noalloc(swfwi_solver h, const float* encsrc, int shot, float* dcal) {
+  return guarded([&] {
+    NEED(h);
+    NEED(encsrc);
+    NEED(dcal);
+    Solver& s = *h->s;
+    float* ws = noalloc_scratch(s.prop().device());
+    s.forward_modeling_ws(encsrc, shot, dcal, ws, kNoAllocScratchFloats);
   });
 }
 

@@ -140,6 +140,7 @@ void Fields::ensure(int e, long long npts, bool receiver) {
 Propagator::Propagator(const Geometry& g, int precision) : g_(g), prec_(precision) {
   g_.derive();
   if (precision != kExact && precision != kFp32) fail(kErrArg, "unknown precision mode");
+  SWFWI_HIP(hipGetDevice(&dev_));
   for (hipEvent_t& e : lane_events_) SWFWI_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   hipStream_t s = stream_.get();
   build_damping(g_.nxpad, g_.nzpad, g_.bx0, damp_, s);
@@ -296,28 +297,30 @@ float Propagator::time_steps(int kind, int E, const float* rv, long long rvstrid
   return 1e3f * ms / (reps > 0 ? reps : 1);
 }
 
-int Propagator::tile_choice(int kind, int E, const float* rv, long long rvstride, int rvdiv, Fields& w,
-                            bool may_time) {
+int Propagator::fixed_width(int kind, int E) const {
   if (tile_override_) {
     for (int i = 0; i < kNumWidths; i++)
       if (kTileWidths[i] == tile_override_) return i;
   }
+  // measured on the 301x921 grid (372 tiles of 64x16 per entry): the narrow
+  // tile wins while the batch does not fill the chip (more, shorter blocks),
+  // the wide one once it is bandwidth-bound (less halo traffic per output).
+  // Forward: 64x8 at one entry, 64x16 up to 6, 64x32 beyond; backward: 64x8
+  // at one entry, 64x16 beyond (split up to 16 entries, fused beyond)
+  const long long tiles16 = (long long)g_.ntz * tile_count_x(g_.nxpad, 16) * E;
+  if (kind == 1) return tiles16 < 600 ? 0 : 1;
+  return tiles16 < 600 ? 0 : (tiles16 < 2500 ? 1 : 2);
+}
+
+int Propagator::tile_choice(int kind, int E, const float* rv, long long rvstride, int rvdiv, Fields& w,
+                            bool may_time) {
+  if (tile_override_) return fixed_width(kind, E);
   const long long key = tile_key(kind, E);
   for (const auto& kv : tile_cache_)
     if (kv.first == key) return kv.second;
   int best = 1;
   if (!tune_ || !may_time) {
-    // measured on the 301x921 grid (372 tiles of 64x16 per entry): the
-    // narrow tile wins while the batch does not fill the chip (more, shorter
-    // blocks), the wide one once it is bandwidth-bound (less halo traffic per
-    // output).  Forward: 64x8 at one entry, 64x16 up to 6, 64x32 beyond;
-    // backward: 64x8 at one entry, 64x16 beyond (split up to 16 entries,
-    // fused beyond)
-    const long long tiles16 = (long long)g_.ntz * tile_count_x(g_.nxpad, 16) * E;
-    if (kind == 1)
-      best = tiles16 < 600 ? 0 : 1;
-    else
-      best = tiles16 < 600 ? 0 : (tiles16 < 2500 ? 1 : 2);
+    best = fixed_width(kind, E);
   } else {
     // timed trial on scratch wavefields (the caller's workspace is free at
     // this point: forward() tunes before it zeroes and runs); three
@@ -483,6 +486,12 @@ void Propagator::forward(int E, const float* rv, long long rvstride, const Sourc
   flush_timing_if_full();
   fwd_launches += (long long)plan.size() * (lanes ? E : 1);
   fwd_point_updates += (double)E * (g_.nxpad - 2 * kHalo) * (g_.nzpad - 2 * kHalo) * g_.nt;
+}
+
+void Propagator::forward_external(const float* rv, const Sources& src, float* dcal, float* fa, float* fb) {
+  forward_body(1, rv, 0, src, dcal, nullptr, fa, fb, 1, fixed_width(0, 1), stream());
+  fwd_launches += g_.nt;
+  fwd_point_updates += (double)(g_.nxpad - 2 * kHalo) * (g_.nzpad - 2 * kHalo) * g_.nt;
 }
 
 void Propagator::set_timing(bool on) {

@@ -87,6 +87,7 @@ class Propagator {
   void set_precision(int p) { prec_ = p; }
   hipStream_t stream() const { return stream_.get(); }
   void sync() const { stream_.sync(); }
+  int device() const { return dev_; }  // the HIP device current at creation
   Grid grid(int w = 1) const;  // w indexes kTileWidths
   const int* src_xp() const { return sxp_.get(); }  // padded coordinates (device)
   const int* src_zp() const { return szp_.get(); }
@@ -109,6 +110,12 @@ class Propagator {
   // band [E][nt][bsize] when those are non-null.
   void forward(int E, const float* rv, long long rvstride, const Sources& src, float* dcal, float* band, Fields& w,
                int rvdiv = 1);
+
+  // One forward run of one entry (the Sources / dcal contract of forward())
+  // over two caller-provided field buffers of npts() floats each, on the
+  // handle's stream: no graph, no timing, no workspace growth and the fixed
+  // tile rule instead of the tuner, so nothing is allocated.  No band.
+  void forward_external(const float* rv, const Sources& src, float* dcal, float* fa, float* fb);
 
   // The backward half of calgradient on the state forward() left in w:
   // vsrc [E][nt][ng] adjoint sources (+e*vstride), grad accumulates the
@@ -182,6 +189,8 @@ class Propagator {
   // 1 backward) and batch size: fixed override, cached choice, or a timed
   // trial of every width on scratch wavefields (the tile tuner).
   int tile_choice(int kind, int E, const float* rv, long long rvstride, int rvdiv, Fields& w, bool may_time);
+  // the measured width rule (index into kTileWidths) by batch size, or the override
+  int fixed_width(int kind, int E) const;
   static constexpr int kGraphMaxEntries = 8;
   // A forward batch of 2..kLanes entries without a band (line-search trial
   // runs) runs as one-entry loops side by side, lane e on its own stream:
@@ -214,6 +223,7 @@ class Propagator {
 
   Geometry g_;
   int prec_;
+  int dev_ = 0;
   Stream stream_;
   Stream lane_streams_[kLanes - 1];  // lanes 1.. (lane 0 is stream_)
   hipEvent_t lane_events_[kLanes] = {};

@@ -82,6 +82,51 @@ void Solver::forward_modeling(const float* encsrc, int shot, float* dcal_host) {
   prop_.sync();
 }
 
+namespace {
+// 256-byte aligned float count of a workspace region
+size_t ws_round(size_t n) { return (n + 63) / 64 * 64; }
+}  // namespace
+
+size_t Solver::workspace_floats() const {
+  const Geometry& g = geo();
+  // two fields, the recorded seismogram, the source table, lo/hi ids
+  return 2 * ws_round((size_t)npts()) + ws_round((size_t)g.nt * g.ng()) + ws_round((size_t)g.nt * g.ns()) +
+         ws_round(2);
+}
+
+void Solver::forward_modeling_ws(const float* encsrc_host, int shot, float* dcal_host, float* ws, size_t ws_floats) {
+  if (!bound_) fail(kErrState, "bind the velocity first");
+  const Geometry& g = geo();
+  const int ns = g.ns(), nt = g.nt, ng = g.ng();
+  if (shot >= ns) fail(kErrArg, "shot id out of range");
+  const size_t need = workspace_floats();
+  if (need > ws_floats)
+    fail(kErrRange, "forward modelling needs " + std::to_string(need * sizeof(float)) +
+                        " bytes of workspace, more than the " + std::to_string(ws_floats * sizeof(float)) +
+                        " available");
+  hipStream_t s = stream();
+  float* fa = ws;
+  float* fb = fa + ws_round((size_t)npts());
+  float* dc = fb + ws_round((size_t)npts());
+  float* tab = dc + ws_round((size_t)nt * ng);
+  int* lohi = reinterpret_cast<int*>(tab + ws_round((size_t)nt * ns));
+  const int ids[2] = {shot, shot + 1};
+  if (shot >= 0) {
+    // encsrc[t] into column `shot` of the [nt][ns] table; the other columns
+    // are never read (the entry injects source ids [shot, shot+1) only)
+    SWFWI_HIP(hipMemcpy2DAsync(tab + shot, sizeof(float) * ns, encsrc_host, sizeof(float), sizeof(float), nt,
+                               hipMemcpyHostToDevice, s));
+    SWFWI_HIP(hipMemcpyAsync(lohi, ids, sizeof(ids), hipMemcpyHostToDevice, s));
+  } else {
+    SWFWI_HIP(hipMemcpyAsync(tab, encsrc_host, sizeof(float) * nt * ns, hipMemcpyHostToDevice, s));
+  }
+  Propagator::Sources src{tab, ns, 0, shot >= 0 ? lohi : nullptr, shot >= 0 ? lohi + 1 : nullptr};
+  prop_.forward_external(rv_.get(), src, dc, fa, fb);
+  SWFWI_HIP(hipMemcpyAsync(dcal_host, dc, sizeof(float) * nt * ng, hipMemcpyDeviceToHost, s));
+  // the host arrays (ids included) must outlive the copies
+  prop_.sync();
+}
+
 void Solver::calgradient(const float* encsrc, int shot, const float* vsrc_host, float* grad_host) {
   if (!bound_) fail(kErrState, "bind the velocity first");
   const Geometry& g = geo();

@@ -54,6 +54,12 @@ class Solver {
   // calgradient with the same source convention; vsrc [nt][ng]; grad (padded)
   // receives the raw accumulated imaging condition.
   void calgradient(const float* encsrc, int shot, const float* vsrc, float* grad);
+  // forward_modeling with every device buffer carved from a caller-provided
+  // workspace of `ws_floats` floats (256-byte aligned) instead of the
+  // handle's own: allocates nothing.  Fails with kErrRange when the workspace
+  // is too small (workspace_floats() gives the need).
+  void forward_modeling_ws(const float* encsrc, int shot, float* dcal, float* ws, size_t ws_floats);
+  size_t workspace_floats() const;
   // Observed-data generation for every shot (fm-damp): dobs [ns][nt][ng].
   void forward_shots(const float* wlt, float* dobs);
 
