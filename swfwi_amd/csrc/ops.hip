@@ -349,6 +349,102 @@ __global__ void k_gram(const double* A, const double* B, int N, long long m, lon
   if (i < N && j < N) part[(long long)blockIdx.z * N * N + i + (long long)N * j] = acc;
 }
 
+// FP64 MFMA form for ensembles of up to kGMaxN members: each block takes a
+// run of data rows and forms every 16x16 output tile of X^T Y (only the
+// upper-triangle tiles when X == Y) from rows staged through LDS, so each
+// row of X and Y is read from HBM once.  v_mfma_f64_16x16x4f64: lane l
+// supplies A[i = l%16][k = l/16] and B[k = l/16][j = l%16] and holds
+// D[i = l/16 + 4*v][j = l%16] in accumulator item v (measured).
+constexpr int kGMaxN = 112;   // 7 tiles of 16 members
+constexpr int kGKC = 16;      // data rows per staged chunk
+constexpr int kGLd = kGKC + 1;
+constexpr int kGWaves = 4;
+constexpr int kGMaxTilesPerWave = 13;  // ceil(49 / 4)
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+template <bool SYM>
+__global__ __launch_bounds__(kGWaves * 64) void k_gram_mfma(const double* __restrict__ X, const double* __restrict__ Y,
+                                                            int N, long long m, long long rows_per_block,
+                                                            double* __restrict__ part) {
+  __shared__ double sx[kGMaxN * kGLd];
+  __shared__ double sy[SYM ? 1 : kGMaxN * kGLd];
+  const double* sY = SYM ? sx : sy;
+  const int T = (N + 15) / 16;
+  const int npad = T * 16;
+  const int ntiles = SYM ? T * (T + 1) / 2 : T * T;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  // this wave's tiles: t = wave, wave + kGWaves, ...
+  int ti[kGMaxTilesPerWave], tj[kGMaxTilesPerWave];
+  v4d acc[kGMaxTilesPerWave];
+#pragma unroll
+  for (int s = 0; s < kGMaxTilesPerWave; s++) {
+    int t = wave + s * kGWaves, I = 0, J = 0;
+    if (SYM) {  // upper triangle, row by row
+      int rem = t;
+      while (I < T && rem >= T - I) {
+        rem -= T - I;
+        I++;
+      }
+      J = I + rem;
+    } else {
+      I = t / (T > 0 ? T : 1);
+      J = t - I * T;
+    }
+    ti[s] = t < ntiles ? I : -1;
+    tj[s] = J;
+    acc[s] = v4d{0, 0, 0, 0};
+  }
+  const long long r0 = (long long)blockIdx.x * rows_per_block;
+  const long long r1 = r0 + rows_per_block < m ? r0 + rows_per_block : m;
+  for (long long rb = r0; rb < r1; rb += kGKC) {
+    // stage rows [rb, rb + kGKC) of every member: 16 consecutive threads
+    // read one member's 128 contiguous bytes
+    for (int q = threadIdx.x; q < npad * kGKC; q += kGWaves * 64) {
+      const int i = q / kGKC, k = q - i * kGKC;
+      const long long r = rb + k;
+      const bool in = i < N && r < r1;
+      sx[i * kGLd + k] = in ? X[(long long)i * m + r] : 0.0;
+      if (!SYM) sy[i * kGLd + k] = in ? Y[(long long)i * m + r] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < kGMaxTilesPerWave; s++) {
+      if (ti[s] < 0) continue;
+      const double* pa = sx + (ti[s] * 16 + li) * kGLd + lk;
+      const double* pb = sY + (tj[s] * 16 + li) * kGLd + lk;
+#pragma unroll
+      for (int kk = 0; kk < kGKC; kk += 4) acc[s] = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[kk], pb[kk], acc[s], 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  double* out = part + (long long)blockIdx.x * N * N;
+#pragma unroll
+  for (int s = 0; s < kGMaxTilesPerWave; s++) {
+    if (ti[s] < 0) continue;
+    const int j = tj[s] * 16 + li;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int i = ti[s] * 16 + lk + 4 * v;
+      if (i < N && j < N) out[i + (long long)N * j] = acc[s][v];
+    }
+  }
+  (void)npad;
+}
+
+// sum of the per-block partials in block order; for a symmetric product only
+// the upper-triangle tiles were formed, the lower ones are mirrored
+__global__ void k_gram_mfma_finish(const double* part, int nchunk, int N, bool sym, double* C) {
+  const int NN = N * N;
+  for (int q = blockIdx.x * kB + threadIdx.x; q < NN; q += gridDim.x * kB) {
+    const int i = q % N, j = q / N;
+    const int src = (sym && i / 16 > j / 16) ? j + N * i : q;
+    double s = 0;
+    for (int c = 0; c < nchunk; c++) s += part[(long long)c * NN + src];
+    C[q] = s;
+  }
+}
+
 __global__ void k_gram_finish(const double* part, int nchunk, int NN, double* C) {
   for (int q = blockIdx.x * kB + threadIdx.x; q < NN; q += gridDim.x * kB) {
     double s = 0;
@@ -526,6 +622,27 @@ double op_launch_latency(int blocks, int reps, bool graph, hipStream_t s) {
 }
 
 void op_gram_f64(const double* A, const double* B, int N, long long m, double* C, hipStream_t s) {
+  if (N >= 1 && N <= kGMaxN) {
+    // one block per run of rows, ~4 blocks per CU; partials summed in order
+    long long nblk = 1024;
+    long long rows = (m + nblk - 1) / nblk;
+    rows = ((rows + kGKC - 1) / kGKC) * kGKC;
+    if (rows < kGKC) rows = kGKC;
+    nblk = (m + rows - 1) / rows;
+    if (nblk < 1) nblk = 1;
+    DBuf<double> part((size_t)nblk * N * N);
+    const bool sym = A == B;
+    if (sym)
+      hipLaunchKernelGGL(k_gram_mfma<true>, dim3((unsigned)nblk), dim3(kGWaves * 64), 0, s, A, B, N, m, rows,
+                         part.get());
+    else
+      hipLaunchKernelGGL(k_gram_mfma<false>, dim3((unsigned)nblk), dim3(kGWaves * 64), 0, s, A, B, N, m, rows,
+                         part.get());
+    hipLaunchKernelGGL(k_gram_mfma_finish, dim3(nblocks((long long)N * N)), dim3(kB), 0, s, part.get(), (int)nblk, N,
+                       sym, C);
+    SWFWI_HIP(hipStreamSynchronize(s));
+    return;
+  }
   const int nt = (N + kGT - 1) / kGT;
   long long nchunk = 64;
   long long chunk = (m + nchunk - 1) / nchunk;

@@ -108,3 +108,27 @@ def models(case, sigma=None):
     v = layered_model(case.nz, case.nx, case.vmin, case.vmax, case.seed)
     s = sigma if sigma is not None else (20.0 if case.nx >= 200 else 4.0)
     return v, smoothed(v, s)
+
+
+def ensemble_members(case, v0, N, seed=2, corr=20.0, std=0.03):
+    """Ensemble-Kalman FWI members: v0 times (1 + a seeded Gaussian random
+    field of correlation length `corr` cells and standard deviation `std`),
+    expanded to the padded grid in transformed units, [N][nxpad][nzpad]."""
+    from scipy.ndimage import gaussian_filter
+
+    from . import capi
+
+    rng = np.random.default_rng(seed)
+    mem = []
+    for _ in range(N):
+        f = gaussian_filter(rng.standard_normal(v0.shape), corr)
+        f = f / f.std() * std
+        mem.append(capi.expand_domain((v0 * (1 + f)).astype(np.float32), case.nb, case.dx, case.dt))
+    return np.stack(mem)
+
+
+def add_noise(dobs, frac=0.01, seed=2):
+    """Observed data plus seeded Gaussian noise at `frac` of its RMS amplitude."""
+    rng = np.random.default_rng(seed)
+    rms = float(np.sqrt(np.mean(dobs.astype(np.float64) ** 2)))
+    return (dobs + rng.standard_normal(dobs.shape) * frac * rms).astype(np.float32)
