@@ -31,10 +31,20 @@ extern "C" {
 #define SWFWI_PREC_EXACT 0 /* the reference's FP32/FP64 expression trees, bit-for-bit */
 #define SWFWI_PREC_FP32 1  /* pure FP32 with fused multiply-adds */
 
+#define SWFWI_SNAP_F32 0  /* wavefield checkpoints stored in FP32 */
+#define SWFWI_SNAP_BF16 1 /* ... in BF16 with a per-snapshot max-abs scale */
+#define SWFWI_SNAP_F16 2  /* ... in FP16 with a per-snapshot max-abs scale */
+#define SWFWI_SNAP_NONE 3 /* no checkpoints (forward runs only) */
+
+#define SWFWI_ORDER_10 10 /* the reference's damped 10th-order-in-space, 4th-order-in-time step */
+#define SWFWI_ORDER_4 4   /* damped 4th-order-in-space, 2nd-order-in-time step for coarse grids */
+
 typedef struct swfwi_solver_s *swfwi_solver;       /* Damp4t10d: geometry + bound model */
 typedef struct swfwi_inversion_s *swfwi_inversion; /* FwiFramework/EssFwiFramework state */
 typedef struct swfwi_enkf_s *swfwi_enkf;           /* EnkfAnalyze state */
 typedef struct swfwi_enfwi_s *swfwi_enfwi;         /* ensemble of EssFwiFramework drivers */
+typedef struct swfwi_batch_s *swfwi_batch;         /* mixed-size batch with reduced-precision checkpoints */
+typedef struct swfwi_net_s *swfwi_net;             /* learned coarse-grid correction network */
 
 const char *swfwi_last_error(void);
 const char *swfwi_version(void);
@@ -200,6 +210,87 @@ int swfwi_enfwi_mean_objective(swfwi_enfwi f, float *mean, float *obj);
 int swfwi_enfwi_stats(swfwi_enfwi f, float *out, double *ms, int *counts);
 /* scaled + masked gradients of the last epoch, [N][nxpad][nzpad] */
 int swfwi_enfwi_gradients(swfwi_enfwi f, float *grads);
+
+/* ---- mixed-size batched gradients with reduced-precision checkpoints ----- */
+/* No single counterpart in the reference: calgradient (fwiframework.cpp:
+ * 114-177) for many shots of many models on grids of different sizes, all
+ * advancing in one launch per time step, with the source field kept as
+ * checkpoints every 4 steps (in place of the boundary band of
+ * damp4t10d.cpp:447-536 and the reconstruction it drives) and the imaging
+ * condition formed on the checkpoint steps with weight 4 w(it).  The stencil
+ * computes in FP32.
+ * Group g: dims[9g..9g+8] = nx, nz, nb, nt, nshots, gx0, ng, gz, nmodels
+ * (receivers at physical x = gx0 .. gx0+ng-1, depth gz); params[3g..3g+2] =
+ * dx, dt, fm (Ricker wavelet of amplitude 1); sx/sz: the shots' physical
+ * positions, groups concatenated.  Entries (group, model, shot) are ordered
+ * group by group, model by model, shot by shot in every array below. */
+int swfwi_batch_create(swfwi_batch *out, int ngroups, const int *dims, const float *params, const int *sx,
+                       const int *sz, int snap_prec, int order);
+int swfwi_batch_destroy(swfwi_batch b);
+/* BF16 <-> FP16 at any time; FP32 only for a batch created in FP32 */
+int swfwi_batch_set_snapshot_precision(swfwi_batch b, int snap_prec);
+/* Capture each time loop into a HIP graph on first use (on by default). */
+int swfwi_batch_set_graphs(swfwi_batch b, int on);
+/* model `model` of group `group`: padded transformed velocity (Damp4t10d) */
+int swfwi_batch_bind_model(swfwi_batch b, int group, int model, const float *vel_t);
+/* Forward run of every entry: seismograms into dcal[entry][nt][ng] (may be
+ * null) and the checkpoints kept on the device. */
+int swfwi_batch_forward(swfwi_batch b, float *dcal);
+/* Adjoint sources vsrc = transVsrc(dobs - dcal) per entry on the device
+ * (fwiframework.cpp:75-105,293-302 without the mute), from the last forward
+ * run; vsrc (may be null) receives them. */
+int swfwi_batch_adjoint_source(swfwi_batch b, const float *dobs, float *vsrc);
+/* Backward run from the adjoint sources vsrc[entry][nt][ng] (null: the
+ * ones swfwi_batch_adjoint_source left on the device): raw imaging condition
+ * per entry into grads[entry][nxpad][nzpad] and per model summed over its
+ * shots into model_grads[model][nxpad][nzpad] (either may be null). */
+int swfwi_batch_backward(swfwi_batch b, const float *vsrc, float *grads, float *model_grads);
+/* out[6] = entries, forward and backward point updates per run, checkpoint
+ * bytes, milliseconds of the last forward and backward run. */
+int swfwi_batch_info(swfwi_batch b, double *out);
+/* Field taps of the forward runs: at each of the n listed steps, the
+ * restriction of every entry's current field to the 2x-coarser grid is kept:
+ * physical fine cell (2i, 2k + oz) -> coarse cell (i, k) of the grid of
+ * nx' = (nx+1)/2, nz' = (nz+1-oz)/2, nb' = nb/2 (nb must be even; oz = 1
+ * puts both grids' zero top rows at the same depth); swfwi_batch_taps copies
+ * them out as [entry][n][nxpad'][nzpad']. */
+int swfwi_batch_set_taps(swfwi_batch b, int n, const int *steps, int oz);
+int swfwi_batch_taps(swfwi_batch b, float *out);
+/* Steps it0 .. it0+nsteps-1 of every entry from the state prev (field at
+ * it0-1) and cur (at it0), [entry][nxpad][nzpad] each; the state after them
+ * (prev_out at it0+nsteps-1, cur_out at it0+nsteps; either may be null). */
+int swfwi_batch_step_from(swfwi_batch b, const float *prev, const float *cur, int it0, int nsteps,
+                          float *prev_out, float *cur_out);
+/* The learned correction p <- p + net(p) on both time levels of every entry
+ * every 4 forward steps, over the physical interior (null: off).
+ * Single-group batches. */
+int swfwi_batch_set_correction(swfwi_batch b, swfwi_net net);
+
+/* ---- learned coarse-grid correction network --------------------------------
+ * No counterpart in the reference.  A 3x3 convolution network 1 -> 16 -> 16
+ * -> 1 with ReLU between and no biases, y = net(p) added to the field p.
+ * Weights (2592 floats): W1[t][c] (9 x 16), W2[t][ci][co] (9 x 16 x 16),
+ * W3[t][ci] (9 x 16), tap t = (dx + 1) * 3 + (dz + 1) for the neighbour
+ * (x + dx, z + dz).  The 16 -> 16 layer and the weight-gradient products run
+ * on BF16 MFMA with FP32 accumulation.  Field stacks are [entries][nx][nz]
+ * (z contiguous); region[4] = x0, x1, z0, z1: the outputs and the loss cover
+ * [x0, x1) x [z0, z1), which must lie 5 cells inside the array. */
+int swfwi_net_create(swfwi_net *out);
+int swfwi_net_destroy(swfwi_net n);
+int swfwi_net_set_weights(swfwi_net n, const float *w);
+int swfwi_net_get_weights(swfwi_net n, float *w);
+/* y = net(fields) over the region (zero elsewhere) */
+int swfwi_net_apply(swfwi_net n, const float *fields, int nx, int nz, int entries, const int *region, float *y);
+/* loss = mean over the region of (in + net(in) - target)^2 and its gradient
+ * with respect to the weights (grad may be null) */
+int swfwi_net_loss_grad(swfwi_net n, const float *in, const float *target, int nx, int nz, int entries,
+                        const int *region, float *grad, double *loss);
+/* Training set kept on the device: nsamples inputs and targets [n][nx][nz]. */
+int swfwi_net_set_data(swfwi_net n, const float *in, const float *target, int nx, int nz, int nsamples,
+                       const int *region);
+/* `steps` Adam steps (beta 0.9 / 0.999) on minibatches of `batch` samples in a
+ * seeded shuffled order; losses[steps] (may be null) = loss before each step. */
+int swfwi_net_train(swfwi_net n, int batch, int steps, float lr, unsigned long long seed, double *losses);
 
 #ifdef __cplusplus
 }

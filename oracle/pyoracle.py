@@ -75,6 +75,8 @@ def lib():
             "orc_ess_copy_grad": (None, [VP, f32p]),
             "orc_ess_copy_dir": (None, [VP, f32p]),
             "orc_set_objective_hook": (None, [OBJ_HOOK, VP]),
+            "o64_forward_modeling": (None, [I, I, I, I, F, f32p, f64p, I, I, I, i32p, i32p, f64p]),
+            "o64_calgradient": (None, [I, I, I, I, F, f32p, f64p, I, I, I, i32p, i32p, f64p, f64p]),
             "orc_steplen_scripted": (None, [F, F, I, f32p, f32p]),
         }
         for name, (res, args) in sig.items():
@@ -214,6 +216,22 @@ class Geom:
         g = self.zeros() if g0 is None else g0
         lib().orc_calgradient(self.h, vel, np.ascontiguousarray(encsrc, np.float32), shot,
                               np.ascontiguousarray(vsrc, np.float32), g)
+        return g
+
+    def forward_modeling64(self, vel, src, shot):
+        """FP64 run of FwiForwardModeling for source `shot` (amplitudes src[nt])."""
+        dcal = np.zeros((self.nt, self.ng), np.float64)
+        lib().o64_forward_modeling(self.nxpad, self.nzpad, self.nb, self.nt, self.dt, np.ascontiguousarray(vel),
+                                   np.ascontiguousarray(src, np.float64), int(self.sx[shot]), int(self.sz[shot]),
+                                   self.ng, self.gx, self.gz, dcal)
+        return dcal
+
+    def calgradient64(self, vel, src, vsrc, shot):
+        """FP64 run of calgradient for source `shot`: raw imaging condition."""
+        g = np.zeros(self.shape, np.float64)
+        lib().o64_calgradient(self.nxpad, self.nzpad, self.nb, self.nt, self.dt, np.ascontiguousarray(vel),
+                              np.ascontiguousarray(src, np.float64), int(self.sx[shot]), int(self.sz[shot]), self.ng,
+                              self.gx, self.gz, np.ascontiguousarray(vsrc, np.float64), g)
         return g
 
     def mute_windows(self, vel, src=0):

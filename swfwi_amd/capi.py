@@ -13,6 +13,9 @@ LIB_PATH = os.path.join(HERE, "libswfwi_hip.so")
 PREC_EXACT = 0
 PREC_FP32 = 1
 
+SNAP_F32, SNAP_BF16, SNAP_F16, SNAP_NONE = 0, 1, 2, 3
+ORDER_10, ORDER_4 = 10, 4
+
 # status codes (include/swfwi_hip.h)
 OK, ERR_ARG, ERR_STATE, ERR_HIP, ERR_NUMERIC, ERR_RANGE = 0, 1, 2, 3, 4, 5
 
@@ -69,6 +72,27 @@ SIGNATURES = {
     "swfwi_enfwi_mean_objective": (_I, [_VP, C.c_void_p, C.POINTER(_F)]),
     "swfwi_enfwi_stats": (_I, [_VP, _f32, _f64, _i32]),
     "swfwi_enfwi_gradients": (_I, [_VP, _f32]),
+    "swfwi_batch_create": (_I, [C.POINTER(_VP), _I, _i32, _f32, _i32, _i32, _I, _I]),
+    "swfwi_batch_set_taps": (_I, [_VP, _I, _i32, _I]),
+    "swfwi_batch_taps": (_I, [_VP, _f32]),
+    "swfwi_batch_step_from": (_I, [_VP, _f32, _f32, _I, _I, C.c_void_p, C.c_void_p]),
+    "swfwi_batch_set_correction": (_I, [_VP, _VP]),
+    "swfwi_net_create": (_I, [C.POINTER(_VP)]),
+    "swfwi_net_destroy": (_I, [_VP]),
+    "swfwi_net_set_weights": (_I, [_VP, _f32]),
+    "swfwi_net_get_weights": (_I, [_VP, _f32]),
+    "swfwi_net_apply": (_I, [_VP, _f32, _I, _I, _I, _i32, _f32]),
+    "swfwi_net_loss_grad": (_I, [_VP, _f32, _f32, _I, _I, _I, _i32, C.c_void_p, C.POINTER(C.c_double)]),
+    "swfwi_net_set_data": (_I, [_VP, _f32, _f32, _I, _I, _I, _i32]),
+    "swfwi_net_train": (_I, [_VP, _I, _I, _F, C.c_ulonglong, C.c_void_p]),
+    "swfwi_batch_destroy": (_I, [_VP]),
+    "swfwi_batch_set_snapshot_precision": (_I, [_VP, _I]),
+    "swfwi_batch_set_graphs": (_I, [_VP, _I]),
+    "swfwi_batch_bind_model": (_I, [_VP, _I, _I, _f32]),
+    "swfwi_batch_forward": (_I, [_VP, C.c_void_p]),
+    "swfwi_batch_adjoint_source": (_I, [_VP, _f32, C.c_void_p]),
+    "swfwi_batch_backward": (_I, [_VP, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "swfwi_batch_info": (_I, [_VP, _f64]),
     "swfwi_inversion_create": (_I, [C.POINTER(_VP), _VP, _f32, _f32, _F, _F, _I, _F]),
     "swfwi_inversion_destroy": (_I, [_VP]),
     "swfwi_ess_epoch": (_I, [_VP, _I]),

@@ -1,0 +1,278 @@
+// Step kernels of the mixed-size batch (batch.h): one launch advances every
+// active entry by one time step, each block one 64 (z) x 16 (x) tile of one
+// entry.  Per block: stage the entry's current field with a 6-cell halo in
+// LDS, form the 10th-order Laplacian u2 in LDS, then the damped leapfrog
+// update of fd4t10s-damp-zjh.c:14-115 in FP32 (fused multiply-adds).
+//
+// Forward (FWD): the next step's source amplitude is added to `next`
+// (addSource before stepForward, damp4t10d.cpp:272-289), receivers are
+// recorded from the staged current field (recordSeis after the swap), the
+// current field is stored as a snapshot on steps 4k+3 with the max-abs scale
+// that the previous launch accumulated, and on steps before a snapshot the
+// block's max |next| is folded into the next snapshot's scale.
+//
+// Backward: the receiver field of calgradient (fwiframework.cpp:155-176):
+// the next iteration's adjoint source is added to `next` at the receivers,
+// and on iterations 4k+4 the imaging condition g -= 4 w(it) s r is formed
+// with the source field C[it-1] read back from snapshot k.
+#include <hip/hip_runtime.h>
+
+#include "batch.h"
+#include "kernels.h"
+
+namespace swfwi {
+namespace {
+
+constexpr int kBT = 256;
+constexpr int kBZ = 64, kBX = 16, kBH = 6;
+constexpr int kSZ = kBZ + 2 * kBH;  // staged z
+constexpr int kSX = kBX + 2 * kBH;  // staged x
+constexpr int kUZ = kBZ + 2;        // u2 z (outputs +-1)
+constexpr int kUX = kBX + 2;
+constexpr float kF16Scale = 16384.f;  // snapshot max maps to 2^14 in FP16
+
+// Zhang Jinhai coefficients as the reference stores them (fd4t10s-damp-zjh.c:24-29)
+constexpr float kA0 = 1.53400796f, kA1 = 1.78858721f, kA2 = -0.31660756f, kA3 = 0.07612173f, kA4 = -0.01626042f,
+                kA5 = 0.00216736f;
+
+__device__ __forceinline__ float damping(int ix, int iz, int nx, int nz, int nb) {
+  // fd4t10s-damp-zjh.c:64-80: left, right, then bottom (wins in the corners)
+  float dist = 0.f;
+  if (ix < nb) dist = (float)(nb - ix) / nb;
+  if (ix >= nx - nb) dist = (float)(ix - (nx - nb) + 1) / nb;
+  if (iz >= nz - nb) dist = (float)(iz - (nz - nb) + 1) / nb;
+  return 0.05f * dist * dist;
+}
+
+__device__ __forceinline__ unsigned short to_bf16(float x) {  // round to nearest even
+  unsigned u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float from_bf16(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
+template <int PREC>
+__device__ __forceinline__ void snap_store(void* base, long long i, float v, float scale) {
+  if constexpr (PREC == kSnapF32) {
+    static_cast<float*>(base)[i] = v;
+  } else if constexpr (PREC == kSnapBF16) {
+    const float inv = scale > 0.f ? 1.f / scale : 0.f;
+    static_cast<unsigned short*>(base)[i] = to_bf16(v * inv);
+  } else {
+    const float inv = scale > 0.f ? kF16Scale / scale : 0.f;
+    static_cast<_Float16*>(base)[i] = (_Float16)(v * inv);
+  }
+}
+template <int PREC>
+__device__ __forceinline__ float snap_load(const void* base, long long i, float scale) {
+  if constexpr (PREC == kSnapF32) {
+    return static_cast<const float*>(base)[i];
+  } else if constexpr (PREC == kSnapBF16) {
+    return from_bf16(static_cast<const unsigned short*>(base)[i]) * scale;
+  } else {
+    return (float)static_cast<const _Float16*>(base)[i] * (scale * (1.f / kF16Scale));
+  }
+}
+
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+template <int PREC, bool FWD, int ORDER>
+__global__ __launch_bounds__(kBT) void k_batch_step(BatchKernelArgs a) {
+  __shared__ float sc[kSX * kSZ];
+  __shared__ float su[kUX * kUZ];
+  __shared__ float sm[kBT / 64];
+  // XCD-contiguous runs of the work list: blocks are dealt round-robin to the
+  // 8 XCDs, so consecutive tiles of one entry share an XCD's L2
+  const int n = a.nblocks, per = n / 8, rem = n % 8;
+  const int xcd = blockIdx.x % 8, local = blockIdx.x / 8;
+  const int l = xcd * per + (xcd < rem ? xcd : rem) + local;
+  const int e = a.blk_entry[l];
+  const BatchEntry E = a.ent[e];
+  const int tile = l - a.tile0[e];
+  const int it = FWD ? a.it_or_k : E.nt - 1 - a.it_or_k;
+  const int tx = tile / E.ntz, tz = tile - tx * E.ntz;
+  const int ox = kBH + tx * kBX, oz = kBH + tz * kBZ;
+  const int ld = E.nzpad;
+  const long long npts = (long long)E.nxpad * ld;
+  const bool even = (a.rot & 1) == 0;
+  const float* cur = (even ? a.fb : a.fa) + E.f;
+  float* nxt = (even ? a.fa : a.fb) + E.f;  // holds prev; next is written over it
+
+  const bool inside = ox - kBH + kSX <= E.nxpad && oz - kBH + kSZ <= E.nzpad;
+  for (int q = threadIdx.x; q < kSX * kSZ; q += kBT) {
+    const int xx = q / kSZ, zz = q - xx * kSZ;
+    const int gx = ox - kBH + xx, gz = oz - kBH + zz;
+    sc[q] = (inside || (gx < E.nxpad && gz < E.nzpad)) ? cur[gx * ld + gz] : 0.f;
+  }
+  __syncthreads();
+  for (int q = threadIdx.x; q < kUX * kUZ; q += kBT) {
+    const int xx = q / kUZ, zz = q - xx * kUZ;
+    const int c = (xx + kBH - 1) * kSZ + zz + kBH - 1;
+    float acc;
+    if constexpr (ORDER == kOrder10) {
+      acc = -4.f * kA0 * sc[c];
+      acc = fmaf(kA1, (sc[c - 1] + sc[c + 1]) + (sc[c - kSZ] + sc[c + kSZ]), acc);
+      acc = fmaf(kA2, (sc[c - 2] + sc[c + 2]) + (sc[c - 2 * kSZ] + sc[c + 2 * kSZ]), acc);
+      acc = fmaf(kA3, (sc[c - 3] + sc[c + 3]) + (sc[c - 3 * kSZ] + sc[c + 3 * kSZ]), acc);
+      acc = fmaf(kA4, (sc[c - 4] + sc[c + 4]) + (sc[c - 4 * kSZ] + sc[c + 4 * kSZ]), acc);
+      acc = fmaf(kA5, (sc[c - 5] + sc[c + 5]) + (sc[c - 5 * kSZ] + sc[c + 5 * kSZ]), acc);
+    } else {  // 4th-order Laplacian: -5 p + 4/3 (ring 1) - 1/12 (ring 2)
+      acc = -5.f * sc[c];
+      acc = fmaf(4.f / 3.f, (sc[c - 1] + sc[c + 1]) + (sc[c - kSZ] + sc[c + kSZ]), acc);
+      acc = fmaf(-1.f / 12.f, (sc[c - 2] + sc[c + 2]) + (sc[c - 2 * kSZ] + sc[c + 2 * kSZ]), acc);
+    }
+    su[q] = acc;
+  }
+  __syncthreads();
+
+  const int zl = threadIdx.x & (kBZ - 1), xg = threadIdx.x / kBZ;  // 4 consecutive x per thread
+  const int gz = oz + zl;
+  const int rz = E.rec0 % ld, rx0 = E.rec0 / ld;
+  const bool snap_now = FWD && PREC != kSnapNone && (it % kSnapEvery == kSnapEvery - 1);
+  const bool max_now = FWD && ((it + 1) % kSnapEvery == kSnapEvery - 1) && (it + 1) / kSnapEvery < E.nsnap;
+  const bool image_now =
+      !FWD && PREC != kSnapNone && it >= kSnapEvery && (it % kSnapEvery == 0) && a.wts[E.wts0 + it] != 0.f;
+  const bool inject_next = FWD ? it + 1 < E.nt : it - 1 >= E.it_last;
+  float scale = 0.f;
+  if (snap_now) scale = __uint_as_float(a.scale[E.scale0 + it / kSnapEvery]);
+  if (image_now) scale = __uint_as_float(a.scale[E.scale0 + it / kSnapEvery - 1]);
+  const float wimg = image_now ? a.wts[E.wts0 + it] : 0.f;
+  const long long snap_off = E.snap + (long long)(FWD ? it / kSnapEvery : it / kSnapEvery - 1) * npts;
+  float vmax = 0.f;
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const int xl = xg * 4 + j;
+    const int gx = ox + xl;
+    if (gx >= E.nxpad - kBH || gz >= E.nzpad - kBH) continue;
+    const int p = gx * ld + gz;
+    const int uc = (xl + 1) * kUZ + zl + 1;
+    const float u = su[uc];
+    const float r = a.rv[E.model + p];
+    const float c = sc[(xl + kBH) * kSZ + zl + kBH];
+    const float d = damping(gx, gz, E.nxpad, E.nzpad, E.damp);
+    const float pv = nxt[p];
+    float v = fmaf(2.f - 2.f * d + d * d, c, -(1.f - 2.f * d) * pv);
+    v = fmaf(r, u, v);
+    if constexpr (ORDER == kOrder10) {  // the 4th-order-in-time term r^2/12 Lap5(u2)
+      const float lap = (su[uc - 1] + su[uc + 1]) + (su[uc - kUZ] + su[uc + kUZ]) - 4.f * u;
+      v = fmaf(r * r * (1.f / 12.f), lap, v);
+    }
+    const bool rec = gz == rz && gx >= rx0 && gx < rx0 + E.ng;
+    if constexpr (FWD) {
+      if (inject_next && p == E.src) v += a.amp[E.amp0 + it + 1];
+      vmax = fmaxf(vmax, fabsf(v));
+      if (rec && a.dcal) a.dcal[E.dcal + (long long)it * E.ng + (gx - rx0)] = c;
+      if constexpr (PREC != kSnapNone) {
+        if (snap_now) snap_store<PREC>(a.snap, snap_off + p, c, scale);
+      }
+      if (a.tap_k >= 0 && ((gx | (gz - E.tap_oz)) & 1) == 0) {  // a coarse point
+        const int xc = (gx + kBH) >> 1, zc = (gz + kBH - E.tap_oz) >> 1;
+        if (xc < E.tap_nx && zc < E.tap_nz)
+          a.tap[E.tap + (long long)a.tap_k * E.tap_nx * E.tap_nz + xc * E.tap_nz + zc] = c;
+      }
+    } else {
+      if (inject_next && rec) v += a.dcal[E.dcal + (long long)(it - 1) * E.ng + (gx - rx0)];
+      if (image_now && PREC != kSnapNone) {
+        const float s = snap_load<PREC == kSnapNone ? kSnapF32 : PREC>(a.snap, snap_off + p, scale);
+        float* g = a.grad + E.grad + p;
+        *g = *g - wimg * s * c;
+      }
+    }
+    nxt[p] = v;
+  }
+  if (max_now) {  // the next step's field is snapshot (it+1)/4: fold its max
+    vmax = wave_max(vmax);
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = vmax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float m = sm[0];
+      for (int w = 1; w < kBT / 64; w++) m = fmaxf(m, sm[w]);
+      // non-negative floats order like their bit patterns
+      atomicMax(&a.scale[E.scale0 + (it + 1) / kSnapEvery], __float_as_uint(m));
+    }
+  }
+}
+
+// step-0 fields: both buffers zero (by the caller), then the first source
+// amplitude (forward) or the last adjoint-source sample (backward) in fb
+__global__ void k_batch_inject0(BatchKernelArgs a, int nentries, int fwd) {
+  const int e = blockIdx.x;
+  if (e >= nentries) return;
+  const BatchEntry E = a.ent[e];
+  if (fwd) {
+    if (threadIdx.x == 0) a.fb[E.f + E.src] = a.amp[E.amp0];
+  } else {
+    const int ld = E.nzpad;
+    for (int r = threadIdx.x; r < E.ng; r += blockDim.x)
+      a.fb[E.f + E.rec0 + (long long)r * ld] = a.dcal[E.dcal + (long long)(E.nt - 1) * E.ng + r];
+  }
+}
+
+__global__ void k_sub(const float* a, const float* b, float* out, long long n) {
+  for (long long i = blockIdx.x * (long long)kBT + threadIdx.x; i < n; i += (long long)gridDim.x * kBT)
+    out[i] = a[i] - b[i];
+}
+
+__global__ void k_fill_u32(unsigned* p, long long n, unsigned v) {
+  for (long long i = blockIdx.x * (long long)kBT + threadIdx.x; i < n; i += (long long)gridDim.x * kBT) p[i] = v;
+}
+
+}  // namespace
+
+void launch_fill_u32(unsigned* p, long long n, unsigned v, hipStream_t s) {
+  long long b = (n + kBT - 1) / kBT;
+  if (b > 4096) b = 4096;
+  if (b < 1) b = 1;
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)b), dim3(kBT), 0, s, p, n, v);
+}
+
+void launch_sub(const float* a, const float* b, float* out, long long n, hipStream_t s) {
+  long long blk = (n + kBT - 1) / kBT;
+  if (blk > 8192) blk = 8192;
+  if (blk < 1) blk = 1;
+  hipLaunchKernelGGL(k_sub, dim3((unsigned)blk), dim3(kBT), 0, s, a, b, out, n);
+}
+
+void launch_batch_init_fwd(const BatchKernelArgs& a, int nentries, hipStream_t s) {
+  hipLaunchKernelGGL(k_batch_inject0, dim3(nentries), dim3(64), 0, s, a, nentries, 1);
+}
+void launch_batch_init_bwd(const BatchKernelArgs& a, int nentries, hipStream_t s) {
+  hipLaunchKernelGGL(k_batch_inject0, dim3(nentries), dim3(kBT), 0, s, a, nentries, 0);
+}
+
+template <bool FWD, int ORDER>
+static void launch_step(const BatchKernelArgs& a, int prec, hipStream_t s) {
+  if (a.nblocks <= 0) return;
+  switch (prec) {
+    case kSnapF32:
+      hipLaunchKernelGGL((k_batch_step<kSnapF32, FWD, ORDER>), dim3(a.nblocks), dim3(kBT), 0, s, a);
+      break;
+    case kSnapBF16:
+      hipLaunchKernelGGL((k_batch_step<kSnapBF16, FWD, ORDER>), dim3(a.nblocks), dim3(kBT), 0, s, a);
+      break;
+    case kSnapF16:
+      hipLaunchKernelGGL((k_batch_step<kSnapF16, FWD, ORDER>), dim3(a.nblocks), dim3(kBT), 0, s, a);
+      break;
+    default:
+      hipLaunchKernelGGL((k_batch_step<kSnapNone, FWD, ORDER>), dim3(a.nblocks), dim3(kBT), 0, s, a);
+      break;
+  }
+}
+void launch_batch_fwd(const BatchKernelArgs& a, int prec, int order, hipStream_t s) {
+  if (order == kOrder4)
+    launch_step<true, kOrder4>(a, prec, s);
+  else
+    launch_step<true, kOrder10>(a, prec, s);
+}
+void launch_batch_bwd(const BatchKernelArgs& a, int prec, int order, hipStream_t s) {
+  if (order == kOrder4)
+    launch_step<false, kOrder4>(a, prec, s);
+  else
+    launch_step<false, kOrder10>(a, prec, s);
+}
+
+}  // namespace swfwi

@@ -9,7 +9,9 @@
 #include <string>
 #include <vector>
 
+#include "batch.h"
 #include "hostmath.h"
+#include "learned.h"
 #include "noalloc.h"
 #include "solver.h"
 
@@ -20,6 +22,13 @@ struct swfwi_solver_s {
 };
 struct swfwi_inversion_s {
   std::unique_ptr<Inversion> inv;
+};
+struct swfwi_batch_s {
+  std::unique_ptr<Batch> b;
+};
+struct swfwi_net_s {
+  std::unique_ptr<ConvNet> n;
+  std::unique_ptr<Stream> s;
 };
 
 namespace {
@@ -651,6 +660,232 @@ int swfwi_enfwi_gradients(swfwi_enfwi f, float* grads) {
     NEED(f);
     NEED(grads);
     f->f->download_grads(grads);
+  });
+}
+
+// ---- mixed-size batch with reduced-precision checkpoints ------------------
+
+int swfwi_batch_create(swfwi_batch* out, int ngroups, const int* dims, const float* params, const int* sx,
+                       const int* sz, int snap_prec, int order) {
+  return guarded([&] {
+    NEED(out);
+    NEED(dims);
+    NEED(params);
+    NEED(sx);
+    NEED(sz);
+    if (ngroups < 1) fail(kErrArg, "a batch needs at least one group");
+    std::vector<BatchGroup> gs((size_t)ngroups);
+    int o = 0;
+    for (int g = 0; g < ngroups; g++) {
+      const int* d = dims + 9 * g;
+      BatchGroup& G = gs[g];
+      G.nx = d[0];
+      G.nz = d[1];
+      G.nb = d[2];
+      G.nt = d[3];
+      const int ns = d[4];
+      G.gx0 = d[5];
+      G.ng = d[6];
+      G.gz = d[7];
+      G.nmodels = d[8];
+      G.dx = params[3 * g];
+      G.dt = params[3 * g + 1];
+      G.fm = params[3 * g + 2];
+      if (ns < 1) fail(kErrArg, "a group needs at least one shot");
+      G.sx.assign(sx + o, sx + o + ns);
+      G.sz.assign(sz + o, sz + o + ns);
+      o += ns;
+    }
+    *out = nullptr;
+    std::unique_ptr<swfwi_batch_s> h(new swfwi_batch_s);
+    h->b.reset(new Batch(gs, snap_prec, order));
+    *out = h.release();
+  });
+}
+
+int swfwi_batch_destroy(swfwi_batch b) {
+  return guarded([&] { delete b; });
+}
+
+int swfwi_batch_set_snapshot_precision(swfwi_batch b, int snap_prec) {
+  return guarded([&] {
+    NEED(b);
+    b->b->set_snap_prec(snap_prec);
+  });
+}
+
+int swfwi_batch_set_graphs(swfwi_batch b, int on) {
+  return guarded([&] {
+    NEED(b);
+    b->b->set_graphs(on != 0);
+  });
+}
+
+int swfwi_batch_bind_model(swfwi_batch b, int group, int model, const float* vel_t) {
+  return guarded([&] {
+    NEED(b);
+    NEED(vel_t);
+    b->b->bind_model(group, model, vel_t);
+  });
+}
+
+int swfwi_batch_forward(swfwi_batch b, float* dcal) {
+  return guarded([&] {
+    NEED(b);
+    b->b->forward(dcal);
+  });
+}
+
+int swfwi_batch_adjoint_source(swfwi_batch b, const float* dobs, float* vsrc) {
+  return guarded([&] {
+    NEED(b);
+    NEED(dobs);
+    b->b->adjoint_from_observed(dobs, vsrc);
+  });
+}
+
+int swfwi_batch_backward(swfwi_batch b, const float* vsrc, float* grads, float* model_grads) {
+  return guarded([&] {
+    NEED(b);
+    b->b->backward(vsrc, grads, model_grads);
+  });
+}
+
+int swfwi_batch_info(swfwi_batch b, double* out) {
+  return guarded([&] {
+    NEED(b);
+    NEED(out);
+    out[0] = b->b->entries();
+    out[1] = b->b->forward_points();
+    out[2] = b->b->backward_points();
+    out[3] = (double)b->b->snapshot_bytes();
+    out[4] = b->b->last_ms[0];
+    out[5] = b->b->last_ms[1];
+  });
+}
+
+int swfwi_batch_set_taps(swfwi_batch b, int n, const int* steps, int oz) {
+  return guarded([&] {
+    NEED(b);
+    if (n < 0 || (n > 0 && !steps)) fail(kErrArg, "bad tap list");
+    b->b->set_taps(n > 0 ? std::vector<int>(steps, steps + n) : std::vector<int>(), oz);
+  });
+}
+
+int swfwi_batch_taps(swfwi_batch b, float* out) {
+  return guarded([&] {
+    NEED(b);
+    NEED(out);
+    b->b->download_taps(out);
+  });
+}
+
+int swfwi_batch_step_from(swfwi_batch b, const float* prev, const float* cur, int it0, int nsteps, float* prev_out,
+                          float* cur_out) {
+  return guarded([&] {
+    NEED(b);
+    NEED(prev);
+    NEED(cur);
+    b->b->step_from(prev, cur, it0, nsteps, prev_out, cur_out);
+  });
+}
+
+int swfwi_batch_set_correction(swfwi_batch b, swfwi_net net) {
+  return guarded([&] {
+    NEED(b);
+    b->b->set_correction(net ? net->n.get() : nullptr);
+  });
+}
+
+// ---- learned coarse-grid correction network --------------------------------
+
+namespace {
+NetGeom net_geom(int nx, int nz, int entries, const int* region) {
+  NEED(region);
+  NetGeom g{nx, nz, (long long)nx * nz, entries, region[0], region[1], region[2], region[3]};
+  if (nx < 1 || nz < 1 || entries < 1) fail(kErrArg, "bad field stack");
+  return g;
+}
+}  // namespace
+
+int swfwi_net_create(swfwi_net* out) {
+  return guarded([&] {
+    NEED(out);
+    *out = nullptr;
+    std::unique_ptr<swfwi_net_s> h(new swfwi_net_s);
+    h->n.reset(new ConvNet());
+    h->s.reset(new Stream());
+    *out = h.release();
+  });
+}
+
+int swfwi_net_destroy(swfwi_net n) {
+  return guarded([&] { delete n; });
+}
+
+int swfwi_net_set_weights(swfwi_net n, const float* w) {
+  return guarded([&] {
+    NEED(n);
+    NEED(w);
+    n->n->set_weights(w);
+  });
+}
+
+int swfwi_net_get_weights(swfwi_net n, float* w) {
+  return guarded([&] {
+    NEED(n);
+    NEED(w);
+    n->n->get_weights(w);
+  });
+}
+
+int swfwi_net_apply(swfwi_net n, const float* fields, int nx, int nz, int entries, const int* region, float* y) {
+  return guarded([&] {
+    NEED(n);
+    NEED(fields);
+    NEED(y);
+    const NetGeom g = net_geom(nx, nz, entries, region);
+    hipStream_t s = n->s->get();
+    DBuf<float> p, out;
+    p.upload(fields, (size_t)g.stride * entries, s);
+    out.resize((size_t)g.stride * entries);
+    launch_fill(out.get(), g.stride * entries, 0.f, s);
+    net_apply(p.get(), out.get(), g, n->n->dev(), s);
+    out.download(y, (size_t)g.stride * entries, s);
+    SWFWI_HIP(hipStreamSynchronize(s));
+  });
+}
+
+int swfwi_net_loss_grad(swfwi_net n, const float* in, const float* target, int nx, int nz, int entries,
+                        const int* region, float* grad, double* loss) {
+  return guarded([&] {
+    NEED(n);
+    NEED(in);
+    NEED(target);
+    const NetGeom g = net_geom(nx, nz, entries, region);
+    hipStream_t s = n->s->get();
+    DBuf<float> p, t;
+    p.upload(in, (size_t)g.stride * entries, s);
+    t.upload(target, (size_t)g.stride * entries, s);
+    const double l = n->n->loss_grad(p.get(), t.get(), g, grad, s);
+    if (loss) *loss = l;
+  });
+}
+
+int swfwi_net_set_data(swfwi_net n, const float* in, const float* target, int nx, int nz, int nsamples,
+                       const int* region) {
+  return guarded([&] {
+    NEED(n);
+    NEED(in);
+    NEED(target);
+    n->n->set_data(in, target, nsamples, net_geom(nx, nz, nsamples, region));
+  });
+}
+
+int swfwi_net_train(swfwi_net n, int batch, int steps, float lr, unsigned long long seed, double* losses) {
+  return guarded([&] {
+    NEED(n);
+    n->n->train(batch, steps, lr, seed, losses, n->s->get());
   });
 }
 

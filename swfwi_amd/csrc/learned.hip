@@ -1,0 +1,505 @@
+// Kernels of the learned coarse-grid correction (learned.h).
+//
+// One block per 16 x 16 tile of output points (z fastest), 256 threads.
+// Forward, per tile: the field with a 3-cell halo (22 x 22) in LDS; layer 1
+// (1 -> 16, FP32 FMAs) over the tile + 2 (20 x 20), ReLU, stored as BF16;
+// layer 2 (16 -> 16) over the tile + 1 (18 x 18) as 9 tap products of
+// v_mfma_f32_16x16x16_bf16 per 16 points (A = 16 points x 16 input channels
+// of the tap's neighbours, B = that tap's 16 x 16 weights), ReLU, FP32 in LDS;
+// layer 3 (16 -> 1, FP32 FMAs) on the tile.
+//
+// v_mfma_f32_16x16x16_bf16 lane layout (measured on the device): lane l
+// supplies A[i = l%16][k = 4*(l/16) + v] and B[k = 4*(l/16) + v][j = l%16]
+// as its 4 BF16 items v, and holds D[i = 4*(l/16) + v][j = l%16] in
+// accumulator item v.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <utility>
+#include <vector>
+
+#include "learned.h"
+
+namespace swfwi {
+namespace {
+
+constexpr int kT = 256;
+constexpr int kTile = 16;
+constexpr int kP = kTile + 6;   // staged field (halo 3)
+constexpr int kH1 = kTile + 4;  // layer-1 region (halo 2)
+constexpr int kH2 = kTile + 2;  // layer-2 region (halo 1)
+constexpr int kNH1 = kH1 * kH1;  // 400
+constexpr int kNH2 = kH2 * kH2;  // 324
+constexpr int kGroups2 = (kNH2 + 15) / 16;  // 21 groups of 16 layer-2 points
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t bf16_of(float x) {  // round to nearest even
+  unsigned u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float f32_of(uint16_t h) { return __uint_as_float((unsigned)h << 16); }
+
+// accumulator item v of lane l holds D[row][col]
+__device__ __forceinline__ int d_row(int l, int v) { return 4 * (l >> 4) + v; }
+__device__ __forceinline__ int d_col(int l) { return l & 15; }
+
+struct TileMap {
+  int e, ox, oz;  // entry, tile origin (global padded coords)
+};
+
+// tiles over the region grown by `margin` cells on every side
+__device__ __forceinline__ TileMap tile_of(const NetGeom& g, int margin) {
+  const int tz = (g.z1 - g.z0 + 2 * margin + kTile - 1) / kTile, tx = (g.x1 - g.x0 + 2 * margin + kTile - 1) / kTile;
+  const int per = tz * tx;
+  TileMap m;
+  m.e = blockIdx.x / per;
+  const int t = blockIdx.x - m.e * per;
+  m.ox = g.x0 - margin + (t / tz) * kTile;
+  m.oz = g.z0 - margin + (t % tz) * kTile;
+  return m;
+}
+
+// Shared forward work of a tile: stage the field, layer 1 (BF16 out), layer
+// 2 on MFMA (FP32 out).  sp[kP*kP], h1[kNH1*16] (BF16), h2[kNH2*16].
+__device__ void forward_tile(const float* __restrict__ p, const NetGeom& g, const TileMap& m, const NetDev& w,
+                             float* sp, uint16_t* h1, float* h2) {
+  const int tid = threadIdx.x;
+  for (int q = tid; q < kP * kP; q += kT) {
+    const int x = m.ox - 3 + q / kP, z = m.oz - 3 + q % kP;
+    sp[q] = (x >= 0 && x < g.nx && z >= 0 && z < g.nz) ? p[(long long)x * g.nz + z] : 0.f;
+  }
+  __syncthreads();
+  // layer 1: W1[t][c] read through the scalar cache (wave-uniform indices)
+  for (int q = tid; q < kNH1; q += kT) {
+    const int x = q / kH1, z = q % kH1;  // region coords; staged = +1
+    float a[kNetC];
+#pragma unroll
+    for (int c = 0; c < kNetC; c++) a[c] = 0.f;
+#pragma unroll
+    for (int dx = -1; dx <= 1; dx++)
+#pragma unroll
+      for (int dz = -1; dz <= 1; dz++) {
+        const float v = sp[(x + 1 + dx) * kP + (z + 1 + dz)];
+        const int t = (dx + 1) * 3 + (dz + 1);
+#pragma unroll
+        for (int c = 0; c < kNetC; c++) a[c] = fmaf(w.w1[t * kNetC + c], v, a[c]);
+      }
+#pragma unroll
+    for (int c = 0; c < kNetC; c++) h1[q * kNetC + c] = bf16_of(fmaxf(a[c], 0.f));
+  }
+  __syncthreads();
+  // layer 2 on MFMA: wave w takes groups w, w+4, ...
+  const int lane = tid & 63, wave = tid >> 6;
+  bf16x4 b[9];
+#pragma unroll
+  for (int t = 0; t < 9; t++) b[t] = *reinterpret_cast<const bf16x4*>(w.w2f + (t * 64 + lane) * 4);
+  for (int grp = wave; grp < kGroups2; grp += kT / 64) {
+    const int qa = grp * 16 + (lane & 15);          // this lane's A row (a layer-2 point)
+    const int qs = qa < kNH2 ? qa : kNH2 - 1;       // rows past the region read a valid point
+    const int x = qs / kH2, z = qs % kH2;           // layer-2 coords; layer-1 = +1
+    const int kofs = 4 * (lane >> 4);
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int t = 0; t < 9; t++) {
+      const int dx = t / 3 - 1, dz = t % 3 - 1;
+      const int q1 = (x + 1 + dx) * kH1 + (z + 1 + dz);
+      const bf16x4 a = *reinterpret_cast<const bf16x4*>(h1 + q1 * kNetC + kofs);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[t], acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int q2 = grp * 16 + d_row(lane, v);
+      if (q2 < kNH2) h2[q2 * kNetC + d_col(lane)] = fmaxf(acc[v], 0.f);
+    }
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(kT) void k_net_apply(const float* __restrict__ p_all, float* __restrict__ y_all,
+                                                  NetGeom g, NetDev w) {
+  __shared__ float sp[kP * kP];
+  __shared__ uint16_t h1[kNH1 * kNetC];
+  __shared__ float h2[kNH2 * kNetC];
+  const TileMap m = tile_of(g, 0);
+  const float* p = p_all + (long long)m.e * g.stride;
+  forward_tile(p, g, m, w, sp, h1, h2);
+  const int x = threadIdx.x / kTile, z = threadIdx.x % kTile;
+  const int gx = m.ox + x, gz = m.oz + z;
+  if (gx >= g.x1 || gz >= g.z1) return;
+  float y = 0.f;
+#pragma unroll
+  for (int t = 0; t < 9; t++) {
+    const int dx = t / 3 - 1, dz = t % 3 - 1;
+    const float* hv = h2 + ((x + 1 + dx) * kH2 + (z + 1 + dz)) * kNetC;
+#pragma unroll
+    for (int c = 0; c < kNetC; c++) y = fmaf(w.w3[t * kNetC + c], hv[c], y);
+  }
+  y_all[(long long)m.e * g.stride + (long long)gx * g.nz + gz] = y;
+}
+
+__global__ void k_net_add(float* p_all, const float* y_all, NetGeom g) {
+  const long long per = (long long)(g.x1 - g.x0) * (g.z1 - g.z0);
+  const long long n = per * g.entries;
+  for (long long i = blockIdx.x * (long long)kT + threadIdx.x; i < n; i += (long long)gridDim.x * kT) {
+    const long long e = i / per, r = i - e * per;
+    const int x = g.x0 + (int)(r / (g.z1 - g.z0)), z = g.z0 + (int)(r % (g.z1 - g.z0));
+    const long long o = e * g.stride + (long long)x * g.nz + z;
+    p_all[o] += y_all[o];
+  }
+}
+
+__device__ __forceinline__ double block_sum_d(double v, double* red) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double s = 0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < kT / 64; i++) s += red[i];
+  return s;
+}
+
+// r = p + y - t over the region; dy = 2 r / count; per-block sum of r^2
+__global__ void k_net_loss(const float* p, const float* y, const float* t, float* dy, NetGeom g, double inv_count,
+                           double* parts) {
+  __shared__ double red[kT / 64];
+  const long long per = (long long)(g.x1 - g.x0) * (g.z1 - g.z0);
+  const long long n = per * g.entries;
+  double acc = 0;
+  for (long long i = blockIdx.x * (long long)kT + threadIdx.x; i < n; i += (long long)gridDim.x * kT) {
+    const long long e = i / per, r = i - e * per;
+    const int x = g.x0 + (int)(r / (g.z1 - g.z0)), z = g.z0 + (int)(r % (g.z1 - g.z0));
+    const long long o = e * g.stride + (long long)x * g.nz + z;
+    const float res = p[o] + y[o] - t[o];
+    dy[o] = (float)(2.0 * res * inv_count);
+    acc += (double)res * res;
+  }
+  const double s = block_sum_d(acc, red);
+  if (threadIdx.x == 0) parts[blockIdx.x] = s;
+}
+
+// d loss / d weights of one tile: recompute the forward pieces, then
+//   dW3[t][c] = sum_tile dy(x) h2[c](x + o_t)
+//   dz2[c](x') = relu'(z2) sum_t W3[t][c] dy(x' - o_t)        (tile + 1)
+//   dW2[t][ci][co] = sum_tile h1[ci](x + o_t) dz2[co](x)        (MFMA, K = tile points)
+//   dz1[ci](x) = relu'(z1) sum_t sum_co W2[t][ci][co] dz2[co](x - o_t)   (MFMA)
+//   dW1[t][c] = sum_tile p(x + o_t) dz1[c](x)
+// into parts[block][kNetParams].
+__global__ __launch_bounds__(kT) void k_net_backward(const float* __restrict__ p_all, const float* __restrict__ dy_all,
+                                                     NetGeom g, NetDev w, float* __restrict__ parts) {
+  __shared__ float sp[kP * kP];
+  __shared__ uint16_t h1[kNH1 * kNetC];
+  __shared__ float h2[kNH2 * kNetC];
+  __shared__ float sdy[kNH1];                   // dy over the tile + 2
+  __shared__ uint16_t dz2[kNH2 * kNetC];        // BF16 (MFMA operand)
+  __shared__ float dz1[kTile * kTile * kNetC];  // tile points
+  // tiles cover the region grown by 2: dz2 is non-zero up to 1 and dz1 up
+  // to 2 cells outside the region, and every point's terms belong to the
+  // one tile that owns it
+  const TileMap m = tile_of(g, 2);
+  const long long eo = (long long)m.e * g.stride;
+  forward_tile(p_all + eo, g, m, w, sp, h1, h2);
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int q = tid; q < kNH1; q += kT) {
+    const int x = m.ox - 2 + q / kH1, z = m.oz - 2 + q % kH1;
+    const bool in = x >= g.x0 && x < g.x1 && z >= g.z0 && z < g.z1;  // dy is zero outside the region
+    sdy[q] = in ? dy_all[eo + (long long)x * g.nz + z] : 0.f;
+  }
+  __syncthreads();
+  float* out = parts + (long long)blockIdx.x * kNetParams;
+  // dz2 over the tile + 1
+  for (int q = tid; q < kNH2; q += kT) {
+    const int x = q / kH2, z = q % kH2;  // layer-2 coords; sdy coords = +1
+#pragma unroll
+    for (int c = 0; c < kNetC; c++) {
+      float a = 0.f;
+#pragma unroll
+      for (int t = 0; t < 9; t++) {
+        const int dx = t / 3 - 1, dz = t % 3 - 1;
+        a = fmaf(w.w3[t * kNetC + c], sdy[(x + 1 - dx) * kH1 + (z + 1 - dz)], a);
+      }
+      dz2[q * kNetC + c] = bf16_of(h2[q * kNetC + c] > 0.f ? a : 0.f);
+    }
+  }
+  // dW3: 144 sums over the tile's 256 points, one per thread
+  if (tid < kNetW3) {
+    const int t = tid / kNetC, c = tid % kNetC;
+    const int dx = t / 3 - 1, dz = t % 3 - 1;
+    float a = 0.f;
+    for (int x = 0; x < kTile; x++)
+      for (int z = 0; z < kTile; z++)
+        a = fmaf(sdy[(x + 2) * kH1 + (z + 2)], h2[((x + 1 + dx) * kH2 + (z + 1 + dz)) * kNetC + c], a);
+    out[kNetW1 + kNetW2 + tid] = a;
+  }
+  __syncthreads();
+  // dW2 on MFMA: per tap, D[ci][co] = sum over the 256 tile points (16 K-steps)
+  for (int t = wave; t < 9; t += kT / 64) {
+    const int dx = t / 3 - 1, dz = t % 3 - 1;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int kc = 0; kc < kTile * kTile / 16; kc++) {
+      bf16x4 a, b;
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int pt = kc * 16 + 4 * (lane >> 4) + v;  // tile point of this K item
+        const int x = pt / kTile, z = pt % kTile;
+        a[v] = (short)h1[((x + 2 + dx) * kH1 + (z + 2 + dz)) * kNetC + (lane & 15)];
+        b[v] = (short)dz2[((x + 1) * kH2 + (z + 1)) * kNetC + (lane & 15)];
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int v = 0; v < 4; v++) out[kNetW1 + (t * kNetC + d_row(lane, v)) * kNetC + d_col(lane)] = acc[v];
+  }
+  // dz1 on MFMA: per group of 16 tile points, D[pt][ci] = sum_t A_t[pt][co] B_t[co][ci]
+  {
+    bf16x4 b[9];
+#pragma unroll
+    for (int t = 0; t < 9; t++) b[t] = *reinterpret_cast<const bf16x4*>(w.w2b + (t * 64 + lane) * 4);
+    for (int grp = wave; grp < kTile * kTile / 16; grp += kT / 64) {
+      const int pt = grp * 16 + (lane & 15);
+      const int x = pt / kTile, z = pt % kTile;
+      const int kofs = 4 * (lane >> 4);
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int t = 0; t < 9; t++) {
+        const int dx = t / 3 - 1, dz = t % 3 - 1;
+        const int q2 = (x + 1 - dx) * kH2 + (z + 1 - dz);
+        const bf16x4 a = *reinterpret_cast<const bf16x4*>(dz2 + q2 * kNetC + kofs);
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[t], acc, 0, 0, 0);
+      }
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int pr = grp * 16 + d_row(lane, v), c = d_col(lane);
+        const int xr = pr / kTile, zr = pr % kTile;
+        const bool on = f32_of(h1[((xr + 2) * kH1 + (zr + 2)) * kNetC + c]) > 0.f;
+        dz1[pr * kNetC + c] = on ? acc[v] : 0.f;
+      }
+    }
+  }
+  __syncthreads();
+  // dW1: 144 sums over the tile, one per thread
+  if (tid < kNetW1) {
+    const int t = tid / kNetC, c = tid % kNetC;
+    const int dx = t / 3 - 1, dz = t % 3 - 1;
+    float a = 0.f;
+    for (int x = 0; x < kTile; x++)
+      for (int z = 0; z < kTile; z++) a = fmaf(sp[(x + 3 + dx) * kP + (z + 3 + dz)], dz1[(x * kTile + z) * kNetC + c], a);
+    out[tid] = a;
+  }
+}
+
+// grad[j] = sum over blocks of parts[b][j], in block order (FP64)
+__global__ void k_net_reduce(const float* parts, long long nblocks, float* grad) {
+  const int j = blockIdx.x * kT + threadIdx.x;
+  if (j >= kNetParams) return;
+  double s = 0;
+  for (long long b = 0; b < nblocks; b++) s += parts[b * kNetParams + j];
+  grad[j] = (float)s;
+}
+
+__global__ void k_net_pack(const float* theta, float* w1, uint16_t* w2f, uint16_t* w2b, float* w3) {
+  const int i = blockIdx.x * kT + threadIdx.x;
+  if (i < kNetW1) w1[i] = theta[i];
+  if (i < kNetW3) w3[i] = theta[kNetW1 + kNetW2 + i];
+  if (i < 9 * 64 * 4) {
+    // item v of lane l for tap t: k = 4*(l/16) + v, n = l%16
+    const int t = i / 256, l = (i / 4) % 64, v = i % 4;
+    const int k = 4 * (l >> 4) + v, n = l & 15;
+    const float* W2 = theta + kNetW1;
+    w2f[i] = bf16_of(W2[(t * kNetC + k) * kNetC + n]);  // k = ci, n = co
+    w2b[i] = bf16_of(W2[(t * kNetC + n) * kNetC + k]);  // k = co, n = ci
+  }
+}
+
+__global__ void k_net_adam(float* theta, const float* grad, float* m, float* v, float c1, float c2, float lr,
+                           float b1, float b2, float eps) {
+  const int i = blockIdx.x * kT + threadIdx.x;
+  if (i >= kNetParams) return;
+  const float gr = grad[i];
+  const float mi = b1 * m[i] + (1.f - b1) * gr;
+  const float vi = b2 * v[i] + (1.f - b2) * gr * gr;
+  m[i] = mi;
+  v[i] = vi;
+  theta[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
+}
+
+__global__ void k_sum_parts(const double* parts, int n, double* out) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    double s = 0;
+    for (int i = 0; i < n; i++) s += parts[i];
+    *out = s;
+  }
+}
+
+long long tiles_of(const NetGeom& g, int margin) {
+  const long long tz = (g.z1 - g.z0 + 2 * margin + kTile - 1) / kTile,
+                  tx = (g.x1 - g.x0 + 2 * margin + kTile - 1) / kTile;
+  return tz * tx * g.entries;
+}
+
+// the staged field reaches 3 cells beyond the tiles, which reach `margin`
+// beyond the region
+void check_geom(const NetGeom& g, int margin) {
+  const int r = 3 + margin;
+  if (g.x0 < r || g.z0 < r || g.x1 > g.nx - r || g.z1 > g.nz - r || g.x1 <= g.x0 || g.z1 <= g.z0 || g.entries < 1)
+    fail(kErrArg, "network region must lie 5 cells inside the field");
+  if (tiles_of(g, margin) > 0x7fffffffLL) fail(kErrRange, "network launch too large");
+}
+
+}  // namespace
+
+void net_pack(const float* theta, float* w1, uint16_t* w2f, uint16_t* w2b, float* w3, hipStream_t s) {
+  hipLaunchKernelGGL(k_net_pack, dim3((9 * 64 * 4 + kT - 1) / kT), dim3(kT), 0, s, theta, w1, w2f, w2b, w3);
+}
+
+void net_apply(const float* p, float* y, const NetGeom& g, const NetDev& w, hipStream_t s) {
+  check_geom(g, 2);
+  hipLaunchKernelGGL(k_net_apply, dim3((unsigned)tiles_of(g, 0)), dim3(kT), 0, s, p, y, g, w);
+}
+
+void net_add(float* p, const float* y, const NetGeom& g, hipStream_t s) {
+  hipLaunchKernelGGL(k_net_add, dim3(2048), dim3(kT), 0, s, p, y, g);
+}
+
+void net_loss(const float* p, const float* y, const float* t, float* dy, const NetGeom& g, double* loss_parts,
+              int nparts, hipStream_t s) {
+  const double count = (double)(g.x1 - g.x0) * (g.z1 - g.z0) * g.entries;
+  hipLaunchKernelGGL(k_net_loss, dim3(nparts), dim3(kT), 0, s, p, y, t, dy, g, 1.0 / count, loss_parts);
+}
+
+long long net_backward_blocks(const NetGeom& g) { return tiles_of(g, 2); }
+
+void net_backward(const float* p, const float* dy, const NetGeom& g, const NetDev& w, float* parts, float* grad,
+                  hipStream_t s) {
+  check_geom(g, 2);
+  const long long nb = tiles_of(g, 2);
+  hipLaunchKernelGGL(k_net_backward, dim3((unsigned)nb), dim3(kT), 0, s, p, dy, g, w, parts);
+  hipLaunchKernelGGL(k_net_reduce, dim3((kNetParams + kT - 1) / kT), dim3(kT), 0, s, parts, nb, grad);
+}
+
+void net_adam(float* theta, const float* grad, float* m, float* v, int step, float lr, float b1, float b2, float eps,
+              hipStream_t s) {
+  const float c1 = 1.f / (1.f - std::pow(b1, (float)step)), c2 = 1.f / (1.f - std::pow(b2, (float)step));
+  hipLaunchKernelGGL(k_net_adam, dim3((kNetParams + kT - 1) / kT), dim3(kT), 0, s, theta, grad, m, v, c1, c2, lr, b1,
+                     b2, eps);
+}
+
+void net_reduce_parts(const double* parts, int nparts, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, s, parts, nparts, out);
+}
+
+// ---- host handle -------------------------------------------------------------
+
+ConvNet::ConvNet() {
+  theta_.resize(kNetParams);
+  m_.resize(kNetParams);
+  v_.resize(kNetParams);
+  grad_.resize(kNetParams);
+  w1_.resize(kNetW1);
+  w3_.resize(kNetW3);
+  w2f_.resize(9 * 64 * 4);
+  w2b_.resize(9 * 64 * 4);
+  lparts_.resize(1024);
+  lsum_.resize(1);
+}
+
+void ConvNet::repack(hipStream_t s) { net_pack(theta_.get(), w1_.get(), w2f_.get(), w2b_.get(), w3_.get(), s); }
+
+void ConvNet::set_weights(const float* host) {
+  SWFWI_HIP(hipMemcpy(theta_.get(), host, sizeof(float) * kNetParams, hipMemcpyHostToDevice));
+  SWFWI_HIP(hipMemset(m_.get(), 0, sizeof(float) * kNetParams));
+  SWFWI_HIP(hipMemset(v_.get(), 0, sizeof(float) * kNetParams));
+  step_ = 0;
+  repack(nullptr);
+  SWFWI_HIP(hipDeviceSynchronize());
+}
+
+void ConvNet::get_weights(float* host) const {
+  SWFWI_HIP(hipMemcpy(host, theta_.get(), sizeof(float) * kNetParams, hipMemcpyDeviceToHost));
+}
+
+void ConvNet::correct(float* p, const NetGeom& g, float* yscratch, hipStream_t s) const {
+  net_apply(p, yscratch, g, dev(), s);
+  net_add(p, yscratch, g, s);
+}
+
+double ConvNet::loss_grad(const float* in, const float* target, const NetGeom& g, float* grad_host, hipStream_t s) {
+  const size_t n = (size_t)g.stride * g.entries;
+  y_.resize(n);
+  dy_.resize(n);
+  parts_.resize((size_t)net_backward_blocks(g) * kNetParams);
+  net_apply(in, y_.get(), g, dev(), s);
+  const int np = 1024;
+  net_loss(in, y_.get(), target, dy_.get(), g, lparts_.get(), np, s);
+  net_reduce_parts(lparts_.get(), np, lsum_.get(), s);
+  net_backward(in, dy_.get(), g, dev(), parts_.get(), grad_.get(), s);
+  double loss = 0;
+  SWFWI_HIP(hipMemcpyAsync(&loss, lsum_.get(), sizeof(double), hipMemcpyDeviceToHost, s));
+  if (grad_host) SWFWI_HIP(hipMemcpyAsync(grad_host, grad_.get(), sizeof(float) * kNetParams, hipMemcpyDeviceToHost, s));
+  SWFWI_HIP(hipStreamSynchronize(s));
+  const double count = (double)(g.x1 - g.x0) * (g.z1 - g.z0) * g.entries;
+  return loss / count;
+}
+
+double ConvNet::train_step(const float* in, const float* target, const NetGeom& g, float lr, hipStream_t s) {
+  const double loss = loss_grad(in, target, g, nullptr, s);
+  step_++;
+  net_adam(theta_.get(), grad_.get(), m_.get(), v_.get(), step_, lr, 0.9f, 0.999f, 1e-8f, s);
+  repack(s);
+  return loss;
+}
+
+void ConvNet::set_data(const float* in_host, const float* target_host, int n, const NetGeom& g) {
+  if (n < 1) fail(kErrArg, "empty training set");
+  dg_ = g;
+  dg_.entries = 1;
+  dg_.stride = (long long)g.nx * g.nz;
+  dn_ = n;
+  const size_t len = (size_t)n * dg_.stride;
+  din_.resize(len);
+  dtgt_.resize(len);
+  SWFWI_HIP(hipMemcpy(din_.get(), in_host, sizeof(float) * len, hipMemcpyHostToDevice));
+  SWFWI_HIP(hipMemcpy(dtgt_.get(), target_host, sizeof(float) * len, hipMemcpyHostToDevice));
+}
+
+void ConvNet::train(int batch, int steps, float lr, unsigned long long seed, double* losses, hipStream_t s) {
+  if (!dn_) fail(kErrState, "no training set");
+  if (batch < 1 || batch > dn_) fail(kErrArg, "minibatch size out of range");
+  NetGeom g = dg_;
+  g.entries = batch;
+  bin_.resize((size_t)batch * g.stride);
+  btgt_.resize((size_t)batch * g.stride);
+  // seeded shuffle of the sample order (splitmix64 + Fisher-Yates), refreshed each pass
+  std::vector<int> order(dn_);
+  for (int i = 0; i < dn_; i++) order[i] = i;
+  unsigned long long st = seed;
+  auto next = [&]() {
+    st += 0x9E3779B97F4A7C15ull;
+    unsigned long long z = st;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  };
+  int pos = dn_;
+  for (int k = 0; k < steps; k++) {
+    for (int b = 0; b < batch; b++) {
+      if (pos == dn_) {
+        for (int i = dn_ - 1; i > 0; i--) std::swap(order[i], order[(int)(next() % (unsigned long long)(i + 1))]);
+        pos = 0;
+      }
+      const int smp = order[pos++];
+      SWFWI_HIP(hipMemcpyAsync(bin_.get() + (size_t)b * g.stride, din_.get() + (size_t)smp * g.stride,
+                               sizeof(float) * g.stride, hipMemcpyDeviceToDevice, s));
+      SWFWI_HIP(hipMemcpyAsync(btgt_.get() + (size_t)b * g.stride, dtgt_.get() + (size_t)smp * g.stride,
+                               sizeof(float) * g.stride, hipMemcpyDeviceToDevice, s));
+    }
+    const double l = train_step(bin_.get(), btgt_.get(), g, lr, s);
+    if (losses) losses[k] = l;
+  }
+  SWFWI_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace swfwi

@@ -1,0 +1,273 @@
+"""Learned coarse-grid correction: data generation from the fine propagator,
+training on the GPU, and the coarse propagator with the correction.
+
+The fine grid (10th-order stencil, dx, dt) and the coarse grid (4th-order
+stencil, 2 dx, 2 dt, half the sponge) share the physical model: coarse cell
+(i, k) is fine cell (2i, 2k).  Training pairs come from fine runs of random
+layered models: the coarse leapfrog is started from the fine wavefield at
+coarse steps j-1, j (fine steps 2j-2, 2j), run 4 coarse steps, and the
+network learns to map the result at coarse steps j+3, j+4 onto the fine
+wavefield there (fine steps 2j+6, 2j+8).  The coarse forward runs apply the
+correction every 4 steps; the fine propagator judges the result."""
+import numpy as np
+
+from . import capi, workloads
+from .batch import Batch, Group, Net
+
+
+def small_case():
+    """A test-sized fine grid: 161 x 80 cells of 10 m, nb=20, dt=1 ms,
+    nt=1200, 8 Hz Ricker at the centre (depth 3 cells), receivers on row 1;
+    velocities 1500-3500 m/s."""
+    return dict(nx=161, nz=80, nb=20, dx=10.0, dt=0.001, fm=8.0, nt=1200, vmin=1500.0, vmax=3500.0)
+
+
+def config4_case():
+    """The 501 x 1501 fine grid at 12.5 m (nb=30, dt=1 ms, 8 Hz, nt=3000)
+    and its 250 x 751 coarse grid at 25 m (nb=15, dt=2 ms)."""
+    return dict(nx=1501, nz=501, nb=30, dx=12.5, dt=0.001, fm=8.0, nt=3000, vmin=1500.0, vmax=4500.0)
+
+
+# Coarse cell (i, k) is fine cell (2i, 2k + 1): the zero rows above the
+# first row of either grid (the free surface of the reference's scheme) then
+# lie at the same depth, two fine cells above fine row 1.
+OZ = 1
+
+
+def grids(case, shots=None):
+    """(fine, coarse) Group keyword dicts of one shot per model.  shots: fine
+    physical source x positions (even; default the centre); the sources sit
+    on fine row 3 (coarse row 1), the receivers on fine row 1 (coarse row 0)
+    at every coarse column."""
+    nx, nz, nb = case["nx"], case["nz"], case["nb"]
+    if nb % 2 or nx % 2 == 0:
+        raise ValueError("the fine grid needs an odd nx and an even nb")
+    sx = np.array(shots if shots is not None else [(nx // 2) & ~1], np.int32)
+    if np.any(sx % 2):
+        raise ValueError("fine source positions must be even")
+    cx, cz = (nx + 1) // 2, (nz + 1 - OZ) // 2
+    fine = dict(nx=nx, nz=nz, nb=nb, dx=case["dx"], dt=case["dt"], fm=case["fm"], nt=case["nt"], sx=sx,
+                sz=np.full(sx.size, 2 + OZ, np.int32), gx0=0, ng=nx, gz=OZ, nmodels=1)
+    coarse = dict(nx=cx, nz=cz, nb=nb // 2, dx=2 * case["dx"], dt=2 * case["dt"], fm=case["fm"],
+                  nt=case["nt"] // 2, sx=sx // 2, sz=np.full(sx.size, 1, np.int32), gx0=0, ng=cx, gz=0, nmodels=1)
+    return fine, coarse
+
+
+def random_models(case, n, seed):
+    """n seeded random layered models on the fine grid (config-0 recipe),
+    [n][nx][nz] in m/s."""
+    return np.stack([workloads.layered_model(case["nz"], case["nx"], case["vmin"], case["vmax"], seed + k)
+                     for k in range(n)])
+
+
+def restrict(v):
+    """Physical fine model -> coarse model: fine cell (2i, 2k + 1)."""
+    v = np.asarray(v)
+    cz = (v.shape[1] + 1 - OZ) // 2
+    return np.ascontiguousarray(v[::2, OZ::2][:, :cz])
+
+
+def expand(v, grid):
+    return capi.expand_domain(np.ascontiguousarray(v, np.float32), grid["nb"], grid["dx"], grid["dt"])
+
+
+def region(grid):
+    """The physical interior of a padded grid: (x0, x1, z0, z1)."""
+    nxpad, nzpad, bx0 = grid["nx"] + 2 * grid["nb"] + 12, grid["nz"] + grid["nb"] + 12, grid["nb"] + 6
+    return (bx0, nxpad - bx0, 6, nzpad - bx0)
+
+
+def with_models(grid, n):
+    g = dict(grid)
+    g["nmodels"] = n
+    return g
+
+
+def training_pairs(case, models, times):
+    """Inputs (the coarse leapfrog after 4 steps from the fine wavefield) and
+    targets (the fine wavefield there) for every model and coarse start step j
+    in `times`: [2 * len(models) * len(times)][nxpad'][nzpad'] each, and the
+    fine seismograms [model][nt][ng]."""
+    fine, coarse = grids(case)
+    n = len(models)
+    fb = Batch([Group(**with_models(fine, n))], snap_prec=capi.SNAP_NONE, order=capi.ORDER_10)
+    for m in range(n):
+        fb.bind_model(0, m, expand(models[m], fine))
+    taps = []
+    for j in times:
+        taps += [2 * j - 2, 2 * j, 2 * j + 6, 2 * j + 8]
+    fb.set_taps(taps, oz=OZ)
+    dfine = np.stack(fb.split_data(fb.forward()))  # [model][nt][ng]
+    T = np.stack(fb.taps())  # [model][tap][x][z]
+    fb.close()
+    cb = Batch([Group(**with_models(coarse, n))], snap_prec=capi.SNAP_NONE, order=capi.ORDER_4)
+    for m in range(n):
+        cb.bind_model(0, m, expand(restrict(models[m]), coarse))
+    ins, tgts = [], []
+    shape = T.shape[2:]
+    for k, j in enumerate(times):
+        prev, cur = cb.step_from(T[:, 4 * k], T[:, 4 * k + 1], it0=j, nsteps=4)
+        ins += [prev.reshape((n,) + shape), cur.reshape((n,) + shape)]
+        tgts += [T[:, 4 * k + 2], T[:, 4 * k + 3]]
+    cb.close()
+    return (np.ascontiguousarray(np.concatenate(ins), np.float32),
+            np.ascontiguousarray(np.concatenate(tgts), np.float32), dfine)
+
+
+def coarse_seismograms(case, models, net=None):
+    """Coarse forward runs of the models (with the correction when net is
+    given): [model][nt'][ng']."""
+    _, coarse = grids(case)
+    n = len(models)
+    cb = Batch([Group(**with_models(coarse, n))], snap_prec=capi.SNAP_NONE, order=capi.ORDER_4)
+    for m in range(n):
+        cb.bind_model(0, m, expand(restrict(models[m]), coarse))
+    if net is not None:
+        cb.set_correction(net)
+    d = cb.forward()
+    out = np.stack(cb.split_data(d))
+    cb.close()
+    return out
+
+
+def fine_on_coarse(dfine):
+    """Fine seismograms [model][nt][ng] at the coarse samples and receivers."""
+    return np.ascontiguousarray(np.asarray(dfine)[:, ::2, ::2])
+
+
+def seis_error(dc, df):
+    return float(np.linalg.norm(dc - df) / np.linalg.norm(df))
+
+
+def train_and_evaluate(case, n_train, train_steps, seed=0, times=None, batch=16, lr=2e-3, verbose=False):
+    """Train a network on n_train random models and compare the coarse
+    propagation with and without it against the fine propagation on one
+    held-out model."""
+    _, coarse = grids(case)
+    nt_c = case["nt"] // 2
+    if times is None:
+        times = list(range(nt_c // 6, nt_c - 8, max(nt_c // 12, 1)))
+    models = random_models(case, n_train + 1, seed)
+    ins, tgts, dfine = training_pairs(case, models[:n_train], times)
+    reg = region(coarse)
+    net = Net(Net.initial(seed=seed, scale=1e-3))
+    net.set_data(ins, tgts, reg)
+    base = net.loss_grad(ins[:min(len(ins), 64)], tgts[:min(len(ins), 64)], reg)[0]
+    losses = net.train(batch=min(batch, len(ins)), steps=train_steps, lr=lr, seed=seed)
+    if verbose:
+        print(f"training: {len(ins)} pairs, loss {losses[0]:.3e} -> {losses[-10:].mean():.3e}", flush=True)
+    held = models[n_train:]
+    _, _, dfh = training_pairs(case, held, times[:1])
+    ref = fine_on_coarse(dfh)
+    plain = coarse_seismograms(case, held)
+    corr = coarse_seismograms(case, held, net)
+    out = {"err_plain": seis_error(plain, ref), "err_learned": seis_error(corr, ref), "losses": losses,
+           "pairs": len(ins), "base_loss": base, "net": net}
+    return out
+
+
+# ---- FWI with the plain fine propagator and with the learned coarse one ----
+
+def _mask_scale(g, grid):
+    """scaleGradient + maskGradient (damp4t10d.cpp:309-319,216-226) on a
+    padded gradient: zero outside the physical interior, times sqrt(iz) of
+    the physical depth inside (1 at the surface row)."""
+    x0, x1, z0, z1 = region(grid)
+    out = np.zeros_like(g)
+    iz = np.arange(z1 - z0, dtype=np.float32)
+    iz[0] = 1.0
+    out[x0:x1, z0:z1] = g[x0:x1, z0:z1] * np.sqrt(iz)[None, :]
+    return out
+
+
+def _refill(vt, grid):
+    """refillBoundary (damp4t10d.cpp:228-252): edge-replicate the interior."""
+    x0, x1, z0, z1 = region(grid)
+    ix = np.clip(np.arange(vt.shape[0]), x0, x1 - 1)
+    iz = np.clip(np.arange(vt.shape[1]), z0, z1 - 1)
+    return np.ascontiguousarray(vt[np.ix_(ix, iz)])
+
+
+def fwi_run(grid, order, v0, dobs, iters, vmin, vmax, net=None, dv=30.0, snap_prec=None, log=None):
+    """`iters` FWI iterations on one grid: every shot's gradient in one
+    batched forward (checkpoints) + adjoint source + backward, PR+ conjugate
+    gradient, a two-trial parabolic step on the summed objective, clamped
+    update of the transformed model and boundary refill.  dobs: [shot][nt][ng]
+    on this grid's receivers and samples.  Returns the final physical model,
+    the objectives and the gradient time per shot (forward + backward on the
+    device, ms)."""
+    prec = capi.SNAP_BF16 if snap_prec is None else snap_prec
+    S = len(grid["sx"])
+    b = Batch([Group(**grid)], snap_prec=prec, order=order)
+    if net is not None:
+        b.set_correction(net)
+    tr = Batch([Group(**grid)], snap_prec=capi.SNAP_NONE, order=order)  # trial forward runs
+    if net is not None:
+        tr.set_correction(net)
+    dx, dt = grid["dx"], grid["dt"]
+    tmin, tmax = (dx / dt / vmax) ** 2, (dx / dt / vmin) ** 2
+    vt = expand(v0, grid)
+    obs = np.ascontiguousarray(np.asarray(dobs, np.float32).ravel())
+
+    def objective(model, batch):
+        batch.bind_model(0, 0, model)
+        d = batch.forward()
+        return float(np.sum((obs.astype(np.float64) - d) ** 2)), d
+
+    objs, gms = [], []
+    g_prev = d_prev = None
+    for it in range(iters):
+        b.bind_model(0, 0, vt)
+        dcal = b.forward()
+        J = float(np.sum((obs.astype(np.float64) - dcal) ** 2))
+        b.adjoint_source(obs, want=False)
+        _, gm = b.backward(None, want_entries=False)
+        info = b.info()
+        gms.append((info["forward_ms"] + info["backward_ms"]) / S)
+        g = _mask_scale(gm.reshape(vt.shape), grid)
+        if d_prev is None:
+            d = g
+        else:  # PR+ (fwiframework.cpp:42-72)
+            beta = max(0.0, float(np.sum(g * (g - g_prev)) / max(np.sum(g_prev * g_prev), 1e-30)))
+            d = g + beta * d_prev
+        g_prev, d_prev = g, d
+        # step: the largest velocity change of alpha * d is dv m/s
+        v = dx / (dt * np.sqrt(vt))
+        dvdt = v / (2 * vt)  # |d v / d vel_t|
+        a0 = dv / max(float(np.max(np.abs(d) * dvdt)), 1e-30)
+        trial = []
+        for a in (a0, 2 * a0):
+            m = _refill(np.clip(vt + a * d, tmin, tmax).astype(np.float32), grid)
+            trial.append(objective(m, tr)[0])
+        # the parabola through (0, J), (a0, j1), (2 a0, j2): vertex at
+        # a0 (3J + j2 - 4 j1) / (2 (j2 - 2 j1 + J)) when convex
+        j1, j2 = trial
+        den = j2 - 2 * j1 + J
+        alpha = a0 * 0.5 * (3 * J + j2 - 4 * j1) / den if den > 0 else (2 * a0 if j2 < j1 else a0)
+        alpha = float(np.clip(alpha, 0.25 * a0, 4 * a0))
+        vt = _refill(np.clip(vt + alpha * d, tmin, tmax).astype(np.float32), grid)
+        objs.append(J)
+        if log:
+            log(f"  iter {it}: objective {J:.5e}, gradient {gms[-1]:.2f} ms/shot, step {alpha:.3e}")
+    b.close()
+    tr.close()
+    vfin = dx / (dt * np.sqrt(vt.astype(np.float64)))
+    x0, x1, z0, z1 = region(grid)
+    return np.ascontiguousarray(vfin[x0:x1, z0:z1], np.float32), objs, float(np.median(gms))
+
+
+def upsample(vc, shape):
+    """Coarse physical model -> fine: each fine cell takes the coarse cell
+    nearest to it (fine (2i, 2k + 1) is coarse (i, k))."""
+    ix = np.minimum((np.arange(shape[0]) + 1) // 2, vc.shape[0] - 1)
+    iz = np.clip((np.arange(shape[1]) - OZ + 1) // 2, 0, vc.shape[1] - 1)
+    return np.ascontiguousarray(vc[np.ix_(ix, iz)])
+
+
+def fine_misfit(case, grid_f, v, dobs_f):
+    """The judge: the fine propagator's data misfit of a physical model."""
+    b = Batch([Group(**grid_f)], snap_prec=capi.SNAP_NONE, order=capi.ORDER_10)
+    b.bind_model(0, 0, expand(v, grid_f))
+    d = b.forward()
+    b.close()
+    return float(np.sum((np.asarray(dobs_f, np.float64).ravel() - d) ** 2))

@@ -1,0 +1,145 @@
+"""Mixed-size batched gradients with reduced-precision checkpoints
+(swfwi_batch) through the C ABI.
+
+Groups of different grid sizes and time-step counts run in one launch per
+step.  Per entry (one shot of one model):
+* the seismogram against the oracle's FwiForwardModeling
+  (damp4t10d.cpp:272-289) within the FP32 budget, rel-L2 <= 1e-5 (the batch
+  stencil computes in FP32 with fused multiply-adds);
+* the adjoint source transVsrc(dobs - dcal) against the oracle's
+  (fwiframework.cpp:75-105) within 1e-6;
+* the gradient, formed from FP32 / BF16 / FP16 checkpoints every 4 steps,
+  against the FP64 run of the reference calgradient (band reconstruction,
+  imaging on every step, fwiframework.cpp:114-177), over the region
+  maskGradient keeps: rel-L2 <= 1e-3;
+* per-model gradients equal the per-entry gradients summed in shot order."""
+import numpy as np
+import pytest
+
+from swfwi_amd import capi, workloads
+from swfwi_amd.batch import Batch, Group
+
+pytestmark = pytest.mark.gpu
+
+SEIS_TOL = 1e-5
+VSRC_TOL = 1e-6
+GRAD_TOL = 1e-3
+
+
+@pytest.fixture(scope="module", autouse=True)
+def need_gpu():
+    if capi.device_count() < 1:
+        pytest.skip("no HIP device")
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def groups():
+    # two grid sizes, different nt (the shorter group leaves the launches
+    # first), two models x three / two shots each
+    a = Group(nx=48, nz=32, nb=8, dx=10.0, dt=0.001, fm=12.0, nt=600, sx=np.array([3, 24, 44]),
+              sz=np.array([1, 1, 2]), nmodels=2)
+    b = Group(nx=70, nz=44, nb=10, dx=10.0, dt=0.001, fm=12.0, nt=720, sx=np.array([10, 55]), sz=np.array([1, 3]),
+              nmodels=2)
+    return [a, b]
+
+
+def models_for(g, k):
+    case = workloads.Case("b", g.nz, g.nx, g.dx, g.nb, g.dt, g.nt, g.fm, np.asarray(g.sx, np.int32),
+                          np.asarray(g.sz, np.int32), np.arange(g.nx, dtype=np.int32), np.zeros(g.nx, np.int32),
+                          1500.0, 3500.0, seed=10 + k)
+    vtrue, v0 = workloads.models(case)
+    return (capi.expand_domain(vtrue, g.nb, g.dx, g.dt), capi.expand_domain(v0, g.nb, g.dx, g.dt))
+
+
+def inner_mask(g):
+    bx0 = g.nb + 6
+    m = np.zeros(g.shape, bool)
+    m[bx0:g.nxpad - bx0, 6:g.nzpad - bx0] = True
+    return m
+
+
+@pytest.fixture(scope="module")
+def run(oracle):
+    gs = groups()
+    b = Batch(gs, snap_prec=capi.SNAP_F32)
+    mods = {}
+    for gi, g in enumerate(gs):
+        for m in range(g.nmodels):
+            mods[(gi, m)] = models_for(g, 2 * gi + m)
+    # observed data: the true models
+    for (gi, m), (vt, vi) in mods.items():
+        b.bind_model(gi, m, vt)
+    print("batch forward on the true models", flush=True)
+    dobs = b.forward()
+    # the initial models: forward with checkpoints, adjoint sources
+    for (gi, m), (vt, vi) in mods.items():
+        b.bind_model(gi, m, vi)
+    print("batch forward on the initial models", flush=True)
+    dcal = b.forward()
+    vsrc = b.adjoint_source(dobs)
+    return gs, b, mods, dobs, dcal, vsrc
+
+
+def oracle_geom(oracle, g):
+    return oracle.Geom(g.nx, g.nz, g.nb, g.dt, g.dx, g.fm, g.nt, np.asarray(g.sx, np.int32),
+                       np.asarray(g.sz, np.int32), np.arange(g.nx, dtype=np.int32), np.zeros(g.nx, np.int32))
+
+
+def test_batch_seismograms_and_adjoint_sources(oracle, run):
+    gs, b, mods, dobs, dcal, vsrc = run
+    D, V, O = b.split_data(dcal), b.split_data(vsrc), b.split_data(dobs)
+    for k, (gi, m, s) in enumerate(b.entries):
+        g = gs[gi]
+        og = oracle_geom(oracle, g)
+        wlt = oracle.ricker(g.nt, g.fm, g.dt, 1.0)
+        want = og.forward_modeling(mods[(gi, m)][1], wlt, shot=s)
+        e = rel_l2(D[k], want)
+        assert e <= SEIS_TOL, (gi, m, s, e)
+        r = (O[k] - D[k]).astype(np.float32)
+        oracle.trans_vsrc(r)
+        assert rel_l2(V[k], r) <= VSRC_TOL, (gi, m, s)
+
+
+@pytest.mark.parametrize("prec", [capi.SNAP_F32, capi.SNAP_BF16, capi.SNAP_F16])
+def test_batch_gradient_against_fp64_oracle(oracle, run, prec):
+    gs, b, mods, dobs, dcal, vsrc = run
+    if prec != capi.SNAP_F32:
+        # a fresh batch in the narrow format (its checkpoints are half the size)
+        b = Batch(gs, snap_prec=prec)
+        for (gi, m), (vt, vi) in mods.items():
+            b.bind_model(gi, m, vi)
+        b.forward(want_data=False)
+    print(f"batch backward, checkpoints {['f32', 'bf16', 'f16'][prec]}", flush=True)
+    ge, gm = b.backward(vsrc)
+    GE, GM = b.split_grid(ge), b.split_grid(gm, per_model=True)
+    V = b.split_data(vsrc)
+    errs = []
+    for k, (gi, m, s) in enumerate(b.entries):
+        g = gs[gi]
+        og = oracle_geom(oracle, g)
+        wlt = oracle.ricker(g.nt, g.fm, g.dt, 1.0)
+        want = og.calgradient64(mods[(gi, m)][1], wlt, V[k], shot=s)
+        mask = inner_mask(g)
+        e = rel_l2(GE[k][mask], want[mask])
+        errs.append(e)
+        assert e <= GRAD_TOL, (gi, m, s, e)
+    print(f"  gradient rel-L2 vs FP64: max {max(errs):.2e}, mean {np.mean(errs):.2e}", flush=True)
+    # model gradients: the shots summed in shot order
+    mi = 0
+    for gi, g in enumerate(gs):
+        for m in range(g.nmodels):
+            ks = [k for k, (a, bm, s) in enumerate(b.entries) if a == gi and bm == m]
+            acc = np.zeros(g.shape, np.float32)
+            for k in ks:
+                acc = (acc + GE[k]).astype(np.float32)
+            assert np.array_equal(GM[mi], acc)
+            mi += 1
+    info = b.info()
+    assert info["entries"] == len(b.entries) and info["snapshot_bytes"] > 0
+    if prec != capi.SNAP_F32:
+        b.close()

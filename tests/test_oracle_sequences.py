@@ -465,3 +465,32 @@ def test_random_codes_follow_boost_uniform_int(oracle):
     # the bucket is (brange + 1) / 2 = 1073741823 and never rejects a draw
     assert (Minstd.max - Minstd.min + 1) // 2 == 1073741823
     assert (Minstd.max - Minstd.min) // 1073741823 == 1
+
+
+# -- the FP64 run (oracle64.c) -------------------------------------------------------
+
+def test_fp64_forward_modeling_matches_float64_leapfrog_and_float_path(oracle):
+    g, vel, rng = _case(oracle, nt=300)
+    wlt = oracle.ricker(g.nt, 20.0, 0.001, 1.0)
+    pts = [_pos(g, int(x), int(z)) for x, z in zip(g.sx, g.sz)]
+
+    def inject(f, it):
+        f[pts[2]] += wlt[it]
+
+    C = forward_fields(g.shape, 1.0 / vel.astype(np.float64), damping(*g.shape, g.bx0), inject, g.nt)
+    rec = C[:, g.gx + g.bx0, g.gz + g.bz0]
+    d64 = g.forward_modeling64(vel, wlt, shot=2)
+    assert np.linalg.norm(d64 - rec) / np.linalg.norm(rec) < 1e-12
+    d32 = g.forward_modeling(vel, wlt, shot=2)
+    assert np.linalg.norm(d32 - d64) / np.linalg.norm(d64) < 1e-5
+
+
+def test_fp64_calgradient_matches_brute_force_and_float_path(oracle):
+    g, vel, rng = _case(oracle, nt=340)
+    enc = rng.standard_normal(g.nt).astype(np.float32)
+    vsrc = rng.standard_normal((g.nt, g.ng)).astype(np.float32)
+    g64 = g.calgradient64(vel, enc, vsrc, shot=1)
+    want, inner = brute_gradient(g, vel, enc, 1, 2, vsrc)
+    assert np.linalg.norm(g64[inner] - want[inner]) / np.linalg.norm(want[inner]) < 1e-10
+    g32 = g.calgradient(vel, enc, vsrc, shot=1)
+    assert np.linalg.norm(g32[inner] - g64[inner]) / np.linalg.norm(g64[inner]) < 1e-5
