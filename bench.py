@@ -73,6 +73,192 @@ def cpu_baseline(case, encoded, seconds_cap=30.0):
     return out, po.lib().orc_max_threads()
 
 
+def mixed_groups(shots=16):
+    """The mixed-size batch: 4 models of 201x301 (dx=10 m, nb=20, nt=2000,
+    15 Hz), 2 of 301x921 (12.5 m, nb=30, nt=4000, 10 Hz) and 1 of 501x1501
+    (12.5 m, nb=30, nt=6000, 10 Hz), `shots` Ricker shots each on the surface
+    (depth 1 cell), receivers on every surface cell, dt=1 ms."""
+    from swfwi_amd.batch import Group
+
+    spec = [(301, 201, 20, 10.0, 15.0, 2000, 4, 1500.0, 4500.0), (921, 301, 30, 12.5, 10.0, 4000, 2, 1500.0, 5500.0),
+            (1501, 501, 30, 12.5, 10.0, 6000, 1, 1500.0, 5500.0)]
+    groups = []
+    for nx, nz, nb, dx, fm, nt, nm, vmin, vmax in spec:
+        sx = np.linspace(0, nx - 1, shots).astype(np.int32)
+        groups.append(Group(nx=nx, nz=nz, nb=nb, dx=dx, dt=0.001, fm=fm, nt=nt, sx=sx, sz=np.ones(shots, np.int32),
+                            gx0=0, ng=nx, gz=0, nmodels=nm, extra=dict(vmin=vmin, vmax=vmax)))
+    return groups
+
+
+def mixed_workload(args):
+    """Mixed-size batched gradients with reduced-precision checkpoints: every
+    shot of every model in one launch per time step, the source field kept
+    every 4 steps in BF16 (and, for the error figures, FP16), gradients of
+    one shot per grid size against the FP64 oracle run."""
+    from oracle import pyoracle as po
+    from swfwi_amd import capi, workloads
+    from swfwi_amd.batch import Batch
+
+    groups = mixed_groups(args.shots)
+    models = []
+    for gi, g in enumerate(groups):
+        for m in range(g.nmodels):
+            vt = workloads.layered_model(g.nz, g.nx, g.extra["vmin"], g.extra["vmax"], seed=100 + 10 * gi + m)
+            v0 = workloads.smoothed(vt, 20.0)
+            models.append((gi, m, capi.expand_domain(vt, g.nb, g.dx, g.dt), capi.expand_domain(v0, g.nb, g.dx, g.dt)))
+    res = {}
+    timing = None
+    for prec in (capi.SNAP_BF16, capi.SNAP_F16):
+        name = {capi.SNAP_BF16: "bf16", capi.SNAP_F16: "f16"}[prec]
+        b = Batch(groups, snap_prec=prec)
+        b.set_graphs(not args.no_graphs)
+        for gi, m, vt, _ in models:
+            b.bind_model(gi, m, vt)
+        dobs = b.forward()
+        for gi, m, _, vi in models:
+            b.bind_model(gi, m, vi)
+        b.forward(want_data=False)  # captures the loop (graph) on first use
+        b.adjoint_source(dobs, want=False)
+        b.backward(None, want_entries=False, want_models=False)
+        steps, fms, bms = max(args.steps, 1), [], []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            b.forward(want_data=False)
+            b.adjoint_source(dobs, want=False)
+            b.backward(None, want_entries=False, want_models=False)
+            inf = b.info()
+            fms.append(inf["forward_ms"])
+            bms.append(inf["backward_ms"])
+        wall = (time.perf_counter() - t0) / steps
+        inf = b.info()
+        vsrc = b.adjoint_source(dobs)
+        ge, _ = b.backward(None, want_entries=True, want_models=False)
+        GE, V = b.split_grid(ge), b.split_data(vsrc)
+        errs = {}
+        for gi, g in enumerate(groups):  # model 0, shot 0 of each size
+            k = b.entries.index((gi, 0, 0))
+            og = po.Geom(g.nx, g.nz, g.nb, g.dt, g.dx, g.fm, g.nt, g.sx, g.sz, np.arange(g.nx, dtype=np.int32),
+                         np.zeros(g.nx, np.int32))
+            wlt = po.ricker(g.nt, g.fm, g.dt, 1.0)
+            vi = [v for gg, mm, _, v in models if gg == gi and mm == 0][0]
+            t1 = time.perf_counter()
+            want = og.calgradient64(vi, wlt, V[k], shot=0)
+            bx0 = g.nb + 6
+            sl = (slice(bx0, g.nxpad - bx0), slice(6, g.nzpad - bx0))
+            err = float(np.linalg.norm(GE[k][sl] - want[sl]) / np.linalg.norm(want[sl]))
+            errs[f"{g.nz}x{g.nx}"] = round(err, 7)
+            print(f"[bench] {name}: gradient of {g.nz}x{g.nx} shot 0 against the FP64 run: rel-L2 {err:.3e} "
+                  f"(oracle {time.perf_counter() - t1:.1f} s)", file=sys.stderr, flush=True)
+        res[name] = errs
+        if prec == capi.SNAP_BF16:
+            pts = inf["forward_points"] + 2 * inf["backward_points"]  # the backward advances two fields
+            dev_ms = float(np.mean(fms) + np.mean(bms))
+            timing = {"wall_ms": 1e3 * wall, "device_ms": dev_ms, "forward_ms": float(np.mean(fms)),
+                      "backward_ms": float(np.mean(bms)), "points": pts, "snapshot_bytes": inf["snapshot_bytes"],
+                      "entries": inf["entries"]}
+        b.close()
+    gpts = timing["points"] / (timing["wall_ms"] * 1e-3) / 1e9
+    line = {
+        "metric": "FWI gradient time/shot & Gpts/s, learned vs plain FP32 at equal data misfit",
+        "value": round(gpts, 3), "unit": "Gpts/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": round(timing["wall_ms"], 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded layered models; observed data from the same propagator on the true models)",
+        "config": {"workload": "mixed-size batch: 4x(201x301, nt=2000), 2x(301x921, nt=4000), 1x(501x1501, "
+                               f"nt=6000), {args.shots} shots each, one launch per time step, checkpoints every 4 "
+                               "steps in BF16 with per-snapshot max-abs scale",
+                   "parallelism": "single GPU"},
+        "gradient_ms_per_shot": round(timing["device_ms"] / timing["entries"], 4),
+        "forward_ms": round(timing["forward_ms"], 2), "backward_ms": round(timing["backward_ms"], 2),
+        "checkpoint_bytes": timing["snapshot_bytes"],
+        "gradient_rel_l2_vs_fp64": res,
+    }
+    print(json.dumps(line))
+
+
+def learned_workload(args):
+    """Learned coarse-grid correction FWI against the plain fine FP32 FWI.
+
+    The network is trained on the GPU (its own kernels) from fine 10th-order
+    runs of 64 seeded random layered models on a 161x80 grid with the same
+    spacing, time step and band as the test grid (it is a 7x7-cell local
+    operator, so it applies to any grid of that spacing); its output gain is
+    picked on two validation models.  On a held-out 501x1501 model (12.5 m,
+    nb=30, nt=3000, 8 Hz, `shots` surface shots, a receiver on every surface
+    cell) `iters` FWI iterations run three ways: the plain fine FP32
+    propagator, the 2x-coarser 4th-order propagator (25 m, 2 ms) with the
+    network applied every 4 steps, and the same coarse propagator without
+    it.  The fine propagator judges every final model (data misfit on the
+    fine grid)."""
+    from swfwi_amd import capi, learned, workloads
+    from swfwi_amd.batch import Batch, Group
+
+    learned.GRAPHS = not args.no_graphs
+    t0 = time.perf_counter()
+    train_case = dict(nx=161, nz=80, nb=20, dx=12.5, dt=0.001, fm=8.0, nt=1200, vmin=1500.0, vmax=4500.0)
+    res = learned.train_and_evaluate(train_case, n_train=64, seed=21)
+    net = res["net"]
+    t_train = time.perf_counter() - t0
+    print(f"[bench] trained in {t_train:.1f} s: held-out seismogram error plain coarse {res['err_plain']:.4f}, "
+          f"learned {res['err_learned']:.4f} (gain {res['gain']})", file=sys.stderr, flush=True)
+    case = learned.config4_case()
+    nx = case["nx"]
+    shots = (np.linspace(0, nx - 1, args.shots).astype(np.int64) & ~1).astype(np.int32)
+    fine, coarse = learned.grids(case, shots)
+    vtrue = learned.random_models(case, 1, 9001)[0]  # [nx][nz], m/s
+    v0 = workloads.smoothed(vtrue, 20.0)
+    fb = Batch([Group(**fine)], snap_prec=capi.SNAP_NONE, order=capi.ORDER_10)
+    fb.set_graphs(learned.GRAPHS)
+    fb.bind_model(0, 0, learned.expand(vtrue, fine))
+    dobs_f = np.stack(fb.split_data(fb.forward()))  # [shot][nt][ng]
+    fb.close()
+    dobs_c = learned.fine_on_coarse(dobs_f)
+    iters = args.iters
+    log = (lambda m: print(m, file=sys.stderr, flush=True))
+    out = {}
+    runs = (("plain_fine_fp32", fine, capi.ORDER_10, v0, dobs_f, None),
+            ("learned_coarse", coarse, capi.ORDER_4, learned.restrict(v0), dobs_c, net),
+            ("plain_coarse", coarse, capi.ORDER_4, learned.restrict(v0), dobs_c, None))
+    for name, grid, order, vstart, d, n in runs:
+        t1 = time.perf_counter()
+        print(f"[bench] {name}: {iters} FWI iterations", file=sys.stderr, flush=True)
+        v, objs, gms, pts = learned.fwi_run(grid, order, vstart, d, iters, case["vmin"], case["vmax"], net=n, log=log)
+        wall = time.perf_counter() - t1
+        vf = v if grid is fine else learned.upsample(v, (nx, case["nz"]))
+        misfit = learned.fine_misfit(case, fine, vf, dobs_f)
+        out[name] = {"gradient_ms_per_shot": round(gms, 3), "gpts_per_s": round(pts / (gms * len(shots) * 1e-3) / 1e9, 2),
+                     "fine_judged_misfit": float(f"{misfit:.6e}"), "own_grid_misfit": [float(f"{o:.5e}") for o in objs],
+                     "wall_s": round(wall, 1)}
+    m0 = learned.fine_misfit(case, fine, v0, dobs_f)
+    pf, lc, pc = out["plain_fine_fp32"], out["learned_coarse"], out["plain_coarse"]
+    speedup = pf["gradient_ms_per_shot"] / lc["gradient_ms_per_shot"]
+    ratio = lc["fine_judged_misfit"] / pf["fine_judged_misfit"]
+    line = {
+        "metric": "FWI gradient time/shot & Gpts/s, learned vs plain FP32 at equal data misfit",
+        "value": round(speedup, 3), "unit": "x (plain fine FP32 gradient time/shot / learned coarse)",
+        "n_gpus": 1, "steps": iters, "warmup": 0, "ms_per_step": round(1e3 * lc["wall_s"] / iters, 1),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32 (network: bf16 MFMA)",
+        "data": "synthetic (seeded random layered models; observed data from the fine propagator on the true "
+                "held-out model)",
+        "config": {"workload": f"FWI {iters} iterations on a held-out 501x1501 model (12.5 m, nb=30, nt=3000, 8 Hz, "
+                               f"{len(shots)} shots x 1501 receivers): plain fine 10th-order FP32 vs 2x-coarser "
+                               "4th-order (25 m, 2 ms) with the 3-layer 3x3 16-channel correction every 4 steps",
+                   "parallelism": "single GPU"},
+        "learned_vs_plain": {
+            "gradient_speedup": round(speedup, 3),
+            "fine_misfit_ratio_learned_over_plain": round(ratio, 4),
+            "equal_misfit_within_2pct": bool(abs(ratio - 1.0) <= 0.02),
+            "fine_misfit_initial_model": float(f"{m0:.6e}"),
+            "plain_coarse_fine_misfit_ratio": round(pc["fine_judged_misfit"] / pf["fine_judged_misfit"], 4),
+        },
+        "runs": out,
+        "training": {"models": 64, "grid": "161x80 at 12.5 m", "seconds": round(t_train, 1), "gain": res["gain"],
+                     "held_out_seismogram_error": {"plain_coarse": round(res["err_plain"], 4),
+                                                   "learned": round(res["err_learned"], 4)}},
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,7 +267,9 @@ def main():
     ap.add_argument("--precision", choices=["exact", "fp32"], default="exact")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--nt", type=int, default=None, help="override nt (testing only)")
-    ap.add_argument("--workload", choices=["ess_301x921", "fwi_201x301"], default="ess_301x921")
+    ap.add_argument("--workload", choices=["ess_301x921", "fwi_201x301", "mixed", "learned"], default="ess_301x921")
+    ap.add_argument("--shots", type=int, default=16, help="shots per model (mixed and learned workloads)")
+    ap.add_argument("--iters", type=int, default=20, help="FWI iterations (learned workload)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch the time loops directly instead of replaying HIP graphs (for kernel tracers)")
     args = ap.parse_args()
@@ -95,6 +283,10 @@ def main():
     if capi.device_count() < 1:
         print("no HIP device", file=sys.stderr)
         sys.exit(1)
+    if args.workload == "mixed":
+        return mixed_workload(args)
+    if args.workload == "learned":
+        return learned_workload(args)
     ess = args.workload == "ess_301x921"
     case = workloads.ess_301x921(nt=args.nt or 4000) if ess else workloads.fwi_201x301(nt=args.nt or 2000)
     prec = capi.PREC_EXACT if args.precision == "exact" else capi.PREC_FP32

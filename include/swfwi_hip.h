@@ -220,8 +220,9 @@ int swfwi_enfwi_gradients(swfwi_enfwi f, float *grads);
  * condition formed on the checkpoint steps with weight 4 w(it).  The stencil
  * computes in FP32.
  * Group g: dims[9g..9g+8] = nx, nz, nb, nt, nshots, gx0, ng, gz, nmodels
- * (receivers at physical x = gx0 .. gx0+ng-1, depth gz); params[3g..3g+2] =
- * dx, dt, fm (Ricker wavelet of amplitude 1); sx/sz: the shots' physical
+ * (receivers at physical x = gx0 .. gx0+ng-1, depth gz); params[4g..4g+3] =
+ * dx, dt, fm (Ricker wavelet of amplitude 1), sponge strength factor (1 for
+ * the reference's profile, k for a grid stepping k times its dt); sx/sz: the shots' physical
  * positions, groups concatenated.  Entries (group, model, shot) are ordered
  * group by group, model by model, shot by shot in every array below. */
 int swfwi_batch_create(swfwi_batch *out, int ngroups, const int *dims, const float *params, const int *sx,
@@ -267,27 +268,31 @@ int swfwi_batch_step_from(swfwi_batch b, const float *prev, const float *cur, in
 int swfwi_batch_set_correction(swfwi_batch b, swfwi_net net);
 
 /* ---- learned coarse-grid correction network --------------------------------
- * No counterpart in the reference.  A 3x3 convolution network 1 -> 16 -> 16
- * -> 1 with ReLU between and no biases, y = net(p) added to the field p.
- * Weights (2592 floats): W1[t][c] (9 x 16), W2[t][ci][co] (9 x 16 x 16),
- * W3[t][ci] (9 x 16), tap t = (dx + 1) * 3 + (dz + 1) for the neighbour
- * (x + dx, z + dz).  The 16 -> 16 layer and the weight-gradient products run
- * on BF16 MFMA with FP32 accumulation.  Field stacks are [entries][nx][nz]
- * (z contiguous); region[4] = x0, x1, z0, z1: the outputs and the loss cover
+ * No counterpart in the reference.  A 3x3 convolution network 2 -> 16 -> 16
+ * -> 1 with ReLU between and no biases, y = net(p) added to the field p; its
+ * inputs are p and r p, r = 1 / vel_t (the local Courant number squared,
+ * given as the rv stack beside the fields).  Weights (2736 floats):
+ * W1[t][k][c] (9 x 2 x 16), W2[t][ci][co] (9 x 16 x 16), W3[t][ci] (9 x 16),
+ * tap t = (dx + 1) * 3 + (dz + 1) for the neighbour (x + dx, z + dz).  The
+ * 16 -> 16 layer and the weight-gradient products run on BF16 MFMA with FP32
+ * accumulation.  Field stacks are [entries][nx][nz] (z contiguous);
+ * region[4] = x0, x1, z0, z1: the outputs and the loss cover
  * [x0, x1) x [z0, z1), which must lie 5 cells inside the array. */
 int swfwi_net_create(swfwi_net *out);
 int swfwi_net_destroy(swfwi_net n);
 int swfwi_net_set_weights(swfwi_net n, const float *w);
 int swfwi_net_get_weights(swfwi_net n, float *w);
 /* y = net(fields) over the region (zero elsewhere) */
-int swfwi_net_apply(swfwi_net n, const float *fields, int nx, int nz, int entries, const int *region, float *y);
+int swfwi_net_apply(swfwi_net n, const float *fields, const float *rv, int nx, int nz, int entries, const int *region,
+                    float *y);
 /* loss = mean over the region of (in + net(in) - target)^2 and its gradient
  * with respect to the weights (grad may be null) */
-int swfwi_net_loss_grad(swfwi_net n, const float *in, const float *target, int nx, int nz, int entries,
-                        const int *region, float *grad, double *loss);
-/* Training set kept on the device: nsamples inputs and targets [n][nx][nz]. */
-int swfwi_net_set_data(swfwi_net n, const float *in, const float *target, int nx, int nz, int nsamples,
-                       const int *region);
+int swfwi_net_loss_grad(swfwi_net n, const float *in, const float *rv, const float *target, int nx, int nz,
+                        int entries, const int *region, float *grad, double *loss);
+/* Training set kept on the device: nsamples inputs, their rv and targets
+ * [n][nx][nz] each. */
+int swfwi_net_set_data(swfwi_net n, const float *in, const float *rv, const float *target, int nx, int nz,
+                       int nsamples, const int *region);
 /* `steps` Adam steps (beta 0.9 / 0.999) on minibatches of `batch` samples in a
  * seeded shuffled order; losses[steps] (may be null) = loss before each step. */
 int swfwi_net_train(swfwi_net n, int batch, int steps, float lr, unsigned long long seed, double *losses);

@@ -6,15 +6,17 @@ product never imports it.
 
 Network: y = sum_t sum_ci W3[t][ci] h2[ci](x + o_t),
 h2 = relu(sum_t sum_ci W2[t][ci][co] h1[ci](x + o_t)),
-h1 = relu(sum_t W1[t][c] p(x + o_t)), taps o_t = (dx, dz) with
-t = (dx + 1) * 3 + (dz + 1), the field zero outside its array.  Loss: mean
-over the region of (p + y - target)^2.
+h1 = relu(sum_t (W1[t][0][c] p(x + o_t) + W1[t][1][c] (r p)(x + o_t))),
+taps o_t = (dx, dz) with t = (dx + 1) * 3 + (dz + 1), the fields zero
+outside their array, r = 1 / vel_t.  Loss: mean over the region of
+(p + y - target)^2.
 
-With emulate=True the BF16 roundings of the device are reproduced: h1 and
-W2 enter the 16 -> 16 product rounded to BF16 (round to nearest even), and
-in the backward pass dz2 is rounded to BF16 before the two products that
-read it; everything else is FP64.  With emulate=False the network is plain
-FP64 (for finite-difference checks)."""
+With emulate=True the BF16 roundings of the device are reproduced: every
+product reads its operands rounded to BF16 (round to nearest even) -- the
+inputs p and r p (r p formed in FP32), W1, h1, W2, h2 and W3 -- and in the
+backward pass dz2 is rounded to BF16 before the two products that read it;
+everything else is FP64.  With emulate=False the network is plain FP64 (for
+finite-difference checks)."""
 import numpy as np
 
 C = 16
@@ -28,11 +30,15 @@ def bf16(x):
     return u.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
+NW1 = 9 * 2 * C
+NPARAMS = NW1 + 9 * C * C + 9 * C
+
+
 def split(theta):
     th = np.asarray(theta, np.float64)
-    w1 = th[:9 * C].reshape(9, C)
-    w2 = th[9 * C:9 * C + 9 * C * C].reshape(9, C, C)
-    w3 = th[9 * C + 9 * C * C:].reshape(9, C)
+    w1 = th[:NW1].reshape(9, 2, C)
+    w2 = th[NW1:NW1 + 9 * C * C].reshape(9, C, C)
+    w3 = th[NW1 + 9 * C * C:].reshape(9, C)
     return w1, w2, w3
 
 
@@ -49,29 +55,27 @@ def shift(a, dx, dz):
 TAPS = [(t // 3 - 1, t % 3 - 1) for t in range(9)]
 
 
-def forward(p, theta, emulate=True):
-    """p: [nx][nz] field.  Returns y and the intermediates."""
+def forward(p, r, theta, emulate=True):
+    """p, r: [nx][nz] field and 1 / vel_t.  Returns y and the intermediates."""
     w1, w2, w3 = split(theta)
     p = np.asarray(p, np.float64)
-    sp = [shift(p, dx, dz) for dx, dz in TAPS]
-    z1 = np.einsum("tc,txz->cxz", w1, np.stack(sp))
-    h1 = np.maximum(z1, 0)
-    if emulate:
-        h1 = bf16(h1)
-        w2e = bf16(w2)
-    else:
-        w2e = w2
+    q = np.float64(np.float32(np.asarray(r, np.float32) * np.asarray(p, np.float32)))  # r p formed in FP32
+    rnd = bf16 if emulate else (lambda a: np.asarray(a, np.float64))
+    pin, qin, w1e, w2e, w3e = rnd(p), rnd(q), rnd(w1), rnd(w2), rnd(w3)
+    sp = np.stack([np.stack([shift(pin, dx, dz), shift(qin, dx, dz)]) for dx, dz in TAPS])  # [t][k][x][z]
+    z1 = np.einsum("tkc,tkxz->cxz", w1e, sp)
+    h1 = rnd(np.maximum(z1, 0))
     s1 = np.stack([shift(h1, dx, dz) for dx, dz in TAPS])          # [t][ci][x][z]
     z2 = np.einsum("tio,tixz->oxz", w2e, s1)
-    h2 = np.maximum(z2, 0)
+    h2 = rnd(np.maximum(z2, 0))
     s2 = np.stack([shift(h2, dx, dz) for dx, dz in TAPS])
-    y = np.einsum("ti,tixz->xz", w3, s2)
-    return y, dict(sp=sp, h1=h1, s1=s1, z2=z2, h2=h2, s2=s2, w2e=w2e)
+    y = np.einsum("ti,tixz->xz", w3e, s2)
+    return y, dict(sp=sp, h1=h1, s1=s1, z2=z2, h2=h2, s2=s2, w2e=w2e, w3e=w3e)
 
 
-def loss_grad(p, target, theta, region, emulate=True):
-    """Mean over the region of r^2, r = p + y - target, summed over a stack
-    p[e][nx][nz]; returns (loss, gradient [2592])."""
+def loss_grad(p, rv, target, theta, region, emulate=True):
+    """Mean over the region of res^2, res = p + y - target, summed over a
+    stack p[e][nx][nz] with models rv[e]; returns (loss, gradient)."""
     x0, x1, z0, z1 = region
     p = np.asarray(p, np.float64)
     target = np.asarray(target, np.float64)
@@ -82,8 +86,9 @@ def loss_grad(p, target, theta, region, emulate=True):
     g2 = np.zeros_like(w2)
     g3 = np.zeros_like(w3)
     loss = 0.0
+    rv = np.asarray(rv, np.float32)
     for e in range(E):
-        y, c = forward(p[e], theta, emulate)
+        y, c = forward(p[e], rv[e], theta, emulate)
         r = p[e] + y - target[e]
         mask = np.zeros_like(r, bool)
         mask[x0:x1, z0:z1] = True
@@ -92,7 +97,7 @@ def loss_grad(p, target, theta, region, emulate=True):
         dy = 2 * r / count
         # layer 3
         g3 += np.einsum("xz,tixz->ti", dy, c["s2"])
-        dh2 = sum(w3[t][:, None, None] * shift(dy, -dx, -dz)[None] for t, (dx, dz) in enumerate(TAPS))
+        dh2 = sum(c["w3e"][t][:, None, None] * shift(dy, -dx, -dz)[None] for t, (dx, dz) in enumerate(TAPS))
         dz2 = np.where(c["h2"] > 0, dh2, 0.0)
         if emulate:
             dz2 = bf16(dz2)
@@ -101,6 +106,6 @@ def loss_grad(p, target, theta, region, emulate=True):
         dh1 = sum(np.einsum("io,oxz->ixz", c["w2e"][t], shift(dz2, -dx, -dz)) for t, (dx, dz) in enumerate(TAPS))
         dz1 = np.where(c["h1"] > 0, dh1, 0.0)
         # layer 1
-        g1 += np.einsum("txz,cxz->tc", np.stack(c["sp"]), dz1)
+        g1 += np.einsum("tkxz,cxz->tkc", c["sp"], dz1)
     grad = np.concatenate([g1.ravel(), g2.ravel(), g3.ravel()])
     return loss / count, grad

@@ -26,6 +26,7 @@ class Group:
     ng: int = 0
     gz: int = 0
     nmodels: int = 1
+    dscale: float = 1.0
     extra: dict = field(default_factory=dict)
 
     @property
@@ -53,7 +54,7 @@ class Batch:
         self.groups = list(groups)
         dims = np.array([[g.nx, g.nz, g.nb, g.nt, g.ns, g.gx0, g.ng or g.nx, g.gz, g.nmodels] for g in self.groups],
                         np.int32)
-        params = np.array([[g.dx, g.dt, g.fm] for g in self.groups], np.float32)
+        params = np.array([[g.dx, g.dt, g.fm, g.dscale] for g in self.groups], np.float32)
         sx = np.concatenate([np.asarray(g.sx, np.int32) for g in self.groups]).astype(np.int32)
         sz = np.concatenate([np.asarray(g.sz, np.int32) for g in self.groups]).astype(np.int32)
         for g in self.groups:
@@ -182,9 +183,10 @@ class Batch:
 
 class Net:
     """The learned coarse-grid correction network (swfwi_net): 3x3 convolutions
-    1 -> 16 -> 16 -> 1, ReLU between, no biases; y = net(p) is added to p."""
+    2 -> 16 -> 16 -> 1, ReLU between, no biases; inputs p and r p (r = 1 /
+    vel_t); y = net(p) is added to p."""
 
-    NPARAMS = 2592
+    NPARAMS = 9 * 2 * 16 + 9 * 16 * 16 + 9 * 16
 
     def __init__(self, weights=None, seed=0, scale=None):
         h = _VP()
@@ -199,7 +201,7 @@ class Net:
         """He-style normal weights for the two ReLU layers, a small last layer
         (the network starts close to the identity correction y = 0)."""
         rng = np.random.default_rng(seed)
-        w1 = rng.standard_normal(9 * 16) * np.sqrt(2.0 / 9)
+        w1 = rng.standard_normal(9 * 2 * 16) * np.sqrt(2.0 / 18)
         w2 = rng.standard_normal(9 * 16 * 16) * np.sqrt(2.0 / (9 * 16))
         w3 = rng.standard_normal(9 * 16) * (scale if scale is not None else 1e-3)
         return np.concatenate([w1, w2, w3]).astype(np.float32)
@@ -221,26 +223,27 @@ class Net:
         check(load().swfwi_net_get_weights(self.h, w))
         return w
 
-    def apply(self, fields, region):
-        f = _f(fields)
+    def apply(self, fields, rv, region):
+        f, r = _f(fields), _f(rv)
         e, nx, nz = f.shape
+        assert r.shape == f.shape
         y = np.zeros_like(f)
-        check(load().swfwi_net_apply(self.h, f, nx, nz, e, np.ascontiguousarray(region, np.int32), y))
+        check(load().swfwi_net_apply(self.h, f, r, nx, nz, e, np.ascontiguousarray(region, np.int32), y))
         return y
 
-    def loss_grad(self, inp, target, region):
-        a, t = _f(inp), _f(target)
+    def loss_grad(self, inp, rv, target, region):
+        a, r, t = _f(inp), _f(rv), _f(target)
         e, nx, nz = a.shape
         g = np.zeros(self.NPARAMS, np.float32)
         loss = C.c_double(0)
-        check(load().swfwi_net_loss_grad(self.h, a, t, nx, nz, e, np.ascontiguousarray(region, np.int32),
+        check(load().swfwi_net_loss_grad(self.h, a, r, t, nx, nz, e, np.ascontiguousarray(region, np.int32),
                                          g.ctypes.data_as(C.c_void_p), C.byref(loss)))
         return loss.value, g
 
-    def set_data(self, inp, target, region):
-        a, t = _f(inp), _f(target)
+    def set_data(self, inp, rv, target, region):
+        a, r, t = _f(inp), _f(rv), _f(target)
         n, nx, nz = a.shape
-        check(load().swfwi_net_set_data(self.h, a, t, nx, nz, n, np.ascontiguousarray(region, np.int32)))
+        check(load().swfwi_net_set_data(self.h, a, r, t, nx, nz, n, np.ascontiguousarray(region, np.int32)))
 
     def train(self, batch, steps, lr, seed=0):
         losses = np.zeros(steps, np.float64)

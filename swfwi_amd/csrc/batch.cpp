@@ -72,7 +72,7 @@ void Batch::build() {
     const BatchGroup& G = g_[gi];
     const int ns = (int)G.sx.size();
     if (G.nx < 1 || G.nz < 1 || G.nb < 6 || G.nt < kSnapEvery || ns < 1 || G.ng < 1 || G.nmodels < 1 ||
-        G.sz.size() != G.sx.size() || !(G.dt > 0) || !(G.dx > 0))
+        G.sz.size() != G.sx.size() || !(G.dt > 0) || !(G.dx > 0) || !(G.dscale > 0))
       fail(kErrArg, "invalid batch group " + std::to_string(gi));
     const int nxpad = G.nx + 2 * G.nb + 2 * kHalo, nzpad = G.nz + G.nb + 2 * kHalo;
     const long long npts = (long long)nxpad * nzpad;
@@ -109,6 +109,7 @@ void Batch::build() {
         e.ntz = (nzpad - 2 * kHalo + 63) / 64;
         e.ntx = (nxpad - 2 * kHalo + 15) / 16;
         e.damp = bx0;
+        e.dscale = G.dscale;
         e.nt = G.nt;
         e.it_last = it_last;
         e.src = (G.sx[k] + bx0) * nzpad + G.sz[k] + bz0;
@@ -314,7 +315,11 @@ void Batch::set_correction(const ConvNet* net) {
   const long long stride = ent_.size() > 1 ? ent_[1].f - ent_[0].f : (long long)nxpad * nzpad;
   ycorr_.resize((size_t)stride * ent_.size());
   float* y = ycorr_.get();
-  NetGeom g{nxpad, nzpad, stride, (int)ent_.size(), bx0, nxpad - bx0, kHalo, nzpad - bx0};
+  // the models (1 / vel_t) of a single group lie model after model, the
+  // entries shot after shot of each model
+  const long long mstride = model_off_.size() > 1 ? model_off_[1] - model_off_[0] : 0;
+  NetGeom g{nxpad, nzpad, stride, (int)ent_.size(), bx0, nxpad - bx0, kHalo, nzpad - bx0,
+            rv_.get() + model_off_[0], mstride, (int)G.sx.size()};
   set_forward_hook([net, g, y](float* prev, float* cur, long long, int entries, hipStream_t s) {
     NetGeom h = g;
     h.entries = entries;

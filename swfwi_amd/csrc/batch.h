@@ -34,6 +34,10 @@ struct BatchGroup {
   std::vector<int> sx, sz;
   int gx0 = 0, ng = 0, gz = 0;
   int nmodels = 1;
+  // sponge strength: delta = 0.05 * dscale * dist^2 (1 = the reference's
+  // fd4t10s-damp-zjh.c:64-80; a grid stepping k times the reference's dt
+  // uses k for the same damping per unit time)
+  float dscale = 1.f;
 };
 
 // Device description of one entry (one shot of one model).
@@ -41,6 +45,7 @@ struct BatchEntry {
   int nxpad, nzpad;      // padded grid; z contiguous (ld = nzpad)
   int ntz, ntx;          // tiles over the update region [6, n-6)
   int damp;              // width of the sponge profile (bx0 = nb + 6)
+  float dscale;          // sponge strength factor
   int nt, it_last;       // forward steps; last backward iteration run
   int src;               // index of the source point
   int rec0, ng;          // first receiver point index, count (stride nzpad)

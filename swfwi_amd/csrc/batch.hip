@@ -153,7 +153,7 @@ __global__ __launch_bounds__(kBT) void k_batch_step(BatchKernelArgs a) {
     const float u = su[uc];
     const float r = a.rv[E.model + p];
     const float c = sc[(xl + kBH) * kSZ + zl + kBH];
-    const float d = damping(gx, gz, E.nxpad, E.nzpad, E.damp);
+    const float d = E.dscale * damping(gx, gz, E.nxpad, E.nzpad, E.damp);
     const float pv = nxt[p];
     float v = fmaf(2.f - 2.f * d + d * d, c, -(1.f - 2.f * d) * pv);
     v = fmaf(r, u, v);

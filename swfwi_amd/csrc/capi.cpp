@@ -688,9 +688,10 @@ int swfwi_batch_create(swfwi_batch* out, int ngroups, const int* dims, const flo
       G.ng = d[6];
       G.gz = d[7];
       G.nmodels = d[8];
-      G.dx = params[3 * g];
-      G.dt = params[3 * g + 1];
-      G.fm = params[3 * g + 2];
+      G.dx = params[4 * g];
+      G.dt = params[4 * g + 1];
+      G.fm = params[4 * g + 2];
+      G.dscale = params[4 * g + 3];
       if (ns < 1) fail(kErrArg, "a group needs at least one shot");
       G.sx.assign(sx + o, sx + o + ns);
       G.sz.assign(sz + o, sz + o + ns);
@@ -802,7 +803,8 @@ int swfwi_batch_set_correction(swfwi_batch b, swfwi_net net) {
 namespace {
 NetGeom net_geom(int nx, int nz, int entries, const int* region) {
   NEED(region);
-  NetGeom g{nx, nz, (long long)nx * nz, entries, region[0], region[1], region[2], region[3]};
+  NetGeom g{nx, nz, (long long)nx * nz, entries, region[0], region[1], region[2], region[3], nullptr, 0, 1};
+  g.rstride = g.stride;
   if (nx < 1 || nz < 1 || entries < 1) fail(kErrArg, "bad field stack");
   return g;
 }
@@ -839,15 +841,19 @@ int swfwi_net_get_weights(swfwi_net n, float* w) {
   });
 }
 
-int swfwi_net_apply(swfwi_net n, const float* fields, int nx, int nz, int entries, const int* region, float* y) {
+int swfwi_net_apply(swfwi_net n, const float* fields, const float* rv, int nx, int nz, int entries,
+                    const int* region, float* y) {
   return guarded([&] {
     NEED(n);
     NEED(fields);
+    NEED(rv);
     NEED(y);
-    const NetGeom g = net_geom(nx, nz, entries, region);
+    NetGeom g = net_geom(nx, nz, entries, region);
     hipStream_t s = n->s->get();
-    DBuf<float> p, out;
+    DBuf<float> p, r, out;
     p.upload(fields, (size_t)g.stride * entries, s);
+    r.upload(rv, (size_t)g.stride * entries, s);
+    g.r = r.get();
     out.resize((size_t)g.stride * entries);
     launch_fill(out.get(), g.stride * entries, 0.f, s);
     net_apply(p.get(), out.get(), g, n->n->dev(), s);
@@ -856,29 +862,33 @@ int swfwi_net_apply(swfwi_net n, const float* fields, int nx, int nz, int entrie
   });
 }
 
-int swfwi_net_loss_grad(swfwi_net n, const float* in, const float* target, int nx, int nz, int entries,
-                        const int* region, float* grad, double* loss) {
+int swfwi_net_loss_grad(swfwi_net n, const float* in, const float* rv, const float* target, int nx, int nz,
+                        int entries, const int* region, float* grad, double* loss) {
   return guarded([&] {
     NEED(n);
     NEED(in);
+    NEED(rv);
     NEED(target);
-    const NetGeom g = net_geom(nx, nz, entries, region);
+    NetGeom g = net_geom(nx, nz, entries, region);
     hipStream_t s = n->s->get();
-    DBuf<float> p, t;
+    DBuf<float> p, r, t;
     p.upload(in, (size_t)g.stride * entries, s);
+    r.upload(rv, (size_t)g.stride * entries, s);
     t.upload(target, (size_t)g.stride * entries, s);
+    g.r = r.get();
     const double l = n->n->loss_grad(p.get(), t.get(), g, grad, s);
     if (loss) *loss = l;
   });
 }
 
-int swfwi_net_set_data(swfwi_net n, const float* in, const float* target, int nx, int nz, int nsamples,
-                       const int* region) {
+int swfwi_net_set_data(swfwi_net n, const float* in, const float* rv, const float* target, int nx, int nz,
+                       int nsamples, const int* region) {
   return guarded([&] {
     NEED(n);
     NEED(in);
+    NEED(rv);
     NEED(target);
-    n->n->set_data(in, target, nsamples, net_geom(nx, nz, nsamples, region));
+    n->n->set_data(in, rv, target, nsamples, net_geom(nx, nz, nsamples, region));
   });
 }
 

@@ -1,16 +1,21 @@
 // Learned coarse-grid correction: a 3-layer 3x3 convolution network
-// (1 -> 16 -> 16 -> 1 channels, ReLU between, no biases, so that the
-// correction scales with the wavefield as the wave equation does) that maps
-// a coarse-grid pressure field p to a correction y, applied as p <- p + y.
-// The 16 -> 16 layer (2304 of the 2592 multiply-adds per point) runs on BF16
-// MFMA with FP32 accumulation (v_mfma_f32_16x16x16_bf16), the 1 -> 16 and
-// 16 -> 1 layers on FP32 vector ALUs.  Training kernels form the loss
-// gradient with respect to every weight (the 16 -> 16 products again on
-// BF16 MFMA) and update the weights with Adam.
+// (2 -> 16 -> 16 -> 1 channels, ReLU between, no biases) that maps a
+// coarse-grid pressure field p to a correction y, applied as p <- p + y.
+// The two input channels are p and r p, r = 1 / vel_t the local Courant
+// number squared: both scale with p, so the bias-free ReLU network is
+// positively homogeneous in the wavefield as the wave equation is linear,
+// and r tells it the local velocity the stencil's dispersion depends on.
+// Every layer runs on BF16 MFMA with FP32 accumulation
+// (v_mfma_f32_16x16x16_bf16): the inputs p, r p, the hidden activations and
+// the weights enter the products rounded to BF16, sums and the output y are
+// FP32.  Training kernels form the loss gradient with respect to every
+// weight (the 16 -> 16 products again on BF16 MFMA) and update the FP32
+// master weights with Adam.
 //
-// Weights, flattened (kNetParams floats): W1[t][c] (9 x 16), W2[t][ci][co]
-// (9 x 16 x 16), W3[t][ci] (9 x 16), tap t = (dx + 1) * 3 + (dz + 1) for the
-// neighbour at (x + dx, z + dz); the field is zero outside its array.
+// Weights, flattened (kNetParams floats): W1[t][k][c] (9 x 2 x 16, k = 0
+// for p, 1 for r p), W2[t][ci][co] (9 x 16 x 16), W3[t][ci] (9 x 16), tap
+// t = (dx + 1) * 3 + (dz + 1) for the neighbour at (x + dx, z + dz); the
+// fields are zero outside their array.
 #pragma once
 #include <cstdint>
 #include <vector>
@@ -20,7 +25,8 @@
 namespace swfwi {
 
 constexpr int kNetC = 16;
-constexpr int kNetW1 = 9 * kNetC;
+constexpr int kNetIn = 2;
+constexpr int kNetW1 = 9 * kNetIn * kNetC;
 constexpr int kNetW2 = 9 * kNetC * kNetC;
 constexpr int kNetW3 = 9 * kNetC;
 constexpr int kNetParams = kNetW1 + kNetW2 + kNetW3;
@@ -33,19 +39,23 @@ struct NetGeom {
   long long stride;
   int entries;
   int x0, x1, z0, z1;
+  const float* r;     // 1 / vel_t of entry e at r + (e / rdiv) * rstride, same layout
+  long long rstride;
+  int rdiv;
 };
 
-// Device weights in the layouts the kernels read: W1, W3 as FP32; W2 as
-// BF16 packed per MFMA lane for the forward product (k = ci, n = co) and for
-// the transposed one of the backward pass (k = co, n = ci).
+// Device weights in the layouts the kernels read: every layer's BF16 MFMA
+// operands packed per lane (W2 also transposed for the backward pass) and
+// W3 rounded to BF16, as FP32.
 struct NetDev {
-  const float* w1;
-  const uint16_t* w2f;
-  const uint16_t* w2b;
-  const float* w3;
+  const uint16_t* w1a;  // layer-1 A operands, [2][64 lanes][4]
+  const uint16_t* w2f;  // layer-2 A operands, [9][64][4] (item v of lane l: ci = 4 (l / 16) + v, co = l % 16)
+  const uint16_t* w2b;  // backward B operands, [9][64][4] (co = 4 (l / 16) + v, ci = l % 16)
+  const uint16_t* w3a;  // layer-3 A operands, [64][4]
+  const float* w3;      // W3 rounded to BF16
 };
 
-void net_pack(const float* theta, float* w1, uint16_t* w2f, uint16_t* w2b, float* w3, hipStream_t s);
+void net_pack(const float* theta, const NetDev& w, hipStream_t s);
 // y = net(p) over the region (y elsewhere untouched)
 void net_apply(const float* p, float* y, const NetGeom& g, const NetDev& w, hipStream_t s);
 // p += y over the region
@@ -71,7 +81,7 @@ class ConvNet {
   ConvNet();
   void set_weights(const float* host);
   void get_weights(float* host) const;
-  NetDev dev() const { return NetDev{w1_.get(), w2f_.get(), w2b_.get(), w3_.get()}; }
+  NetDev dev() const { return NetDev{w1a_.get(), w2f_.get(), w2b_.get(), w3a_.get(), w3_.get()}; }
   // p <- p + net(p) over the region, for a stack of fields (device)
   void correct(float* p, const NetGeom& g, float* yscratch, hipStream_t s) const;
   // One Adam step on the batch (inputs, targets on the device): returns the
@@ -82,16 +92,16 @@ class ConvNet {
   int steps() const { return step_; }
   // Training set on the device: inputs / targets [n][nx][nz] (one padded
   // coarse grid), the loss region in every sample.
-  void set_data(const float* in_host, const float* target_host, int n, const NetGeom& g);
+  void set_data(const float* in_host, const float* r_host, const float* target_host, int n, const NetGeom& g);
   // `steps` Adam steps over minibatches of `batch` samples drawn in a
   // seeded shuffled order; losses[k] = the loss before step k.
   void train(int batch, int steps, float lr, unsigned long long seed, double* losses, hipStream_t s);
 
  private:
   void repack(hipStream_t s);
-  DBuf<float> theta_, m_, v_, grad_, w1_, w3_;
-  DBuf<uint16_t> w2f_, w2b_;
-  DBuf<float> y_, dy_, parts_, din_, dtgt_, bin_, btgt_;
+  DBuf<float> theta_, m_, v_, grad_, w3_;
+  DBuf<uint16_t> w1a_, w2f_, w2b_, w3a_;
+  DBuf<float> y_, dy_, parts_, din_, dr_, dtgt_, bin_, br_, btgt_;
   NetGeom dg_{};
   int dn_ = 0;
   DBuf<double> lparts_, lsum_;

@@ -1,12 +1,20 @@
 // Kernels of the learned coarse-grid correction (learned.h).
 //
-// One block per 16 x 16 tile of output points (z fastest), 256 threads.
-// Forward, per tile: the field with a 3-cell halo (22 x 22) in LDS; layer 1
-// (1 -> 16, FP32 FMAs) over the tile + 2 (20 x 20), ReLU, stored as BF16;
-// layer 2 (16 -> 16) over the tile + 1 (18 x 18) as 9 tap products of
-// v_mfma_f32_16x16x16_bf16 per 16 points (A = 16 points x 16 input channels
-// of the tap's neighbours, B = that tap's 16 x 16 weights), ReLU, FP32 in LDS;
-// layer 3 (16 -> 1, FP32 FMAs) on the tile.
+// One block per 16 x 16 tile of output points (z fastest), 256 threads (4
+// waves).  Forward, per tile, every layer on v_mfma_f32_16x16x16_bf16 with
+// the 16 points of a group on the product's N side:
+//   stage   p and r p (BF16 pairs) with a 3-cell halo (22 x 22) in LDS;
+//   layer 1 D1[c][pt] = sum_k W1^T[c][k] X[k][pt], k = 2 t + (p | r p),
+//           over the tile + 2 (20 x 20): two products (K = 18 padded to 32),
+//           ReLU, BF16 into LDS;
+//   layer 2 D2[co][pt] = sum_t sum_ci W2_t^T[co][ci] h1[ci](pt + o_t) over
+//           the tile + 1 (18 x 18): nine products, ReLU, BF16;
+//   layer 3 U[t][pt] = sum_c W3[t][c] h2[c](pt) on the same 18 x 18 points,
+//           one product whose B operand is layer 2's accumulator as it
+//           stands (its D layout is the B layout), U into LDS;
+//   output  y(x) = sum_t U[t](x + o_t) on the tile.
+// So a 16-point group costs 2 + 9 + 1 products and h2 never leaves the
+// registers.
 //
 // v_mfma_f32_16x16x16_bf16 lane layout (measured on the device): lane l
 // supplies A[i = l%16][k = 4*(l/16) + v] and B[k = 4*(l/16) + v][j = l%16]
@@ -30,21 +38,30 @@ constexpr int kH1 = kTile + 4;  // layer-1 region (halo 2)
 constexpr int kH2 = kTile + 2;  // layer-2 region (halo 1)
 constexpr int kNH1 = kH1 * kH1;  // 400
 constexpr int kNH2 = kH2 * kH2;  // 324
+constexpr int kGroups1 = kNH1 / 16;         // 25 groups of 16 layer-1 points
 constexpr int kGroups2 = (kNH2 + 15) / 16;  // 21 groups of 16 layer-2 points
+static_assert(kNH1 % 16 == 0, "layer-1 points fill whole groups");
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ uint16_t bf16_of(float x) {  // round to nearest even
-  unsigned u = __float_as_uint(x);
+__device__ __host__ __forceinline__ uint16_t bf16_bits(unsigned u) {  // round to nearest even
   u += 0x7fffu + ((u >> 16) & 1u);
   return (uint16_t)(u >> 16);
 }
+__device__ __forceinline__ uint16_t bf16_of(float x) { return bf16_bits(__float_as_uint(x)); }
 __device__ __forceinline__ float f32_of(uint16_t h) { return __uint_as_float((unsigned)h << 16); }
 
 // accumulator item v of lane l holds D[row][col]
 __device__ __forceinline__ int d_row(int l, int v) { return 4 * (l >> 4) + v; }
 __device__ __forceinline__ int d_col(int l) { return l & 15; }
+
+__device__ __forceinline__ bf16x4 relu_bf16(const f32x4& a) {
+  bf16x4 r;
+#pragma unroll
+  for (int v = 0; v < 4; v++) r[v] = (short)bf16_of(fmaxf(a[v], 0.f));
+  return r;
+}
 
 struct TileMap {
   int e, ox, oz;  // entry, tile origin (global padded coords)
@@ -62,57 +79,72 @@ __device__ __forceinline__ TileMap tile_of(const NetGeom& g, int margin) {
   return m;
 }
 
-// Shared forward work of a tile: stage the field, layer 1 (BF16 out), layer
-// 2 on MFMA (FP32 out).  sp[kP*kP], h1[kNH1*16] (BF16), h2[kNH2*16].
-__device__ void forward_tile(const float* __restrict__ p, const NetGeom& g, const TileMap& m, const NetDev& w,
-                             float* sp, uint16_t* h1, float* h2) {
-  const int tid = threadIdx.x;
+// Shared forward work of a tile.  sx[kP*kP]: staged (p, r p) as BF16 pairs
+// (p in the low half); h1[kNH1*16]: layer 1 (BF16).  Layer 2 then goes to
+// h2[kNH2*16] (BF16) when h2 is given and through layer 3 into
+// u[9][kNH2] (FP32) when u is given.
+__device__ void forward_tile(const float* __restrict__ p, const float* __restrict__ r, const NetGeom& g,
+                             const TileMap& m, const NetDev& w, unsigned* sx, uint16_t* h1, uint16_t* h2, float* u) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int q = tid; q < kP * kP; q += kT) {
     const int x = m.ox - 3 + q / kP, z = m.oz - 3 + q % kP;
-    sp[q] = (x >= 0 && x < g.nx && z >= 0 && z < g.nz) ? p[(long long)x * g.nz + z] : 0.f;
+    const bool in = x >= 0 && x < g.nx && z >= 0 && z < g.nz;
+    const long long o = (long long)x * g.nz + z;
+    const float v = in ? p[o] : 0.f;
+    const float vq = in ? r[o] * v : 0.f;
+    sx[q] = (unsigned)bf16_of(v) | ((unsigned)bf16_of(vq) << 16);
+  }
+  const int kq = lane >> 4;  // this lane's K quarter
+  const bf16x4 a1a = *reinterpret_cast<const bf16x4*>(w.w1a + lane * 4);
+  const bf16x4 a1b = *reinterpret_cast<const bf16x4*>(w.w1a + (64 + lane) * 4);
+  __syncthreads();
+  // layer 1: lane's B items are (p, r p) at taps 2 kq and 2 kq + 1 (first
+  // product) and at tap 8 for kq = 0 (second product, the rest zero)
+  {
+    const int t0 = 2 * kq, t1 = 2 * kq + 1;
+    const int o0 = (t0 / 3 - 1) * kP + (t0 % 3 - 1), o1 = (t1 / 3 - 1) * kP + (t1 % 3 - 1);
+    const int o8 = kP + 1;
+    for (int grp = wave; grp < kGroups1; grp += kT / 64) {
+      const int q = grp * 16 + (lane & 15);
+      const int s = (q / kH1 + 1) * kP + (q % kH1 + 1);  // staged centre
+      union { unsigned u2[2]; bf16x4 b; } b0, b1;
+      b0.u2[0] = sx[s + o0];
+      b0.u2[1] = sx[s + o1];
+      b1.u2[0] = kq == 0 ? sx[s + o8] : 0u;
+      b1.u2[1] = 0u;
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1a, b0.b, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1b, b1.b, acc, 0, 0, 0);
+      *reinterpret_cast<bf16x4*>(h1 + q * kNetC + 4 * kq) = relu_bf16(acc);  // channels 4 kq .. 4 kq + 3
+    }
   }
   __syncthreads();
-  // layer 1: W1[t][c] read through the scalar cache (wave-uniform indices)
-  for (int q = tid; q < kNH1; q += kT) {
-    const int x = q / kH1, z = q % kH1;  // region coords; staged = +1
-    float a[kNetC];
+  // layers 2 and 3
+  bf16x4 a2[9];
 #pragma unroll
-    for (int c = 0; c < kNetC; c++) a[c] = 0.f;
-#pragma unroll
-    for (int dx = -1; dx <= 1; dx++)
-#pragma unroll
-      for (int dz = -1; dz <= 1; dz++) {
-        const float v = sp[(x + 1 + dx) * kP + (z + 1 + dz)];
-        const int t = (dx + 1) * 3 + (dz + 1);
-#pragma unroll
-        for (int c = 0; c < kNetC; c++) a[c] = fmaf(w.w1[t * kNetC + c], v, a[c]);
-      }
-#pragma unroll
-    for (int c = 0; c < kNetC; c++) h1[q * kNetC + c] = bf16_of(fmaxf(a[c], 0.f));
-  }
-  __syncthreads();
-  // layer 2 on MFMA: wave w takes groups w, w+4, ...
-  const int lane = tid & 63, wave = tid >> 6;
-  bf16x4 b[9];
-#pragma unroll
-  for (int t = 0; t < 9; t++) b[t] = *reinterpret_cast<const bf16x4*>(w.w2f + (t * 64 + lane) * 4);
+  for (int t = 0; t < 9; t++) a2[t] = *reinterpret_cast<const bf16x4*>(w.w2f + (t * 64 + lane) * 4);
+  const bf16x4 a3 = *reinterpret_cast<const bf16x4*>(w.w3a + lane * 4);
   for (int grp = wave; grp < kGroups2; grp += kT / 64) {
-    const int qa = grp * 16 + (lane & 15);          // this lane's A row (a layer-2 point)
-    const int qs = qa < kNH2 ? qa : kNH2 - 1;       // rows past the region read a valid point
-    const int x = qs / kH2, z = qs % kH2;           // layer-2 coords; layer-1 = +1
-    const int kofs = 4 * (lane >> 4);
+    const int qa = grp * 16 + (lane & 15);     // this lane's point
+    const int qs = qa < kNH2 ? qa : kNH2 - 1;  // points past the region read a valid one
+    const int x = qs / kH2, z = qs % kH2;      // layer-2 coords; layer-1 = +1
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int t = 0; t < 9; t++) {
-      const int dx = t / 3 - 1, dz = t % 3 - 1;
-      const int q1 = (x + 1 + dx) * kH1 + (z + 1 + dz);
-      const bf16x4 a = *reinterpret_cast<const bf16x4*>(h1 + q1 * kNetC + kofs);
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[t], acc, 0, 0, 0);
+      const int q1 = (x + 1 + t / 3 - 1) * kH1 + (z + 1 + t % 3 - 1);
+      const bf16x4 b = *reinterpret_cast<const bf16x4*>(h1 + q1 * kNetC + 4 * kq);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[t], b, acc, 0, 0, 0);
     }
+    const bf16x4 hb = relu_bf16(acc);  // h2[4 kq + v][qa]
+    if (h2 && qa < kNH2) *reinterpret_cast<bf16x4*>(h2 + qa * kNetC + 4 * kq) = hb;
+    if (u) {
+      f32x4 y3 = {0.f, 0.f, 0.f, 0.f};
+      y3 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a3, hb, y3, 0, 0, 0);  // U[4 kq + v][qa]
 #pragma unroll
-    for (int v = 0; v < 4; v++) {
-      const int q2 = grp * 16 + d_row(lane, v);
-      if (q2 < kNH2) h2[q2 * kNetC + d_col(lane)] = fmaxf(acc[v], 0.f);
+      for (int v = 0; v < 4; v++) {
+        const int t = 4 * kq + v;
+        if (t < 9 && qa < kNH2) u[t * kNH2 + qa] = y3[v];
+      }
     }
   }
   __syncthreads();
@@ -120,23 +152,19 @@ __device__ void forward_tile(const float* __restrict__ p, const NetGeom& g, cons
 
 __global__ __launch_bounds__(kT) void k_net_apply(const float* __restrict__ p_all, float* __restrict__ y_all,
                                                   NetGeom g, NetDev w) {
-  __shared__ float sp[kP * kP];
+  __shared__ unsigned sx[kP * kP];
   __shared__ uint16_t h1[kNH1 * kNetC];
-  __shared__ float h2[kNH2 * kNetC];
+  __shared__ float u[9 * kNH2];
   const TileMap m = tile_of(g, 0);
   const float* p = p_all + (long long)m.e * g.stride;
-  forward_tile(p, g, m, w, sp, h1, h2);
+  const float* r = g.r + (long long)(m.e / g.rdiv) * g.rstride;
+  forward_tile(p, r, g, m, w, sx, h1, nullptr, u);
   const int x = threadIdx.x / kTile, z = threadIdx.x % kTile;
   const int gx = m.ox + x, gz = m.oz + z;
   if (gx >= g.x1 || gz >= g.z1) return;
   float y = 0.f;
 #pragma unroll
-  for (int t = 0; t < 9; t++) {
-    const int dx = t / 3 - 1, dz = t % 3 - 1;
-    const float* hv = h2 + ((x + 1 + dx) * kH2 + (z + 1 + dz)) * kNetC;
-#pragma unroll
-    for (int c = 0; c < kNetC; c++) y = fmaf(w.w3[t * kNetC + c], hv[c], y);
-  }
+  for (int t = 0; t < 9; t++) y += u[t * kNH2 + (x + 1 + t / 3 - 1) * kH2 + (z + 1 + t % 3 - 1)];
   y_all[(long long)m.e * g.stride + (long long)gx * g.nz + gz] = y;
 }
 
@@ -186,22 +214,23 @@ __global__ void k_net_loss(const float* p, const float* y, const float* t, float
 //   dz2[c](x') = relu'(z2) sum_t W3[t][c] dy(x' - o_t)        (tile + 1)
 //   dW2[t][ci][co] = sum_tile h1[ci](x + o_t) dz2[co](x)        (MFMA, K = tile points)
 //   dz1[ci](x) = relu'(z1) sum_t sum_co W2[t][ci][co] dz2[co](x - o_t)   (MFMA)
-//   dW1[t][c] = sum_tile p(x + o_t) dz1[c](x)
-// into parts[block][kNetParams].
+//   dW1[t][k][c] = sum_tile in_k(x + o_t) dz1[c](x), in_0 = p, in_1 = r p
+// into parts[block][kNetParams].  The forward values enter as the forward
+// products saw them: inputs, h1, h2 and W3 rounded to BF16.
 __global__ __launch_bounds__(kT) void k_net_backward(const float* __restrict__ p_all, const float* __restrict__ dy_all,
                                                      NetGeom g, NetDev w, float* __restrict__ parts) {
-  __shared__ float sp[kP * kP];
+  __shared__ unsigned sx[kP * kP];
   __shared__ uint16_t h1[kNH1 * kNetC];
-  __shared__ float h2[kNH2 * kNetC];
-  __shared__ float sdy[kNH1];                   // dy over the tile + 2
-  __shared__ uint16_t dz2[kNH2 * kNetC];        // BF16 (MFMA operand)
-  __shared__ float dz1[kTile * kTile * kNetC];  // tile points
+  __shared__ uint16_t h2[kNH2 * kNetC];
+  __shared__ float sdy[kNH1];                 // dy over the tile + 2
+  __shared__ uint16_t dz2[kNH2 * kNetC];      // BF16 (MFMA operand)
+  __shared__ float dz1[kTile * kTile * kNetC];
   // tiles cover the region grown by 2: dz2 is non-zero up to 1 and dz1 up
   // to 2 cells outside the region, and every point's terms belong to the
   // one tile that owns it
   const TileMap m = tile_of(g, 2);
   const long long eo = (long long)m.e * g.stride;
-  forward_tile(p_all + eo, g, m, w, sp, h1, h2);
+  forward_tile(p_all + eo, g.r + (long long)(m.e / g.rdiv) * g.rstride, g, m, w, sx, h1, h2, nullptr);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int q = tid; q < kNH1; q += kT) {
     const int x = m.ox - 2 + q / kH1, z = m.oz - 2 + q % kH1;
@@ -221,7 +250,7 @@ __global__ __launch_bounds__(kT) void k_net_backward(const float* __restrict__ p
         const int dx = t / 3 - 1, dz = t % 3 - 1;
         a = fmaf(w.w3[t * kNetC + c], sdy[(x + 1 - dx) * kH1 + (z + 1 - dz)], a);
       }
-      dz2[q * kNetC + c] = bf16_of(h2[q * kNetC + c] > 0.f ? a : 0.f);
+      dz2[q * kNetC + c] = bf16_of(f32_of(h2[q * kNetC + c]) > 0.f ? a : 0.f);
     }
   }
   // dW3: 144 sums over the tile's 256 points, one per thread
@@ -231,7 +260,7 @@ __global__ __launch_bounds__(kT) void k_net_backward(const float* __restrict__ p
     float a = 0.f;
     for (int x = 0; x < kTile; x++)
       for (int z = 0; z < kTile; z++)
-        a = fmaf(sdy[(x + 2) * kH1 + (z + 2)], h2[((x + 1 + dx) * kH2 + (z + 1 + dz)) * kNetC + c], a);
+        a = fmaf(sdy[(x + 2) * kH1 + (z + 2)], f32_of(h2[((x + 1 + dx) * kH2 + (z + 1 + dz)) * kNetC + c]), a);
     out[kNetW1 + kNetW2 + tid] = a;
   }
   __syncthreads();
@@ -280,14 +309,18 @@ __global__ __launch_bounds__(kT) void k_net_backward(const float* __restrict__ p
     }
   }
   __syncthreads();
-  // dW1: 144 sums over the tile, one per thread
-  if (tid < kNetW1) {
-    const int t = tid / kNetC, c = tid % kNetC;
+  // dW1: 288 sums over the tile
+  for (int j = tid; j < kNetW1; j += kT) {
+    const int t = j / (kNetIn * kNetC), k = (j / kNetC) % kNetIn, c = j % kNetC;
     const int dx = t / 3 - 1, dz = t % 3 - 1;
+    const int sh = 16 * k;  // p in the low half, r p in the high half
     float a = 0.f;
     for (int x = 0; x < kTile; x++)
-      for (int z = 0; z < kTile; z++) a = fmaf(sp[(x + 3 + dx) * kP + (z + 3 + dz)], dz1[(x * kTile + z) * kNetC + c], a);
-    out[tid] = a;
+      for (int z = 0; z < kTile; z++) {
+        const float in = f32_of((uint16_t)(sx[(x + 3 + dx) * kP + (z + 3 + dz)] >> sh));
+        a = fmaf(in, dz1[(x * kTile + z) * kNetC + c], a);
+      }
+    out[j] = a;
   }
 }
 
@@ -300,10 +333,22 @@ __global__ void k_net_reduce(const float* parts, long long nblocks, float* grad)
   grad[j] = (float)s;
 }
 
-__global__ void k_net_pack(const float* theta, float* w1, uint16_t* w2f, uint16_t* w2b, float* w3) {
+__global__ void k_net_pack(const float* theta, uint16_t* w1a, uint16_t* w2f, uint16_t* w2b, uint16_t* w3a,
+                           float* w3) {
   const int i = blockIdx.x * kT + threadIdx.x;
-  if (i < kNetW1) w1[i] = theta[i];
-  if (i < kNetW3) w3[i] = theta[kNetW1 + kNetW2 + i];
+  if (i < kNetW3) w3[i] = f32_of(bf16_of(theta[kNetW1 + kNetW2 + i]));
+  if (i < 2 * 64 * 4) {
+    // layer 1, product h: A[c = l%16][k = 16 h + 4*(l/16) + v] = W1 row k (= 2 t + input) or 0 past k = 17
+    const int h = i / 256, l = (i / 4) % 64, v = i % 4;
+    const int k = 16 * h + 4 * (l >> 4) + v, c = l & 15;
+    w1a[i] = k < 9 * kNetIn ? bf16_of(theta[k * kNetC + c]) : (uint16_t)0;
+  }
+  if (i < 64 * 4) {
+    // layer 3: A[t = l%16][c = 4*(l/16) + v] = W3[t][c], rows past tap 8 zero
+    const int l = i / 4, v = i % 4;
+    const int t = l & 15, c = 4 * (l >> 4) + v;
+    w3a[i] = t < 9 ? bf16_of(theta[kNetW1 + kNetW2 + t * kNetC + c]) : (uint16_t)0;
+  }
   if (i < 9 * 64 * 4) {
     // item v of lane l for tap t: k = 4*(l/16) + v, n = l%16
     const int t = i / 256, l = (i / 4) % 64, v = i % 4;
@@ -343,6 +388,7 @@ long long tiles_of(const NetGeom& g, int margin) {
 // the staged field reaches 3 cells beyond the tiles, which reach `margin`
 // beyond the region
 void check_geom(const NetGeom& g, int margin) {
+  if (!g.r || g.rdiv < 1) fail(kErrArg, "the network needs the model (1 / vel_t) of every field");
   const int r = 3 + margin;
   if (g.x0 < r || g.z0 < r || g.x1 > g.nx - r || g.z1 > g.nz - r || g.x1 <= g.x0 || g.z1 <= g.z0 || g.entries < 1)
     fail(kErrArg, "network region must lie 5 cells inside the field");
@@ -351,8 +397,10 @@ void check_geom(const NetGeom& g, int margin) {
 
 }  // namespace
 
-void net_pack(const float* theta, float* w1, uint16_t* w2f, uint16_t* w2b, float* w3, hipStream_t s) {
-  hipLaunchKernelGGL(k_net_pack, dim3((9 * 64 * 4 + kT - 1) / kT), dim3(kT), 0, s, theta, w1, w2f, w2b, w3);
+void net_pack(const float* theta, const NetDev& w, hipStream_t s) {
+  hipLaunchKernelGGL(k_net_pack, dim3((9 * 64 * 4 + kT - 1) / kT), dim3(kT), 0, s, theta, const_cast<uint16_t*>(w.w1a),
+                     const_cast<uint16_t*>(w.w2f), const_cast<uint16_t*>(w.w2b), const_cast<uint16_t*>(w.w3a),
+                     const_cast<float*>(w.w3));
 }
 
 void net_apply(const float* p, float* y, const NetGeom& g, const NetDev& w, hipStream_t s) {
@@ -398,15 +446,16 @@ ConvNet::ConvNet() {
   m_.resize(kNetParams);
   v_.resize(kNetParams);
   grad_.resize(kNetParams);
-  w1_.resize(kNetW1);
   w3_.resize(kNetW3);
+  w1a_.resize(2 * 64 * 4);
   w2f_.resize(9 * 64 * 4);
   w2b_.resize(9 * 64 * 4);
+  w3a_.resize(64 * 4);
   lparts_.resize(1024);
   lsum_.resize(1);
 }
 
-void ConvNet::repack(hipStream_t s) { net_pack(theta_.get(), w1_.get(), w2f_.get(), w2b_.get(), w3_.get(), s); }
+void ConvNet::repack(hipStream_t s) { net_pack(theta_.get(), dev(), s); }
 
 void ConvNet::set_weights(const float* host) {
   SWFWI_HIP(hipMemcpy(theta_.get(), host, sizeof(float) * kNetParams, hipMemcpyHostToDevice));
@@ -452,7 +501,8 @@ double ConvNet::train_step(const float* in, const float* target, const NetGeom& 
   return loss;
 }
 
-void ConvNet::set_data(const float* in_host, const float* target_host, int n, const NetGeom& g) {
+void ConvNet::set_data(const float* in_host, const float* r_host, const float* target_host, int n,
+                       const NetGeom& g) {
   if (n < 1) fail(kErrArg, "empty training set");
   dg_ = g;
   dg_.entries = 1;
@@ -460,8 +510,10 @@ void ConvNet::set_data(const float* in_host, const float* target_host, int n, co
   dn_ = n;
   const size_t len = (size_t)n * dg_.stride;
   din_.resize(len);
+  dr_.resize(len);
   dtgt_.resize(len);
   SWFWI_HIP(hipMemcpy(din_.get(), in_host, sizeof(float) * len, hipMemcpyHostToDevice));
+  SWFWI_HIP(hipMemcpy(dr_.get(), r_host, sizeof(float) * len, hipMemcpyHostToDevice));
   SWFWI_HIP(hipMemcpy(dtgt_.get(), target_host, sizeof(float) * len, hipMemcpyHostToDevice));
 }
 
@@ -471,7 +523,11 @@ void ConvNet::train(int batch, int steps, float lr, unsigned long long seed, dou
   NetGeom g = dg_;
   g.entries = batch;
   bin_.resize((size_t)batch * g.stride);
+  br_.resize((size_t)batch * g.stride);
   btgt_.resize((size_t)batch * g.stride);
+  g.r = br_.get();
+  g.rstride = g.stride;
+  g.rdiv = 1;
   // seeded shuffle of the sample order (splitmix64 + Fisher-Yates), refreshed each pass
   std::vector<int> order(dn_);
   for (int i = 0; i < dn_; i++) order[i] = i;
@@ -494,6 +550,8 @@ void ConvNet::train(int batch, int steps, float lr, unsigned long long seed, dou
       SWFWI_HIP(hipMemcpyAsync(bin_.get() + (size_t)b * g.stride, din_.get() + (size_t)smp * g.stride,
                                sizeof(float) * g.stride, hipMemcpyDeviceToDevice, s));
       SWFWI_HIP(hipMemcpyAsync(btgt_.get() + (size_t)b * g.stride, dtgt_.get() + (size_t)smp * g.stride,
+                               sizeof(float) * g.stride, hipMemcpyDeviceToDevice, s));
+      SWFWI_HIP(hipMemcpyAsync(br_.get() + (size_t)b * g.stride, dr_.get() + (size_t)smp * g.stride,
                                sizeof(float) * g.stride, hipMemcpyDeviceToDevice, s));
     }
     const double l = train_step(bin_.get(), btgt_.get(), g, lr, s);

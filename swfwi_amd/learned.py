@@ -34,6 +34,17 @@ def config4_case():
 OZ = 1
 
 
+# Whether the batches this module creates replay their time loops as HIP
+# graphs (the default) or launch every step directly (for kernel tracers).
+GRAPHS = True
+
+
+def _batch(grid, prec, order):
+    b = Batch([Group(**grid)], snap_prec=prec, order=order)
+    b.set_graphs(GRAPHS)
+    return b
+
+
 def grids(case, shots=None):
     """(fine, coarse) Group keyword dicts of one shot per model.  shots: fine
     physical source x positions (even; default the centre); the sources sit
@@ -48,8 +59,11 @@ def grids(case, shots=None):
     cx, cz = (nx + 1) // 2, (nz + 1 - OZ) // 2
     fine = dict(nx=nx, nz=nz, nb=nb, dx=case["dx"], dt=case["dt"], fm=case["fm"], nt=case["nt"], sx=sx,
                 sz=np.full(sx.size, 2 + OZ, np.int32), gx0=0, ng=nx, gz=OZ, nmodels=1)
+    # the coarse sponge: half the cells, twice the step, so twice the per-step
+    # damping for the same attenuation per unit time
     coarse = dict(nx=cx, nz=cz, nb=nb // 2, dx=2 * case["dx"], dt=2 * case["dt"], fm=case["fm"],
-                  nt=case["nt"] // 2, sx=sx // 2, sz=np.full(sx.size, 1, np.int32), gx0=0, ng=cx, gz=0, nmodels=1)
+                  nt=case["nt"] // 2, sx=sx // 2, sz=np.full(sx.size, 1, np.int32), gx0=0, ng=cx, gz=0, nmodels=1,
+                  dscale=2.0)
     return fine, coarse
 
 
@@ -90,7 +104,7 @@ def training_pairs(case, models, times):
     fine seismograms [model][nt][ng]."""
     fine, coarse = grids(case)
     n = len(models)
-    fb = Batch([Group(**with_models(fine, n))], snap_prec=capi.SNAP_NONE, order=capi.ORDER_10)
+    fb = _batch(with_models(fine, n), capi.SNAP_NONE, capi.ORDER_10)
     for m in range(n):
         fb.bind_model(0, m, expand(models[m], fine))
     taps = []
@@ -100,7 +114,7 @@ def training_pairs(case, models, times):
     dfine = np.stack(fb.split_data(fb.forward()))  # [model][nt][ng]
     T = np.stack(fb.taps())  # [model][tap][x][z]
     fb.close()
-    cb = Batch([Group(**with_models(coarse, n))], snap_prec=capi.SNAP_NONE, order=capi.ORDER_4)
+    cb = _batch(with_models(coarse, n), capi.SNAP_NONE, capi.ORDER_4)
     for m in range(n):
         cb.bind_model(0, m, expand(restrict(models[m]), coarse))
     ins, tgts = [], []
@@ -110,7 +124,9 @@ def training_pairs(case, models, times):
         ins += [prev.reshape((n,) + shape), cur.reshape((n,) + shape)]
         tgts += [T[:, 4 * k + 2], T[:, 4 * k + 3]]
     cb.close()
-    return (np.ascontiguousarray(np.concatenate(ins), np.float32),
+    rv = np.stack([1.0 / expand(restrict(models[m]), coarse) for m in range(n)]).astype(np.float32)
+    rvs = np.concatenate([rv] * (2 * len(times)))
+    return (np.ascontiguousarray(np.concatenate(ins), np.float32), np.ascontiguousarray(rvs),
             np.ascontiguousarray(np.concatenate(tgts), np.float32), dfine)
 
 
@@ -119,7 +135,7 @@ def coarse_seismograms(case, models, net=None):
     given): [model][nt'][ng']."""
     _, coarse = grids(case)
     n = len(models)
-    cb = Batch([Group(**with_models(coarse, n))], snap_prec=capi.SNAP_NONE, order=capi.ORDER_4)
+    cb = _batch(with_models(coarse, n), capi.SNAP_NONE, capi.ORDER_4)
     for m in range(n):
         cb.bind_model(0, m, expand(restrict(models[m]), coarse))
     if net is not None:
@@ -139,31 +155,137 @@ def seis_error(dc, df):
     return float(np.linalg.norm(dc - df) / np.linalg.norm(df))
 
 
-def train_and_evaluate(case, n_train, train_steps, seed=0, times=None, batch=16, lr=2e-3, verbose=False):
-    """Train a network on n_train random models and compare the coarse
-    propagation with and without it against the fine propagation on one
-    held-out model."""
+def rollout_pairs(case, models, starts, rounds, net=None):
+    """Training pairs seen along the corrected propagation.  For every model
+    and start step j (j % 4 == 3, so that a correction falls after every
+    4th step as in the forward runs) and round r < rounds: the coarse run
+    from the fine wavefield at (j-1, j) is corrected by `net` for 4r steps
+    (none for r = 0), then 4 plain steps give the input; the fine wavefield
+    at (j+4r+3, j+4r+4) is the target.  Returns inputs, their 1 / vel_t,
+    targets ([2 * models * starts * rounds][nxpad'][nzpad']) and the fine
+    seismograms."""
+    fine, coarse = grids(case)
+    n = len(models)
+    if any(j % 4 != 3 for j in starts):
+        raise ValueError("rollout starts must be 3 mod 4")
+    fb = _batch(with_models(fine, n), capi.SNAP_NONE, capi.ORDER_10)
+    for m in range(n):
+        fb.bind_model(0, m, expand(models[m], fine))
+    taps, slot = [], {}
+    for j in starts:
+        for k in [j - 1, j] + [j + 4 * r + d for r in range(rounds) for d in (3, 4)]:
+            if k not in slot:
+                slot[k] = len(taps)
+                taps.append(2 * k)
+    fb.set_taps(taps, oz=OZ)
+    dfine = np.stack(fb.split_data(fb.forward()))
+    T = np.stack(fb.taps())  # [model][tap][x][z]
+    fb.close()
+    cb = _batch(with_models(coarse, n), capi.SNAP_NONE, capi.ORDER_4)
+    for m in range(n):
+        cb.bind_model(0, m, expand(restrict(models[m]), coarse))
+    shape = T.shape[2:]
+    ins, tgts = [], []
+    for r in range(rounds):
+        for j in starts:
+            prev, cur = T[:, slot[j - 1]], T[:, slot[j]]
+            if r > 0:
+                cb.set_correction(net)
+                prev, cur = cb.step_from(prev, cur, it0=j, nsteps=4 * r)
+                cb.set_correction(None)
+            prev, cur = cb.step_from(prev, cur, it0=j + 4 * r, nsteps=4)
+            ins += [prev.reshape((n,) + shape), cur.reshape((n,) + shape)]
+            tgts += [T[:, slot[j + 4 * r + 3]], T[:, slot[j + 4 * r + 4]]]
+    cb.close()
+    rv = np.stack([1.0 / expand(restrict(models[m]), coarse) for m in range(n)]).astype(np.float32)
+    rvs = np.concatenate([rv] * (len(ins)))[:len(ins) * n]
+    return (np.ascontiguousarray(np.concatenate(ins), np.float32), np.ascontiguousarray(rvs, np.float32),
+            np.ascontiguousarray(np.concatenate(tgts), np.float32), dfine)
+
+
+def _noisy(ins, sigma, copies, rng):
+    """`copies` copies of the inputs, each with white noise of `sigma` times
+    the input's own RMS added (the targets stay clean, so the network also
+    learns to take such perturbations out again: the corrected propagation
+    stays stable where it meets states the clean pairs never showed)."""
+    rms = np.sqrt(np.mean(np.square(ins, dtype=np.float64), axis=(1, 2)))[:, None, None]
+    return np.ascontiguousarray(np.concatenate(
+        [ins + (sigma * rms * rng.standard_normal(ins.shape)).astype(np.float32) for _ in range(copies)]), np.float32)
+
+
+def train(case, models, rounds=3, steps_per_round=3000, starts=None, batch=16, lr=1e-3, noise=0.02, copies=2,
+          seed=0, net=None, log=None):
+    """Train the correction on the given fine models.  Round 0 uses pairs
+    from the fine wavefield; each later round r adds the pairs met along the
+    current network's own corrected propagation (4 r corrected steps, then
+    4 plain ones) and trains on everything gathered so far.  Every input is
+    used in `copies` noisy copies (`noise` times its RMS).  Returns (net,
+    losses per round)."""
     _, coarse = grids(case)
     nt_c = case["nt"] // 2
-    if times is None:
-        times = list(range(nt_c // 6, nt_c - 8, max(nt_c // 12, 1)))
-    models = random_models(case, n_train + 1, seed)
-    ins, tgts, dfine = training_pairs(case, models[:n_train], times)
+    if starts is None:
+        starts = list(range(11, nt_c - 4 * rounds - 8, 8))  # all 3 mod 4
     reg = region(coarse)
-    net = Net(Net.initial(seed=seed, scale=1e-3))
-    net.set_data(ins, tgts, reg)
-    base = net.loss_grad(ins[:min(len(ins), 64)], tgts[:min(len(ins), 64)], reg)[0]
-    losses = net.train(batch=min(batch, len(ins)), steps=train_steps, lr=lr, seed=seed)
-    if verbose:
-        print(f"training: {len(ins)} pairs, loss {losses[0]:.3e} -> {losses[-10:].mean():.3e}", flush=True)
-    held = models[n_train:]
-    _, _, dfh = training_pairs(case, held, times[:1])
+    rng = np.random.default_rng(seed + 7)
+    net = net or Net(Net.initial(seed=seed, scale=1e-3))
+    data = None
+    hist = []
+    for r in range(rounds):
+        ins, rvs, tgts, _ = rollout_pairs(case, models, starts, r + 1, net if r else None)
+        # keep only the newest round's pairs from this call (earlier ones repeat)
+        per = 2 * len(models) * len(starts)
+        new = (_noisy(ins[-per:], noise, copies, rng), np.concatenate([rvs[-per:]] * copies),
+               np.concatenate([tgts[-per:]] * copies))
+        data = new if data is None else tuple(np.concatenate([a, b]) for a, b in zip(data, new))
+        data = tuple(np.ascontiguousarray(a, np.float32) for a in data)
+        net.set_data(*data, reg)
+        losses = net.train(batch=min(batch, len(data[0])), steps=steps_per_round, lr=lr, seed=seed + r)
+        hist.append(losses)
+        if log:
+            log(f"round {r}: {len(data[0])} pairs, loss {losses[0]:.3e} -> {losses[-50:].mean():.3e}")
+    return net, hist
+
+
+def with_gain(net, gain):
+    """The network with its output scaled by `gain` (the last layer's
+    weights times gain; the network is linear in them)."""
+    w = net.weights()
+    w[-9 * 16:] *= gain
+    return Net(w)
+
+
+def held_out_errors(case, models, nets):
+    """Seismogram relative L2 errors against the fine propagation, at the
+    coarse samples and receivers, of the plain coarse propagation and of the
+    coarse propagation with each network of `nets`."""
+    _, _, _, dfh = rollout_pairs(case, models, [11], 1)
     ref = fine_on_coarse(dfh)
-    plain = coarse_seismograms(case, held)
-    corr = coarse_seismograms(case, held, net)
-    out = {"err_plain": seis_error(plain, ref), "err_learned": seis_error(corr, ref), "losses": losses,
-           "pairs": len(ins), "base_loss": base, "net": net}
-    return out
+    plain = coarse_seismograms(case, models)
+    return seis_error(plain, ref), [seis_error(coarse_seismograms(case, models, n), ref) for n in nets], ref
+
+
+def train_and_evaluate(case, n_train, n_val=2, n_test=2, gains=(1.0, 0.5, 0.25), seed=0, verbose=False, **kw):
+    """Train a network on n_train random models, pick its output gain on
+    n_val validation models (the one whose corrected coarse seismograms are
+    closest to the fine ones) and compare the coarse propagation with and
+    without it against the fine propagation on n_test further held-out
+    models (seismogram relative L2 error at the coarse samples)."""
+    models = random_models(case, n_train + n_val + n_test, seed)
+    net, hist = train(case, models[:n_train], seed=seed, log=(lambda m: print(m, flush=True)) if verbose else None,
+                      **kw)
+    cands = [with_gain(net, g) for g in gains]
+    val_plain, val_errs, _ = held_out_errors(case, models[n_train:n_train + n_val], cands)
+    k = int(np.argmin(val_errs))
+    test = models[n_train + n_val:]
+    err_plain, (err_learned,), ref = held_out_errors(case, test, [cands[k]])
+    for i, c in enumerate(cands):
+        if i != k:
+            c.close()
+    if verbose:
+        print(f"validation: plain {val_plain:.4f}, gains {dict(zip(gains, np.round(val_errs, 4)))}; "
+              f"test: plain {err_plain:.4f} learned {err_learned:.4f}", flush=True)
+    return {"err_plain": err_plain, "err_learned": err_learned, "gain": gains[k], "val_plain": val_plain,
+            "val_errs": val_errs, "losses": hist, "net": cands[k]}
 
 
 # ---- FWI with the plain fine propagator and with the learned coarse one ----
@@ -194,14 +316,15 @@ def fwi_run(grid, order, v0, dobs, iters, vmin, vmax, net=None, dv=30.0, snap_pr
     gradient, a two-trial parabolic step on the summed objective, clamped
     update of the transformed model and boundary refill.  dobs: [shot][nt][ng]
     on this grid's receivers and samples.  Returns the final physical model,
-    the objectives and the gradient time per shot (forward + backward on the
-    device, ms)."""
+    the objectives, the gradient time per shot (forward + backward on the
+    device, ms; median over the iterations) and the stencil point updates of
+    one gradient of all shots."""
     prec = capi.SNAP_BF16 if snap_prec is None else snap_prec
     S = len(grid["sx"])
-    b = Batch([Group(**grid)], snap_prec=prec, order=order)
+    b = _batch(grid, prec, order)
     if net is not None:
         b.set_correction(net)
-    tr = Batch([Group(**grid)], snap_prec=capi.SNAP_NONE, order=order)  # trial forward runs
+    tr = _batch(grid, capi.SNAP_NONE, order)  # trial forward runs
     if net is not None:
         tr.set_correction(net)
     dx, dt = grid["dx"], grid["dt"]
@@ -214,7 +337,7 @@ def fwi_run(grid, order, v0, dobs, iters, vmin, vmax, net=None, dv=30.0, snap_pr
         d = batch.forward()
         return float(np.sum((obs.astype(np.float64) - d) ** 2)), d
 
-    objs, gms = [], []
+    objs, gms, pts = [], [], 0.0
     g_prev = d_prev = None
     for it in range(iters):
         b.bind_model(0, 0, vt)
@@ -224,6 +347,7 @@ def fwi_run(grid, order, v0, dobs, iters, vmin, vmax, net=None, dv=30.0, snap_pr
         _, gm = b.backward(None, want_entries=False)
         info = b.info()
         gms.append((info["forward_ms"] + info["backward_ms"]) / S)
+        pts = info["forward_points"] + 2 * info["backward_points"]  # the backward advances two fields
         g = _mask_scale(gm.reshape(vt.shape), grid)
         if d_prev is None:
             d = g
@@ -253,7 +377,7 @@ def fwi_run(grid, order, v0, dobs, iters, vmin, vmax, net=None, dv=30.0, snap_pr
     tr.close()
     vfin = dx / (dt * np.sqrt(vt.astype(np.float64)))
     x0, x1, z0, z1 = region(grid)
-    return np.ascontiguousarray(vfin[x0:x1, z0:z1], np.float32), objs, float(np.median(gms))
+    return np.ascontiguousarray(vfin[x0:x1, z0:z1], np.float32), objs, float(np.median(gms)), pts
 
 
 def upsample(vc, shape):
@@ -266,7 +390,7 @@ def upsample(vc, shape):
 
 def fine_misfit(case, grid_f, v, dobs_f):
     """The judge: the fine propagator's data misfit of a physical model."""
-    b = Batch([Group(**grid_f)], snap_prec=capi.SNAP_NONE, order=capi.ORDER_10)
+    b = _batch(grid_f, capi.SNAP_NONE, capi.ORDER_10)
     b.bind_model(0, 0, expand(v, grid_f))
     d = b.forward()
     b.close()

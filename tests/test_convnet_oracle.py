@@ -19,11 +19,13 @@ def test_bf16_rounding_is_nearest_even():
 
 def test_forward_matches_direct_loop():
     rng = np.random.default_rng(0)
-    theta = rng.standard_normal(2592) * 0.3
-    p = rng.standard_normal((9, 8))
-    y, _ = cn.forward(p, theta, emulate=False)
+    theta = rng.standard_normal(cn.NPARAMS) * 0.3
+    p = rng.standard_normal((9, 8)).astype(np.float32)
+    rv = rng.uniform(0.05, 0.3, (9, 8)).astype(np.float32)
+    y, _ = cn.forward(p, rv, theta, emulate=False)
     w1, w2, w3 = cn.split(theta)
     nx, nz = p.shape
+    q = (rv * p).astype(np.float64)
 
     def at(a, i, j):
         return a[i, j] if 0 <= i < nx and 0 <= j < nz else 0.0
@@ -32,7 +34,8 @@ def test_forward_matches_direct_loop():
     for i in range(nx):
         for j in range(nz):
             for c in range(16):
-                h1[c, i, j] = max(0.0, sum(w1[t, c] * at(p, i + dx, j + dz) for t, (dx, dz) in enumerate(cn.TAPS)))
+                h1[c, i, j] = max(0.0, sum(w1[t, 0, c] * at(p, i + dx, j + dz) + w1[t, 1, c] * at(q, i + dx, j + dz)
+                                           for t, (dx, dz) in enumerate(cn.TAPS)))
     h2 = np.zeros((16, nx, nz))
     for i in range(nx):
         for j in range(nz):
@@ -52,18 +55,24 @@ def test_forward_matches_direct_loop():
 
 def test_gradient_matches_finite_differences():
     rng = np.random.default_rng(1)
-    theta = rng.standard_normal(2592) * 0.2
+    theta = rng.standard_normal(cn.NPARAMS) * 0.2
     p = rng.standard_normal((2, 14, 12))
+    rv = rng.uniform(0.05, 0.3, p.shape).astype(np.float32)
     t = p + 0.1 * rng.standard_normal(p.shape)
     region = (5, 9, 5, 8)
-    loss, g = cn.loss_grad(p, t, theta, region, emulate=False)
+    loss, g = cn.loss_grad(p, rv, t, theta, region, emulate=False)
     # every W1 and W3 weight, a spread of W2 weights
-    idx = list(range(0, 144)) + list(range(144, 144 + 2304, 37)) + list(range(144 + 2304, 2592))
+    n1, n2 = cn.NW1, 9 * 16 * 16
+    idx = list(range(0, n1)) + list(range(n1, n1 + n2, 37)) + list(range(n1 + n2, cn.NPARAMS))
     h = 1e-6
+
+    def L(th):
+        return cn.loss_grad(p, rv, t, th, region, emulate=False)[0]
+
     for k in idx:
         tp, tm = theta.copy(), theta.copy()
         tp[k] += h
         tm[k] -= h
-        fd = (cn.loss_grad(p, t, tp, region, emulate=False)[0] - cn.loss_grad(p, t, tm, region, emulate=False)[0]) / (2 * h)
+        fd = (L(tp) - L(tm)) / (2 * h)
         assert abs(fd - g[k]) <= 1e-6 * max(1.0, abs(g[k])), (k, fd, g[k])
     assert loss > 0

@@ -42,23 +42,25 @@ def test_net_forward_and_gradient_against_oracle():
     e, nx, nz = 2, 70, 53
     region = (9, 61, 7, 46)  # 52 x 39: partial tiles in both directions
     p = smooth_fields(rng, e, nx, nz)
+    rv = rng.uniform(0.05, 0.3, p.shape).astype(np.float32)
     tgt = (p + 0.05 * smooth_fields(rng, e, nx, nz)).astype(np.float32)
     theta = Net.initial(seed=5, scale=0.1)
     net = Net(theta)
-    y = net.apply(p, region)
+    y = net.apply(p, rv, region)
     x0, x1, z0, z1 = region
     for k in range(e):
-        want, _ = cn.forward(p[k], theta, emulate=True)
+        want, _ = cn.forward(p[k], rv[k], theta, emulate=True)
         got = y[k]
         assert np.all(got[:x0] == 0) and np.all(got[:, :z0] == 0)
         err = rel_l2(got[x0:x1, z0:z1], want[x0:x1, z0:z1])
         print(f"net forward entry {k}: rel-L2 {err:.2e}", flush=True)
         assert err < 1e-4
-    loss, g = net.loss_grad(p, tgt, region)
-    wl, wg = cn.loss_grad(p, tgt, theta, region, emulate=True)
+    loss, g = net.loss_grad(p, rv, tgt, region)
+    wl, wg = cn.loss_grad(p, rv, tgt, theta, region, emulate=True)
     print(f"loss {loss:.6e} oracle {wl:.6e}", flush=True)
     assert abs(loss - wl) <= 1e-4 * wl
-    for name, sl in (("W1", slice(0, 144)), ("W2", slice(144, 144 + 2304)), ("W3", slice(144 + 2304, 2592))):
+    n1, n2 = cn.NW1, 9 * 16 * 16
+    for name, sl in (("W1", slice(0, n1)), ("W2", slice(n1, n1 + n2)), ("W3", slice(n1 + n2, cn.NPARAMS))):
         err = rel_l2(g[sl], wg[sl])
         print(f"  d loss / d {name}: rel-L2 {err:.2e}", flush=True)
         assert err < 2e-3, name
@@ -74,11 +76,12 @@ def test_net_training_lowers_loss():
     n, nx, nz = 24, 48, 40
     region = (6, 42, 6, 34)
     p = smooth_fields(rng, n, nx, nz)
+    rv = np.full(p.shape, 0.2, np.float32)
     F = np.array([[0.0, 0.1, 0.0], [0.1, -0.4, 0.1], [0.0, 0.1, 0.0]])
     tgt = np.stack([p[k] + correlate(p[k].astype(np.float64), F, mode="constant") for k in range(n)]).astype(
         np.float32)
     net = Net(Net.initial(seed=1, scale=0.01))
-    net.set_data(p, tgt, region)
+    net.set_data(p, rv, tgt, region)
     losses = net.train(batch=8, steps=300, lr=3e-3, seed=7)
     print(f"training loss {losses[0]:.3e} -> {losses[-1]:.3e}", flush=True)
     assert np.all(np.isfinite(losses))
