@@ -40,11 +40,31 @@ def measured_traffic(kernel, workload):
     return None
 
 
+def cpu_epoch(case, encoded, dobs):
+    """One whole epoch of the oracle with the reference's threading (ESS:
+    EssFwiFramework::epoch, essfwiframework.cpp:191-239; plain FWI:
+    FwiFramework::epoch over every shot): codes, encoded data, gradient,
+    direction and the line search's trial forward runs
+    (updatesteplenop.cpp:216-265), on the observed data the GPU computed
+    (bit-identical to the oracle's).  Returns (seconds, trial forward runs)."""
+    from oracle import pyoracle as po
+    from swfwi_amd import workloads
+
+    _, v0 = workloads.models(case)
+    g = po.Geom(case.nx, case.nz, case.nb, case.dt, case.dx, case.fm, case.nt, case.sx, case.sz, case.gx, case.gz,
+                parallel_u2=0)
+    wlt = po.ricker(case.nt, case.fm, case.dt, case.amp)
+    inv = po.Ess(g, po.expand_domain(v0, case.nb, case.dx, case.dt), wlt, dobs, case.vmin, case.vmax, 5, 200.0)
+    t0 = time.perf_counter()
+    (inv.epoch if encoded else inv.fwi_epoch)(0)
+    return time.perf_counter() - t0, inv.scalars()["ntrial"]
+
+
 def cpu_baseline(case, encoded, seconds_cap=30.0):
     """The oracle (CPU restatement of the reference path, bit-identical to it)
     timed on this host on a bounded sample of the same workload: one gradient
     (ESS: the encoded supershot of the 301x921 grid, nt=4000: 4000 forward
-    steps, 3700 backward iterations of two steps, ~13 s with 16 threads;
+    steps, 3700 backward iterations of two steps, ~11 s with 16 threads;
     plain FWI: one shot of the 201x301 grid, nt=2000), reference threading
     (serial u2 pass of the damped kernel) and all-parallel."""
     from oracle import pyoracle as po
@@ -107,7 +127,7 @@ def mixed_workload(args):
             v0 = workloads.smoothed(vt, 20.0)
             models.append((gi, m, capi.expand_domain(vt, g.nb, g.dx, g.dt), capi.expand_domain(v0, g.nb, g.dx, g.dt)))
     res = {}
-    timing = None
+    timing = {}
     for prec in (capi.SNAP_BF16, capi.SNAP_F16):
         name = {capi.SNAP_BF16: "bf16", capi.SNAP_F16: "f16"}[prec]
         b = Batch(groups, snap_prec=prec)
@@ -150,14 +170,25 @@ def mixed_workload(args):
             print(f"[bench] {name}: gradient of {g.nz}x{g.nx} shot 0 against the FP64 run: rel-L2 {err:.3e} "
                   f"(oracle {time.perf_counter() - t1:.1f} s)", file=sys.stderr, flush=True)
         res[name] = errs
-        if prec == capi.SNAP_BF16:
-            pts = inf["forward_points"] + 2 * inf["backward_points"]  # the backward advances two fields
-            dev_ms = float(np.mean(fms) + np.mean(bms))
-            timing = {"wall_ms": 1e3 * wall, "device_ms": dev_ms, "forward_ms": float(np.mean(fms)),
-                      "backward_ms": float(np.mean(bms)), "points": pts, "snapshot_bytes": inf["snapshot_bytes"],
-                      "entries": inf["entries"]}
+        pts = inf["forward_points"] + 2 * inf["backward_points"]  # the backward advances two fields
+        dev_ms = float(np.mean(fms) + np.mean(bms))
+        # algorithmic bytes of the step kernels: forward 16 B per point
+        # update (read cur, prev, model; write next) + the checkpoints
+        # written; backward 16 B per point (the receiver field alone) + the
+        # checkpoints read + the gradient read-modify-write on every 4th step
+        byts = (16.0 * inf["forward_points"] + inf["snapshot_bytes"] + 16.0 * inf["backward_points"] +
+                inf["snapshot_bytes"] + 2.0 * inf["backward_points"])
+        timing[name] = {"wall_ms": 1e3 * wall, "device_ms": dev_ms, "forward_ms": float(np.mean(fms)),
+                        "backward_ms": float(np.mean(bms)), "points": pts, "snapshot_bytes": inf["snapshot_bytes"],
+                        "entries": inf["entries"], "gpts": pts / wall / 1e9, "bytes": byts,
+                        # one launch per step of the longest entry, forward, and per backward
+                        # iteration (the taper ends the backward ~0.3 s early)
+                        "launches": max(g.nt for g in groups) + max(g.nt - int(round(0.3 / g.dt)) for g in groups)}
         b.close()
-    gpts = timing["points"] / (timing["wall_ms"] * 1e-3) / 1e9
+    by_format = {k: {"gpts_per_s": round(v["gpts"], 2), "gradient_ms_per_shot": round(v["device_ms"] / v["entries"], 4)}
+                 for k, v in timing.items()}
+    timing = timing["bf16"]
+    gpts = timing["gpts"]
     line = {
         "metric": "FWI gradient time/shot & Gpts/s, learned vs plain FP32 at equal data misfit",
         "value": round(gpts, 3), "unit": "Gpts/s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
@@ -171,7 +202,18 @@ def mixed_workload(args):
         "gradient_ms_per_shot": round(timing["device_ms"] / timing["entries"], 4),
         "forward_ms": round(timing["forward_ms"], 2), "backward_ms": round(timing["backward_ms"], 2),
         "checkpoint_bytes": timing["snapshot_bytes"],
+        "by_checkpoint_format": by_format,
+        "roofline": {"bound": "hbm", "achieved": round(timing["bytes"] / (timing["device_ms"] * 1e-3) / 1e9, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(timing["bytes"] / (timing["device_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "traffic": None, "kernel": "k_batch_step",
+                     "algorithmic_bytes_per_launch": round(timing["bytes"] / timing["launches"]),
+                     "avg_launch_us": round(1e3 * timing["device_ms"] / timing["launches"], 2),
+                     "launches": timing["launches"],
+                     "note": "BF16 checkpoints; forward and backward loops timed with HIP events on the batch's "
+                             "stream; the active entries shrink as the shorter grids finish (112, 48, then 16)"},
         "gradient_rel_l2_vs_fp64": res,
+        "gradient_tolerance": 1e-3,
     }
     print(json.dumps(line))
 
@@ -230,6 +272,18 @@ def learned_workload(args):
                      "fine_judged_misfit": float(f"{misfit:.6e}"), "own_grid_misfit": [float(f"{o:.5e}") for o in objs],
                      "wall_s": round(wall, 1)}
     m0 = learned.fine_misfit(case, fine, v0, dobs_f)
+    # the network kernel alone: one launch over the 16 coarse fields of a
+    # correction (the forward of the learned run holds 2 per 4 steps)
+    from scipy.ndimage import gaussian_filter
+    cshape = (coarse["nx"] + 2 * coarse["nb"] + 12, coarse["nz"] + coarse["nb"] + 12)
+    rng = np.random.default_rng(1)
+    fields = np.stack([gaussian_filter(rng.standard_normal(cshape), 3.0) for _ in shots]).astype(np.float32)
+    rvc = np.broadcast_to(1.0 / learned.expand(learned.restrict(v0), coarse), fields.shape).astype(np.float32)
+    reg = learned.region(coarse)
+    net_ms = net.time_apply(fields, rvc, reg, reps=50)
+    npts = (reg[1] - reg[0]) * (reg[3] - reg[2]) * len(shots)
+    flops = 2.0 * (9 * 2 * 16 + 9 * 16 * 16 + 9 * 16) * npts  # multiply-adds of the three layers per point
+    tfs = flops / (net_ms * 1e-3) / 1e12
     pf, lc, pc = out["plain_fine_fp32"], out["learned_coarse"], out["plain_coarse"]
     speedup = pf["gradient_ms_per_shot"] / lc["gradient_ms_per_shot"]
     ratio = lc["fine_judged_misfit"] / pf["fine_judged_misfit"]
@@ -251,6 +305,12 @@ def learned_workload(args):
             "fine_misfit_initial_model": float(f"{m0:.6e}"),
             "plain_coarse_fine_misfit_ratio": round(pc["fine_judged_misfit"] / pf["fine_judged_misfit"], 4),
         },
+        "roofline": {"bound": "mfma", "achieved": round(tfs, 1), "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": round(tfs / 2500.0, 4), "traffic": None, "kernel": "k_net_apply",
+                     "algorithmic_flop_per_launch": flops, "avg_launch_us": round(1e3 * net_ms, 2),
+                     "note": f"{len(shots)} coarse fields of {cshape[0]}x{cshape[1]} (region {npts // len(shots)} "
+                             "points each), 5472 algorithmic FLOP per point (2736 multiply-adds), BF16 MFMA with "
+                             "FP32 accumulation; 50 back-to-back launches timed with HIP events"},
         "runs": out,
         "training": {"models": 64, "grid": "161x80 at 12.5 m", "seconds": round(t_train, 1), "gain": res["gain"],
                      "held_out_seismogram_error": {"plain_coarse": round(res["err_plain"], 4),
@@ -266,6 +326,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", choices=["exact", "fp32"], default="exact")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-epoch", action="store_true", help="also time a whole plain-FWI epoch on the CPU")
     ap.add_argument("--nt", type=int, default=None, help="override nt (testing only)")
     ap.add_argument("--workload", choices=["ess_301x921", "fwi_201x301", "mixed", "learned"], default="ess_301x921")
     ap.add_argument("--shots", type=int, default=16, help="shots per model (mixed and learned workloads)")
@@ -385,6 +446,17 @@ def main():
                "sample": f"oracle {what}, nt={ref[2]} ({ref[2]} forward steps + {ref[3]} backward iterations), "
                          f"{ref[1]:.1f} s, serial u2 pass as the reference build",
                "all_parallel_value": round(par[0], 4) if par else None}
+        if ess or args.cpu_epoch:
+            # the whole epoch: the ESS one (~18 s) by default, the plain-FWI
+            # one (11 shot gradients + 22 trial runs, ~25 s) on request
+            t_ep, ntr = cpu_epoch(case, ess, dobs)
+            grad_s = ref[1] * (1 if ess else case.ns)
+            cpu["epoch"] = {"seconds": round(t_ep, 2), "gradient_seconds": round(grad_s, 2),
+                            "linesearch_seconds": round(max(t_ep - grad_s, 0.0), 2), "trial_forward_runs": ntr,
+                            "gpu_epoch_seconds": round(wall / args.steps, 4),
+                            "note": "one oracle epoch with the reference's threading; gradient_seconds from the "
+                                    "gradient timed above" + ("" if ess else " times the shots") +
+                                    ", line search = the rest (trial forward runs, objectives, update)"}
 
     line = {
         "metric": "FWI gradient time/shot & Gpts/s, learned vs plain FP32 at equal data misfit",

@@ -285,6 +285,11 @@ int swfwi_net_get_weights(swfwi_net n, float *w);
 /* y = net(fields) over the region (zero elsewhere) */
 int swfwi_net_apply(swfwi_net n, const float *fields, const float *rv, int nx, int nz, int entries, const int *region,
                     float *y);
+/* Milliseconds per launch of the network's forward kernel over the fields
+ * (uploaded once), averaged over `reps` back-to-back launches.  No
+ * counterpart in the reference: it measures the kernel for its roofline. */
+int swfwi_net_time_apply(swfwi_net n, const float *fields, const float *rv, int nx, int nz, int entries,
+                         const int *region, int reps, double *ms);
 /* loss = mean over the region of (in + net(in) - target)^2 and its gradient
  * with respect to the weights (grad may be null) */
 int swfwi_net_loss_grad(swfwi_net n, const float *in, const float *rv, const float *target, int nx, int nz,

@@ -231,6 +231,16 @@ class Net:
         check(load().swfwi_net_apply(self.h, f, r, nx, nz, e, np.ascontiguousarray(region, np.int32), y))
         return y
 
+    def time_apply(self, fields, rv, region, reps=20):
+        """Milliseconds per launch of the forward kernel over device copies
+        of the fields (a measurement probe)."""
+        f, r = _f(fields), _f(rv)
+        e, nx, nz = f.shape
+        ms = C.c_double()
+        check(load().swfwi_net_time_apply(self.h, f, r, nx, nz, e, np.ascontiguousarray(region, np.int32), reps,
+                                          C.byref(ms)))
+        return ms.value
+
     def loss_grad(self, inp, rv, target, region):
         a, r, t = _f(inp), _f(rv), _f(target)
         e, nx, nz = a.shape

@@ -82,6 +82,7 @@ SIGNATURES = {
     "swfwi_net_set_weights": (_I, [_VP, _f32]),
     "swfwi_net_get_weights": (_I, [_VP, _f32]),
     "swfwi_net_apply": (_I, [_VP, _f32, _f32, _I, _I, _I, _i32, _f32]),
+    "swfwi_net_time_apply": (_I, [_VP, _f32, _f32, _I, _I, _I, _i32, _I, C.POINTER(C.c_double)]),
     "swfwi_net_loss_grad": (_I, [_VP, _f32, _f32, _f32, _I, _I, _I, _i32, C.c_void_p, C.POINTER(C.c_double)]),
     "swfwi_net_set_data": (_I, [_VP, _f32, _f32, _f32, _I, _I, _I, _i32]),
     "swfwi_net_train": (_I, [_VP, _I, _I, _F, C.c_ulonglong, C.c_void_p]),

@@ -862,6 +862,37 @@ int swfwi_net_apply(swfwi_net n, const float* fields, const float* rv, int nx, i
   });
 }
 
+int swfwi_net_time_apply(swfwi_net n, const float* fields, const float* rv, int nx, int nz, int entries,
+                         const int* region, int reps, double* ms) {
+  return guarded([&] {
+    NEED(n);
+    NEED(fields);
+    NEED(rv);
+    NEED(ms);
+    if (reps < 1) fail(kErrArg, "reps must be >= 1");
+    NetGeom g = net_geom(nx, nz, entries, region);
+    hipStream_t s = n->s->get();
+    DBuf<float> p, r, out;
+    p.upload(fields, (size_t)g.stride * entries, s);
+    r.upload(rv, (size_t)g.stride * entries, s);
+    g.r = r.get();
+    out.resize((size_t)g.stride * entries);
+    net_apply(p.get(), out.get(), g, n->n->dev(), s);  // warm
+    hipEvent_t e0, e1;
+    SWFWI_HIP(hipEventCreate(&e0));
+    SWFWI_HIP(hipEventCreate(&e1));
+    SWFWI_HIP(hipEventRecord(e0, s));
+    for (int k = 0; k < reps; k++) net_apply(p.get(), out.get(), g, n->n->dev(), s);
+    SWFWI_HIP(hipEventRecord(e1, s));
+    SWFWI_HIP(hipEventSynchronize(e1));
+    float t = 0;
+    SWFWI_HIP(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = t / reps;
+  });
+}
+
 int swfwi_net_loss_grad(swfwi_net n, const float* in, const float* rv, const float* target, int nx, int nz,
                         int entries, const int* region, float* grad, double* loss) {
   return guarded([&] {

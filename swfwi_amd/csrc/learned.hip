@@ -44,6 +44,8 @@ static_assert(kNH1 % 16 == 0, "layer-1 points fill whole groups");
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 hbf16x2 __attribute__((ext_vector_type(2)));
 
 __device__ __host__ __forceinline__ uint16_t bf16_bits(unsigned u) {  // round to nearest even
   u += 0x7fffu + ((u >> 16) & 1u);
@@ -56,11 +58,24 @@ __device__ __forceinline__ float f32_of(uint16_t h) { return __uint_as_float((un
 __device__ __forceinline__ int d_row(int l, int v) { return 4 * (l >> 4) + v; }
 __device__ __forceinline__ int d_col(int l) { return l & 15; }
 
+// (a, b) rounded to BF16 (nearest even) by the hardware conversion
+// (v_cvt_pk_bf16_f32), a in the low half; the same bits as bf16_of for
+// finite values
+__device__ __forceinline__ unsigned pack_bf16(float a, float b) {
+  return __builtin_bit_cast(unsigned, __builtin_convertvector(f32x2{a, b}, hbf16x2));
+}
+
+// ReLU then BF16 of four accumulator items.  Rounded first, then the
+// negative ones (sign bit set: negative as 16-bit integers, -0 included)
+// raised to +0 by one packed 16-bit integer max per pair: the same bits as
+// rounding max(x, 0).
 __device__ __forceinline__ bf16x4 relu_bf16(const f32x4& a) {
-  bf16x4 r;
-#pragma unroll
-  for (int v = 0; v < 4; v++) r[v] = (short)bf16_of(fmaxf(a[v], 0.f));
-  return r;
+  typedef short s16x2 __attribute__((ext_vector_type(2)));
+  const s16x2 zero = {0, 0};
+  union { s16x2 h[2]; bf16x4 b; } r;
+  r.h[0] = __builtin_elementwise_max(__builtin_bit_cast(s16x2, pack_bf16(a[0], a[1])), zero);
+  r.h[1] = __builtin_elementwise_max(__builtin_bit_cast(s16x2, pack_bf16(a[2], a[3])), zero);
+  return r.b;
 }
 
 struct TileMap {
@@ -82,22 +97,27 @@ __device__ __forceinline__ TileMap tile_of(const NetGeom& g, int margin) {
 // Shared forward work of a tile.  sx[kP*kP]: staged (p, r p) as BF16 pairs
 // (p in the low half); h1[kNH1*16]: layer 1 (BF16).  Layer 2 then goes to
 // h2[kNH2*16] (BF16) when h2 is given and through layer 3 into
-// u[9][kNH2] (FP32) when u is given.
-__device__ void forward_tile(const float* __restrict__ p, const float* __restrict__ r, const NetGeom& g,
-                             const TileMap& m, const NetDev& w, unsigned* sx, uint16_t* h1, uint16_t* h2, float* u) {
+// u[9][kNH2] (FP32) when u is given.  Returns false, with nothing computed,
+// when `skip_zero` is set and the staged input is zero everywhere (the
+// network, ReLU without biases, then gives exactly zero).
+__device__ bool forward_tile(const float* __restrict__ p, const float* __restrict__ r, const NetGeom& g,
+                             const TileMap& m, const NetDev& w, unsigned* sx, uint16_t* h1, uint16_t* h2, float* u,
+                             bool skip_zero) {
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  int nonzero = 0;
   for (int q = tid; q < kP * kP; q += kT) {
     const int x = m.ox - 3 + q / kP, z = m.oz - 3 + q % kP;
     const bool in = x >= 0 && x < g.nx && z >= 0 && z < g.nz;
     const long long o = (long long)x * g.nz + z;
     const float v = in ? p[o] : 0.f;
     const float vq = in ? r[o] * v : 0.f;
-    sx[q] = (unsigned)bf16_of(v) | ((unsigned)bf16_of(vq) << 16);
+    sx[q] = pack_bf16(v, vq);
+    nonzero |= v != 0.f;
   }
   const int kq = lane >> 4;  // this lane's K quarter
   const bf16x4 a1a = *reinterpret_cast<const bf16x4*>(w.w1a + lane * 4);
   const bf16x4 a1b = *reinterpret_cast<const bf16x4*>(w.w1a + (64 + lane) * 4);
-  __syncthreads();
+  if (!__syncthreads_or(nonzero) && skip_zero) return false;  // block-uniform
   // layer 1: lane's B items are (p, r p) at taps 2 kq and 2 kq + 1 (first
   // product) and at tap 8 for kq = 0 (second product, the rest zero)
   {
@@ -148,6 +168,7 @@ __device__ void forward_tile(const float* __restrict__ p, const float* __restric
     }
   }
   __syncthreads();
+  return true;
 }
 
 __global__ __launch_bounds__(kT) void k_net_apply(const float* __restrict__ p_all, float* __restrict__ y_all,
@@ -158,14 +179,19 @@ __global__ __launch_bounds__(kT) void k_net_apply(const float* __restrict__ p_al
   const TileMap m = tile_of(g, 0);
   const float* p = p_all + (long long)m.e * g.stride;
   const float* r = g.r + (long long)(m.e / g.rdiv) * g.rstride;
-  forward_tile(p, r, g, m, w, sx, h1, nullptr, u);
   const int x = threadIdx.x / kTile, z = threadIdx.x % kTile;
   const int gx = m.ox + x, gz = m.oz + z;
-  if (gx >= g.x1 || gz >= g.z1) return;
+  float* yo = y_all + (long long)m.e * g.stride + (long long)gx * g.nz + gz;
+  const bool out = gx < g.x1 && gz < g.z1;
+  if (!forward_tile(p, r, g, m, w, sx, h1, nullptr, u, true)) {
+    if (out) *yo = 0.f;
+    return;
+  }
+  if (!out) return;
   float y = 0.f;
 #pragma unroll
   for (int t = 0; t < 9; t++) y += u[t * kNH2 + (x + 1 + t / 3 - 1) * kH2 + (z + 1 + t % 3 - 1)];
-  y_all[(long long)m.e * g.stride + (long long)gx * g.nz + gz] = y;
+  *yo = y;
 }
 
 __global__ void k_net_add(float* p_all, const float* y_all, NetGeom g) {
@@ -230,7 +256,7 @@ __global__ __launch_bounds__(kT) void k_net_backward(const float* __restrict__ p
   // one tile that owns it
   const TileMap m = tile_of(g, 2);
   const long long eo = (long long)m.e * g.stride;
-  forward_tile(p_all + eo, g.r + (long long)(m.e / g.rdiv) * g.rstride, g, m, w, sx, h1, h2, nullptr);
+  forward_tile(p_all + eo, g.r + (long long)(m.e / g.rdiv) * g.rstride, g, m, w, sx, h1, h2, nullptr, false);
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int q = tid; q < kNH1; q += kT) {
     const int x = m.ox - 2 + q / kH1, z = m.oz - 2 + q % kH1;

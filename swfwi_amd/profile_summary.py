@@ -1,8 +1,11 @@
 """Summaries of rocprofv3 CSV output for the profiles/ folder.
 
-    python -m swfwi_amd.profile_summary trace <dir>/<prefix>_kernel_trace.csv
-        per step-kernel variant (template arguments, blocks): calls and
-        mean / median / min duration;
+    python -m swfwi_amd.profile_summary trace <dir>/<prefix>_kernel_trace.csv [--kernels REGEX]
+        per kernel variant (template arguments, blocks) whose name matches
+        REGEX (default: the step kernels): calls and mean / median / min
+        duration;
+    python -m swfwi_amd.profile_summary stats <dir>/<prefix>_kernel_stats.csv [--top N]
+        the --stats summary as a table (calls, total ms, average us, %);
     python -m swfwi_amd.profile_summary hbm <copy_fetch.csv> <copy_write.csv> \
             <fetch.csv> <write.csv> --copy-floats N [--select VARIANT:BLOCKS ...]
         HBM traffic per dispatch from FETCH_SIZE / WRITE_SIZE (KiB), scaled by
@@ -23,17 +26,25 @@ def variant(name):
     return (m.group(1) + (m.group(2) or "")) if m else name.split("(")[0]
 
 
-def trace(path, out):
+def trace(path, out, kernels=r"k_(fwd|bwd)_"):
     rows = list(csv.DictReader(open(path)))
     d = collections.defaultdict(list)
     for r in rows:
-        if not re.search(r"k_(fwd|bwd)_", r["Kernel_Name"]):
+        if not re.search(kernels, r["Kernel_Name"]):
             continue
         blocks = int(r["Grid_Size_X"]) // int(r["Workgroup_Size_X"])
         d[(variant(r["Kernel_Name"]), blocks)].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
     out.write("| kernel | blocks | calls | mean us | median us | min us |\n|---|---|---|---|---|---|\n")
     for (k, b), v in sorted(d.items()):
         out.write(f"| `{k}` | {b} | {len(v)} | {statistics.mean(v):.2f} | {statistics.median(v):.2f} | {min(v):.2f} |\n")
+
+
+def stats(path, out, top=16):
+    rows = list(csv.DictReader(open(path)))
+    out.write("| kernel | calls | total ms | average us | % |\n|---|---|---|---|---|\n")
+    for r in rows[:top]:
+        out.write(f"| `{variant(r['Name'])}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.2f} | "
+                  f"{float(r['AverageNs']) / 1e3:.2f} | {float(r['Percentage']):.2f} |\n")
 
 
 def counter(path, pick):
@@ -81,6 +92,10 @@ def main(argv=None):
     sub = ap.add_subparsers(dest="mode", required=True)
     t = sub.add_parser("trace")
     t.add_argument("csv")
+    t.add_argument("--kernels", default=r"k_(fwd|bwd)_")
+    st = sub.add_parser("stats")
+    st.add_argument("csv")
+    st.add_argument("--top", type=int, default=16)
     h = sub.add_parser("hbm")
     for k in ("copy_fetch", "copy_write", "fetch", "write"):
         h.add_argument(k)
@@ -88,7 +103,9 @@ def main(argv=None):
     h.add_argument("--select", nargs="*", default=[])
     a = ap.parse_args(argv)
     if a.mode == "trace":
-        trace(a.csv, sys.stdout)
+        trace(a.csv, sys.stdout, a.kernels)
+    elif a.mode == "stats":
+        stats(a.csv, sys.stdout, a.top)
     else:
         hbm(a, sys.stdout)
 

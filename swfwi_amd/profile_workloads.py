@@ -109,7 +109,7 @@ def run_fwi(nt, epochs):
               f"{st['linesearch_ms']:.1f} ms, obj {st['obj1']:.4e}", flush=True)
 
 
-def run_enkf(nt, members):
+def run_enkf(nt, members, graphs=True):
     """EnKF analysis on the 301x921 grid: `members` velocity members (initial
     model + 3% Gaussian random fields, correlation ~20 cells), one batched
     encoded forward run of all members and the FP64 analysis."""
@@ -128,6 +128,7 @@ def run_enkf(nt, members):
     wlt = capi.ricker(case.nt, case.fm, case.dt, case.amp)
     s = capi.Solver(case.nx, case.nz, case.nb, case.dt, case.dx, case.fm, case.nt, case.sx, case.sz, case.gx,
                     case.gz)
+    s.set_graphs(graphs)
     s.bind_velocity(capi.expand_domain(vtrue, case.nb, case.dx, case.dt))
     dobs = s.forward_shots(wlt)
     e = Enkf(s, wlt, dobs, 0.2, members)
@@ -143,7 +144,7 @@ def run_enkf(nt, members):
               f"clip={info['clip']} sigma={info['sigma']:.3e}", flush=True)
 
 
-def run_enfwi(nt, members, epochs):
+def run_enfwi(nt, members, epochs, graphs=True):
     """Ensemble-Kalman FWI on the 301x921 grid: `members` velocity members
     (initial model + 3% Gaussian random fields, correlation ~20 cells),
     observed data with 1% RMS Gaussian noise; the initial analysis, then
@@ -163,6 +164,7 @@ def run_enfwi(nt, members, epochs):
     wlt = capi.ricker(case.nt, case.fm, case.dt, case.amp)
     s = capi.Solver(case.nx, case.nz, case.nb, case.dt, case.dx, case.fm, case.nt, case.sx, case.sz, case.gx,
                     case.gz)
+    s.set_graphs(graphs)
     s.bind_velocity(capi.expand_domain(vtrue, case.nb, case.dx, case.dt))
     dobs = s.forward_shots(wlt)
     rms = float(np.sqrt(np.mean(dobs.astype(np.float64) ** 2)))
@@ -213,9 +215,9 @@ def main(argv=None):
         run_sweep([int(x) for x in a.entries.split(",")], a.reps,
                   capi.PREC_FP32 if a.precision == "fp32" else capi.PREC_EXACT)
     elif a.mode == "enfwi":
-        run_enfwi(a.nt or 4000, a.members, a.epochs)
+        run_enfwi(a.nt or 4000, a.members, a.epochs, graphs=not a.no_graphs)
     elif a.mode == "enkf":
-        run_enkf(a.nt or 4000, a.members)
+        run_enkf(a.nt or 4000, a.members, graphs=not a.no_graphs)
     elif a.mode == "fwi":
         run_fwi(a.nt or 2000, a.epochs)
     else:
