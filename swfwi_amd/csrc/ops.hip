@@ -432,6 +432,87 @@ __global__ __launch_bounds__(kGWaves * 64) void k_gram_mfma(const double* __rest
   (void)npad;
 }
 
+// FP64 MFMA Gram product, one wave per 16-member tile row: wave I forms the
+// tiles (I, J) of X^T Y for J = 0 .. T-1 (J >= I when X == Y, the upper
+// triangle), so per K-step it reads its row's A operand once and one B
+// operand per tile.  Each block takes a run of data rows in chunks of
+// kG2KC; the next chunk is loaded into registers while the current one is
+// multiplied from LDS, so the HBM latency of a chunk overlaps the products
+// of the one before.  Same lane layout and output convention as k_gram_mfma.
+constexpr int kG2KC = 16;          // data rows per chunk
+constexpr int kG2Ld = kG2KC + 1;   // LDS row stride (doubles)
+constexpr int kG2MaxT = kGMaxN / 16;
+constexpr int kG2Threads = kG2MaxT * 64;
+constexpr int kG2PerThread = kGMaxN * kG2KC / kG2Threads;  // staged values per thread and matrix
+static_assert(kGMaxN * kG2KC % kG2Threads == 0, "chunk splits evenly over the block");
+
+template <bool SYM>
+__global__ __launch_bounds__(kG2Threads, 4) void k_gram_rows(const double* __restrict__ X, const double* __restrict__ Y,
+                                                          int N, long long m, long long rows_per_block,
+                                                          double* __restrict__ part) {
+  __shared__ double sx[kGMaxN * kG2Ld];
+  __shared__ double sy[SYM ? 1 : kGMaxN * kG2Ld];
+  const double* sY = SYM ? sx : sy;
+  const int T = (N + 15) / 16;
+  const int lane = threadIdx.x & 63, I = threadIdx.x >> 6;  // this wave's tile row
+  const int li = lane & 15, lk = lane >> 4;
+  const long long r0 = (long long)blockIdx.x * rows_per_block;
+  const long long r1 = r0 + rows_per_block < m ? r0 + rows_per_block : m;
+  // staging: value c of this thread is member q / kG2KC, row q % kG2KC of
+  // the chunk, q = threadIdx.x + c * kG2Threads (16 consecutive threads read
+  // one member's 128 contiguous bytes)
+  double nx[kG2PerThread], ny[kG2PerThread];
+  auto fetch = [&](long long rb) {
+#pragma unroll
+    for (int c = 0; c < kG2PerThread; c++) {
+      const int q = threadIdx.x + c * kG2Threads, i = q / kG2KC;
+      const long long r = rb + (q % kG2KC);
+      const bool in = i < N && r < r1;
+      nx[c] = in ? X[(long long)i * m + r] : 0.0;
+      if (!SYM) ny[c] = in ? Y[(long long)i * m + r] : 0.0;
+    }
+  };
+  v4d acc[kG2MaxT];
+#pragma unroll
+  for (int J = 0; J < kG2MaxT; J++) acc[J] = v4d{0, 0, 0, 0};
+  fetch(r0);
+  for (long long rb = r0; rb < r1; rb += kG2KC) {
+#pragma unroll
+    for (int c = 0; c < kG2PerThread; c++) {
+      const int q = threadIdx.x + c * kG2Threads;
+      sx[(q / kG2KC) * kG2Ld + q % kG2KC] = nx[c];
+      if (!SYM) sy[(q / kG2KC) * kG2Ld + q % kG2KC] = ny[c];
+    }
+    __syncthreads();
+    if (rb + kG2KC < r1) fetch(rb + kG2KC);  // block-uniform
+    if (I < T) {
+      const double* pa = sx + (I * 16 + li) * kG2Ld + lk;
+#pragma unroll
+      for (int kk = 0; kk < kG2KC; kk += 4) {
+        const double a = pa[kk];
+#pragma unroll
+        for (int J = 0; J < kG2MaxT; J++)
+          if (J < T && (!SYM || J >= I))  // wave-uniform
+            acc[J] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, sY[(J * 16 + li) * kG2Ld + lk + kk], acc[J], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+  if (I >= T) return;
+  double* out = part + (long long)blockIdx.x * N * N;
+  const int iq = I * 16 + lk;
+#pragma unroll
+  for (int J = 0; J < kG2MaxT; J++) {
+    if (J >= T || (SYM && J < I)) continue;
+    const int j = J * 16 + li;
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int i = iq + 4 * v;
+      if (i < N && j < N) out[i + (long long)N * j] = acc[J][v];
+    }
+  }
+}
+
 // sum of the per-block partials in block order; for a symmetric product only
 // the upper-triangle tiles were formed, the lower ones are mirrored
 __global__ void k_gram_mfma_finish(const double* part, int nchunk, int N, bool sym, double* C) {
@@ -623,20 +704,20 @@ double op_launch_latency(int blocks, int reps, bool graph, hipStream_t s) {
 
 void op_gram_f64(const double* A, const double* B, int N, long long m, double* C, hipStream_t s) {
   if (N >= 1 && N <= kGMaxN) {
-    // one block per run of rows, ~4 blocks per CU; partials summed in order
-    long long nblk = 1024;
+    // one block per run of rows, two blocks per CU; partials summed in order
+    long long nblk = 512;
     long long rows = (m + nblk - 1) / nblk;
-    rows = ((rows + kGKC - 1) / kGKC) * kGKC;
-    if (rows < kGKC) rows = kGKC;
+    rows = ((rows + kG2KC - 1) / kG2KC) * kG2KC;
+    if (rows < kG2KC) rows = kG2KC;
     nblk = (m + rows - 1) / rows;
     if (nblk < 1) nblk = 1;
     DBuf<double> part((size_t)nblk * N * N);
     const bool sym = A == B;
     if (sym)
-      hipLaunchKernelGGL(k_gram_mfma<true>, dim3((unsigned)nblk), dim3(kGWaves * 64), 0, s, A, B, N, m, rows,
+      hipLaunchKernelGGL(k_gram_rows<true>, dim3((unsigned)nblk), dim3(kG2Threads), 0, s, A, B, N, m, rows,
                          part.get());
     else
-      hipLaunchKernelGGL(k_gram_mfma<false>, dim3((unsigned)nblk), dim3(kGWaves * 64), 0, s, A, B, N, m, rows,
+      hipLaunchKernelGGL(k_gram_rows<false>, dim3((unsigned)nblk), dim3(kG2Threads), 0, s, A, B, N, m, rows,
                          part.get());
     hipLaunchKernelGGL(k_gram_mfma_finish, dim3(nblocks((long long)N * N)), dim3(kB), 0, s, part.get(), (int)nblk, N,
                        sym, C);
