@@ -113,8 +113,9 @@ def mixed_groups(shots=16):
 def mixed_workload(args):
     """Mixed-size batched gradients with reduced-precision checkpoints: every
     shot of every model in one launch per time step, the source field kept
-    every 4 steps in BF16 (and, for the error figures, FP16), gradients of
-    one shot per grid size against the FP64 oracle run."""
+    every 4 steps in BF16 (value and roofline), FP16 and FP32 (whose error
+    against the FP64 oracle run is the imaging subsampling alone); gradients
+    of one shot per grid size against the FP64 oracle run."""
     from oracle import pyoracle as po
     from swfwi_amd import capi, workloads
     from swfwi_amd.batch import Batch
@@ -128,8 +129,8 @@ def mixed_workload(args):
             models.append((gi, m, capi.expand_domain(vt, g.nb, g.dx, g.dt), capi.expand_domain(v0, g.nb, g.dx, g.dt)))
     res = {}
     timing = {}
-    for prec in (capi.SNAP_BF16, capi.SNAP_F16):
-        name = {capi.SNAP_BF16: "bf16", capi.SNAP_F16: "f16"}[prec]
+    for prec in (capi.SNAP_F32, capi.SNAP_BF16, capi.SNAP_F16):  # FP32 (136 GB of checkpoints) for the baseline
+        name = {capi.SNAP_F32: "f32", capi.SNAP_BF16: "bf16", capi.SNAP_F16: "f16"}[prec]
         b = Batch(groups, snap_prec=prec)
         b.set_graphs(not args.no_graphs)
         for gi, m, vt, _ in models:

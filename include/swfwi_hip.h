@@ -273,9 +273,9 @@ int swfwi_batch_set_correction(swfwi_batch b, swfwi_net net);
  * inputs are p and r p, r = 1 / vel_t (the local Courant number squared,
  * given as the rv stack beside the fields).  Weights (2736 floats):
  * W1[t][k][c] (9 x 2 x 16), W2[t][ci][co] (9 x 16 x 16), W3[t][ci] (9 x 16),
- * tap t = (dx + 1) * 3 + (dz + 1) for the neighbour (x + dx, z + dz).  The
- * 16 -> 16 layer and the weight-gradient products run on BF16 MFMA with FP32
- * accumulation.  Field stacks are [entries][nx][nz] (z contiguous);
+ * tap t = (dx + 1) * 3 + (dz + 1) for the neighbour (x + dx, z + dz).  Every
+ * layer and the weight-gradient products run on BF16 MFMA with FP32
+ * accumulation (inputs, activations and weights rounded to BF16).  Field stacks are [entries][nx][nz] (z contiguous);
  * region[4] = x0, x1, z0, z1: the outputs and the loss cover
  * [x0, x1) x [z0, z1), which must lie 5 cells inside the array. */
 int swfwi_net_create(swfwi_net *out);

@@ -349,88 +349,12 @@ __global__ void k_gram(const double* A, const double* B, int N, long long m, lon
   if (i < N && j < N) part[(long long)blockIdx.z * N * N + i + (long long)N * j] = acc;
 }
 
-// FP64 MFMA form for ensembles of up to kGMaxN members: each block takes a
-// run of data rows and forms every 16x16 output tile of X^T Y (only the
-// upper-triangle tiles when X == Y) from rows staged through LDS, so each
-// row of X and Y is read from HBM once.  v_mfma_f64_16x16x4f64: lane l
-// supplies A[i = l%16][k = l/16] and B[k = l/16][j = l%16] and holds
-// D[i = l/16 + 4*v][j = l%16] in accumulator item v (measured).
-constexpr int kGMaxN = 112;   // 7 tiles of 16 members
-constexpr int kGKC = 16;      // data rows per staged chunk
-constexpr int kGLd = kGKC + 1;
-constexpr int kGWaves = 4;
-constexpr int kGMaxTilesPerWave = 13;  // ceil(49 / 4)
+// FP64 MFMA forms for ensembles of up to kGMaxN members.
+// v_mfma_f64_16x16x4f64: lane l supplies A[i = l%16][k = l/16] and
+// B[k = l/16][j = l%16] and holds D[i = l/16 + 4*v][j = l%16] in accumulator
+// item v (measured).
+constexpr int kGMaxN = 112;  // 7 tiles of 16 members
 typedef double v4d __attribute__((ext_vector_type(4)));
-
-template <bool SYM>
-__global__ __launch_bounds__(kGWaves * 64) void k_gram_mfma(const double* __restrict__ X, const double* __restrict__ Y,
-                                                            int N, long long m, long long rows_per_block,
-                                                            double* __restrict__ part) {
-  __shared__ double sx[kGMaxN * kGLd];
-  __shared__ double sy[SYM ? 1 : kGMaxN * kGLd];
-  const double* sY = SYM ? sx : sy;
-  const int T = (N + 15) / 16;
-  const int npad = T * 16;
-  const int ntiles = SYM ? T * (T + 1) / 2 : T * T;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int li = lane & 15, lk = lane >> 4;
-  // this wave's tiles: t = wave, wave + kGWaves, ...
-  int ti[kGMaxTilesPerWave], tj[kGMaxTilesPerWave];
-  v4d acc[kGMaxTilesPerWave];
-#pragma unroll
-  for (int s = 0; s < kGMaxTilesPerWave; s++) {
-    int t = wave + s * kGWaves, I = 0, J = 0;
-    if (SYM) {  // upper triangle, row by row
-      int rem = t;
-      while (I < T && rem >= T - I) {
-        rem -= T - I;
-        I++;
-      }
-      J = I + rem;
-    } else {
-      I = t / (T > 0 ? T : 1);
-      J = t - I * T;
-    }
-    ti[s] = t < ntiles ? I : -1;
-    tj[s] = J;
-    acc[s] = v4d{0, 0, 0, 0};
-  }
-  const long long r0 = (long long)blockIdx.x * rows_per_block;
-  const long long r1 = r0 + rows_per_block < m ? r0 + rows_per_block : m;
-  for (long long rb = r0; rb < r1; rb += kGKC) {
-    // stage rows [rb, rb + kGKC) of every member: 16 consecutive threads
-    // read one member's 128 contiguous bytes
-    for (int q = threadIdx.x; q < npad * kGKC; q += kGWaves * 64) {
-      const int i = q / kGKC, k = q - i * kGKC;
-      const long long r = rb + k;
-      const bool in = i < N && r < r1;
-      sx[i * kGLd + k] = in ? X[(long long)i * m + r] : 0.0;
-      if (!SYM) sy[i * kGLd + k] = in ? Y[(long long)i * m + r] : 0.0;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int s = 0; s < kGMaxTilesPerWave; s++) {
-      if (ti[s] < 0) continue;
-      const double* pa = sx + (ti[s] * 16 + li) * kGLd + lk;
-      const double* pb = sY + (tj[s] * 16 + li) * kGLd + lk;
-#pragma unroll
-      for (int kk = 0; kk < kGKC; kk += 4) acc[s] = __builtin_amdgcn_mfma_f64_16x16x4f64(pa[kk], pb[kk], acc[s], 0, 0, 0);
-    }
-    __syncthreads();
-  }
-  double* out = part + (long long)blockIdx.x * N * N;
-#pragma unroll
-  for (int s = 0; s < kGMaxTilesPerWave; s++) {
-    if (ti[s] < 0) continue;
-    const int j = tj[s] * 16 + li;
-#pragma unroll
-    for (int v = 0; v < 4; v++) {
-      const int i = ti[s] * 16 + lk + 4 * v;
-      if (i < N && j < N) out[i + (long long)N * j] = acc[s][v];
-    }
-  }
-  (void)npad;
-}
 
 // FP64 MFMA Gram product, one wave per 16-member tile row: wave I forms the
 // tiles (I, J) of X^T Y for J = 0 .. T-1 (J >= I when X == Y, the upper
@@ -438,7 +362,8 @@ __global__ __launch_bounds__(kGWaves * 64) void k_gram_mfma(const double* __rest
 // operand per tile.  Each block takes a run of data rows in chunks of
 // kG2KC; the next chunk is loaded into registers while the current one is
 // multiplied from LDS, so the HBM latency of a chunk overlaps the products
-// of the one before.  Same lane layout and output convention as k_gram_mfma.
+// of the one before.  Output: block b's partial X^T Y at part[b][i + N j]
+// (for X == Y only the tiles with J >= I).
 constexpr int kG2KC = 16;          // data rows per chunk
 constexpr int kG2Ld = kG2KC + 1;   // LDS row stride (doubles)
 constexpr int kG2MaxT = kGMaxN / 16;
