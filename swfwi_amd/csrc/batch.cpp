@@ -98,6 +98,8 @@ void Batch::build() {
     ricker_wavelet(wl.data(), G.nt, G.fm, G.dt, 1.0f);
     amp.insert(amp.end(), wl.begin(), wl.end());
     wts.insert(wts.end(), w.begin(), w.end());
+    // the step kernel forms per-point byte offsets in 32 bits
+    if (npts >= (1LL << 29)) fail(kErrRange, "padded grid too large for a batch entry");
     for (int m = 0; m < G.nmodels; m++) {
       model_off_.push_back(model_len);
       model_group_.push_back((int)gi);
@@ -106,8 +108,8 @@ void Batch::build() {
         BatchEntry e{};
         e.nxpad = nxpad;
         e.nzpad = nzpad;
-        e.ntz = (nzpad - 2 * kHalo + 63) / 64;
-        e.ntx = (nxpad - 2 * kHalo + 15) / 16;
+        e.ntz = (nzpad - 2 * kHalo + kBatchTileZ - 1) / kBatchTileZ;
+        e.ntx = (nxpad - 2 * kHalo + kBatchTileX - 1) / kBatchTileX;
         e.damp = bx0;
         e.dscale = G.dscale;
         e.nt = G.nt;

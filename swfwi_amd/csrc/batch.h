@@ -18,6 +18,10 @@
 namespace swfwi {
 
 constexpr int kSnapEvery = 4;
+// Step-kernel tile: 64 z x kBatchTileX x output points per block of 256
+// threads (each thread kBatchTileX / 4 consecutive x at one z).
+constexpr int kBatchTileZ = 64;
+constexpr int kBatchTileX = 32;
 enum SnapPrec : int { kSnapF32 = 0, kSnapBF16 = 1, kSnapF16 = 2, kSnapNone = 3 };
 // Stencils: the reference's damped 10th-order-in-space, 4th-order-in-time
 // scheme (fd4t10s-damp-zjh.c), or a damped 4th-order-in-space, 2nd-order-in-

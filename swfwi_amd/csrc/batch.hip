@@ -1,8 +1,11 @@
 // Step kernels of the mixed-size batch (batch.h): one launch advances every
-// active entry by one time step, each block one 64 (z) x 16 (x) tile of one
+// active entry by one time step, each block one 64 (z) x 32 (x) tile of one
 // entry.  Per block: stage the entry's current field with a 6-cell halo in
-// LDS, form the 10th-order Laplacian u2 in LDS, then the damped leapfrog
-// update of fd4t10s-damp-zjh.c:14-115 in FP32 (fused multiply-adds).
+// LDS (whole rows per wavefront), form the 10th-order Laplacian u2 in LDS
+// (each lane sliding its z column over x, the x-direction terms in
+// registers), then the damped leapfrog update of fd4t10s-damp-zjh.c:14-115
+// in FP32 (fused multiply-adds), each thread 8 consecutive x at one z with
+// its model and previous field loaded before the staging.
 //
 // Forward (FWD): the next step's source amplitude is added to `next`
 // (addSource before stepForward, damp4t10d.cpp:272-289), receivers are
@@ -17,6 +20,8 @@
 // with the source field C[it-1] read back from snapshot k.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "batch.h"
 #include "kernels.h"
 
@@ -24,7 +29,9 @@ namespace swfwi {
 namespace {
 
 constexpr int kBT = 256;
-constexpr int kBZ = 64, kBX = 16, kBH = 6;
+constexpr int kBZ = kBatchTileZ, kBX = kBatchTileX, kBH = 6;
+constexpr int kPX = kBX / (kBT / kBZ);  // outputs per thread, consecutive in x
+static_assert(kBZ == 64 && kBX % (kBT / kBZ) == 0, "a wavefront covers one z column of the tile");
 constexpr int kSZ = kBZ + 2 * kBH;  // staged z
 constexpr int kSX = kBX + 2 * kBH;  // staged x
 constexpr int kUZ = kBZ + 2;        // u2 z (outputs +-1)
@@ -51,26 +58,36 @@ __device__ __forceinline__ unsigned short to_bf16(float x) {  // round to neares
 }
 __device__ __forceinline__ float from_bf16(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
 
+// Element i of an array whose base is block-uniform (scalar registers), the
+// byte offset formed in 32 bits: an entry's field, model, checkpoint or
+// gradient holds fewer than 2^29 points (checked at creation), so per-point
+// accesses need no 64-bit address arithmetic.
+template <class T>
+__device__ __forceinline__ T& at(T* base, unsigned i) {
+  using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+  return *reinterpret_cast<T*>(reinterpret_cast<B*>(base) + i * (unsigned)sizeof(T));
+}
+
 template <int PREC>
-__device__ __forceinline__ void snap_store(void* base, long long i, float v, float scale) {
+__device__ __forceinline__ void snap_store(void* base, unsigned i, float v, float scale) {
   if constexpr (PREC == kSnapF32) {
-    static_cast<float*>(base)[i] = v;
+    at(static_cast<float*>(base), i) = v;
   } else if constexpr (PREC == kSnapBF16) {
     const float inv = scale > 0.f ? 1.f / scale : 0.f;
-    static_cast<unsigned short*>(base)[i] = to_bf16(v * inv);
+    at(static_cast<unsigned short*>(base), i) = to_bf16(v * inv);
   } else {
     const float inv = scale > 0.f ? kF16Scale / scale : 0.f;
-    static_cast<_Float16*>(base)[i] = (_Float16)(v * inv);
+    at(static_cast<_Float16*>(base), i) = (_Float16)(v * inv);
   }
 }
 template <int PREC>
-__device__ __forceinline__ float snap_load(const void* base, long long i, float scale) {
+__device__ __forceinline__ float snap_load(const void* base, unsigned i, float scale) {
   if constexpr (PREC == kSnapF32) {
-    return static_cast<const float*>(base)[i];
+    return at(static_cast<const float*>(base), i);
   } else if constexpr (PREC == kSnapBF16) {
-    return from_bf16(static_cast<const unsigned short*>(base)[i]) * scale;
+    return from_bf16(at(static_cast<const unsigned short*>(base), i)) * scale;
   } else {
-    return (float)static_cast<const _Float16*>(base)[i] * (scale * (1.f / kF16Scale));
+    return (float)at(static_cast<const _Float16*>(base), i) * (scale * (1.f / kF16Scale));
   }
 }
 
@@ -78,6 +95,25 @@ __device__ __forceinline__ float wave_max(float v) {
 #pragma unroll
   for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
   return v;
+}
+
+// u2 of the staged field at index c (the unscaled Laplacian of the step)
+template <int ORDER>
+__device__ __forceinline__ float lap_at(const float* sc, int c) {
+  float acc;
+  if constexpr (ORDER == kOrder10) {
+    acc = -4.f * kA0 * sc[c];
+    acc = fmaf(kA1, (sc[c - 1] + sc[c + 1]) + (sc[c - kSZ] + sc[c + kSZ]), acc);
+    acc = fmaf(kA2, (sc[c - 2] + sc[c + 2]) + (sc[c - 2 * kSZ] + sc[c + 2 * kSZ]), acc);
+    acc = fmaf(kA3, (sc[c - 3] + sc[c + 3]) + (sc[c - 3 * kSZ] + sc[c + 3 * kSZ]), acc);
+    acc = fmaf(kA4, (sc[c - 4] + sc[c + 4]) + (sc[c - 4 * kSZ] + sc[c + 4 * kSZ]), acc);
+    acc = fmaf(kA5, (sc[c - 5] + sc[c + 5]) + (sc[c - 5 * kSZ] + sc[c + 5 * kSZ]), acc);
+  } else {  // 4th-order Laplacian: -5 p + 4/3 (ring 1) - 1/12 (ring 2)
+    acc = -5.f * sc[c];
+    acc = fmaf(4.f / 3.f, (sc[c - 1] + sc[c + 1]) + (sc[c - kSZ] + sc[c + kSZ]), acc);
+    acc = fmaf(-1.f / 12.f, (sc[c - 2] + sc[c + 2]) + (sc[c - 2 * kSZ] + sc[c + 2 * kSZ]), acc);
+  }
+  return acc;
 }
 
 template <int PREC, bool FWD, int ORDER>
@@ -103,35 +139,79 @@ __global__ __launch_bounds__(kBT) void k_batch_step(BatchKernelArgs a) {
   float* nxt = (even ? a.fa : a.fb) + E.f;  // holds prev; next is written over it
 
   const bool inside = ox - kBH + kSX <= E.nxpad && oz - kBH + kSZ <= E.nzpad;
-  for (int q = threadIdx.x; q < kSX * kSZ; q += kBT) {
-    const int xx = q / kSZ, zz = q - xx * kSZ;
-    const int gx = ox - kBH + xx, gz = oz - kBH + zz;
-    sc[q] = (inside || (gx < E.nxpad && gz < E.nzpad)) ? cur[gx * ld + gz] : 0.f;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  // this thread's kPX outputs (consecutive x at one z): the model and prev
+  // are loaded before the staging so that their latency overlaps it
+  const int zl = threadIdx.x & (kBZ - 1), xg = threadIdx.x / kBZ;
+  const int gz = oz + zl;
+  const float* rvb = a.rv + E.model;
+  float rvj[kPX], pvj[kPX];
+#pragma unroll
+  for (int j = 0; j < kPX; j++) {
+    const int gx = ox + xg * kPX + j;
+    const bool ok = gx < E.nxpad - kBH && gz < E.nzpad - kBH;
+    const int p = gx * ld + gz;
+    rvj[j] = ok ? at(rvb, p) : 0.f;
+    pvj[j] = ok ? at(nxt, p) : 0.f;
+  }
+  // stage cur + 6-cell halo: each wavefront takes whole rows (one z per
+  // lane, the last 12 z of a row by lanes 0-11), so a row is two coalesced
+  // loads and no lane divides its index
+  for (int xx = wave; xx < kSX; xx += kBT / 64) {
+    const int gx = ox - kBH + xx;
+    const unsigned src = gx * ld + (oz - kBH) + lane;
+    float v0 = 0.f, v1 = 0.f;
+    if (gx < E.nxpad) {  // wave-uniform
+      if (inside || oz - kBH + lane < E.nzpad) v0 = at(cur, src);
+      if (lane < kSZ - 64 && (inside || oz - kBH + 64 + lane < E.nzpad)) v1 = at(cur, src + 64);
+    }
+    sc[xx * kSZ + lane] = v0;
+    if (lane < kSZ - 64) sc[xx * kSZ + 64 + lane] = v1;
   }
   __syncthreads();
-  for (int q = threadIdx.x; q < kUX * kUZ; q += kBT) {
-    const int xx = q / kUZ, zz = q - xx * kUZ;
-    const int c = (xx + kBH - 1) * kSZ + zz + kBH - 1;
-    float acc;
-    if constexpr (ORDER == kOrder10) {
-      acc = -4.f * kA0 * sc[c];
-      acc = fmaf(kA1, (sc[c - 1] + sc[c + 1]) + (sc[c - kSZ] + sc[c + kSZ]), acc);
-      acc = fmaf(kA2, (sc[c - 2] + sc[c + 2]) + (sc[c - 2 * kSZ] + sc[c + 2 * kSZ]), acc);
-      acc = fmaf(kA3, (sc[c - 3] + sc[c + 3]) + (sc[c - 3 * kSZ] + sc[c + 3 * kSZ]), acc);
-      acc = fmaf(kA4, (sc[c - 4] + sc[c + 4]) + (sc[c - 4 * kSZ] + sc[c + 4 * kSZ]), acc);
-      acc = fmaf(kA5, (sc[c - 5] + sc[c + 5]) + (sc[c - 5 * kSZ] + sc[c + 5 * kSZ]), acc);
-    } else {  // 4th-order Laplacian: -5 p + 4/3 (ring 1) - 1/12 (ring 2)
-      acc = -5.f * sc[c];
-      acc = fmaf(4.f / 3.f, (sc[c - 1] + sc[c + 1]) + (sc[c - kSZ] + sc[c + kSZ]), acc);
-      acc = fmaf(-1.f / 12.f, (sc[c - 2] + sc[c + 2]) + (sc[c - 2 * kSZ] + sc[c + 2 * kSZ]), acc);
+  // u2 on the tile + 1 ring: each wavefront takes a run of rows and slides
+  // its lane's column (tile z = lane) over x in registers, so the x-direction
+  // terms are read from LDS once per row instead of once per tap; the ring's
+  // z = -1 and z = 64 points follow, one per thread of wave 0.  The terms
+  // are summed in the same order as lap_at below.
+  {
+    constexpr int R = ORDER == kOrder10 ? 5 : 2;
+    const int r0 = wave * kUX / (kBT / 64), r1 = (wave + 1) * kUX / (kBT / 64);
+    const float* col = sc + lane + kBH;  // staged column of tile z = lane
+    float w[2 * R + 1];                  // staged rows xs - R .. xs + R of the column
+#pragma unroll
+    for (int k = 0; k <= 2 * R; k++) w[k] = col[(r0 + kBH - 1 - R + k) * kSZ];
+    for (int xu = r0; xu < r1; xu++) {
+      const float* row = col + (xu + kBH - 1) * kSZ;
+      float acc;
+      if constexpr (ORDER == kOrder10) {
+        acc = -4.f * kA0 * w[5];
+        acc = fmaf(kA1, (row[-1] + row[1]) + (w[4] + w[6]), acc);
+        acc = fmaf(kA2, (row[-2] + row[2]) + (w[3] + w[7]), acc);
+        acc = fmaf(kA3, (row[-3] + row[3]) + (w[2] + w[8]), acc);
+        acc = fmaf(kA4, (row[-4] + row[4]) + (w[1] + w[9]), acc);
+        acc = fmaf(kA5, (row[-5] + row[5]) + (w[0] + w[10]), acc);
+      } else {
+        acc = -5.f * w[2];
+        acc = fmaf(4.f / 3.f, (row[-1] + row[1]) + (w[1] + w[3]), acc);
+        acc = fmaf(-1.f / 12.f, (row[-2] + row[2]) + (w[0] + w[4]), acc);
+      }
+      su[xu * kUZ + lane + 1] = acc;
+#pragma unroll
+      for (int k = 0; k < 2 * R; k++) w[k] = w[k + 1];
+      if (xu + 1 < r1) w[2 * R] = col[(xu + kBH + R) * kSZ];
     }
-    su[q] = acc;
+    if (threadIdx.x < 2 * kUX) {
+      const int xu = threadIdx.x >> 1, zu = (threadIdx.x & 1) ? kUZ - 1 : 0;
+      su[xu * kUZ + zu] = lap_at<ORDER>(sc, (xu + kBH - 1) * kSZ + zu + kBH - 1);
+    }
   }
   __syncthreads();
 
-  const int zl = threadIdx.x & (kBZ - 1), xg = threadIdx.x / kBZ;  // 4 consecutive x per thread
-  const int gz = oz + zl;
   const int rz = E.rec0 % ld, rx0 = E.rec0 / ld;
+  // the sponge (fd4t10s-damp-zjh.c:64-80) reaches this tile only near the
+  // left, right or bottom edge; elsewhere d = 0 and (2-2d+d^2, 1-2d) = (2, 1)
+  const bool sponge = ox < E.damp || ox + kBX > E.nxpad - E.damp || oz + kBZ > E.nzpad - E.damp;
   const bool snap_now = FWD && PREC != kSnapNone && (it % kSnapEvery == kSnapEvery - 1);
   const bool max_now = FWD && ((it + 1) % kSnapEvery == kSnapEvery - 1) && (it + 1) / kSnapEvery < E.nsnap;
   const bool image_now =
@@ -142,19 +222,23 @@ __global__ __launch_bounds__(kBT) void k_batch_step(BatchKernelArgs a) {
   if (image_now) scale = __uint_as_float(a.scale[E.scale0 + it / kSnapEvery - 1]);
   const float wimg = image_now ? a.wts[E.wts0 + it] : 0.f;
   const long long snap_off = E.snap + (long long)(FWD ? it / kSnapEvery : it / kSnapEvery - 1) * npts;
+  void* snapb = PREC == kSnapNone ? a.snap
+                : PREC == kSnapF32 ? static_cast<void*>(static_cast<float*>(a.snap) + snap_off)
+                                   : static_cast<void*>(static_cast<unsigned short*>(a.snap) + snap_off);
+  float* gradb = a.grad + E.grad;
   float vmax = 0.f;
 #pragma unroll
-  for (int j = 0; j < 4; j++) {
-    const int xl = xg * 4 + j;
+  for (int j = 0; j < kPX; j++) {
+    const int xl = xg * kPX + j;
     const int gx = ox + xl;
     if (gx >= E.nxpad - kBH || gz >= E.nzpad - kBH) continue;
     const int p = gx * ld + gz;
     const int uc = (xl + 1) * kUZ + zl + 1;
     const float u = su[uc];
-    const float r = a.rv[E.model + p];
+    const float r = rvj[j];
     const float c = sc[(xl + kBH) * kSZ + zl + kBH];
-    const float d = E.dscale * damping(gx, gz, E.nxpad, E.nzpad, E.damp);
-    const float pv = nxt[p];
+    const float d = sponge ? E.dscale * damping(gx, gz, E.nxpad, E.nzpad, E.damp) : 0.f;
+    const float pv = pvj[j];
     float v = fmaf(2.f - 2.f * d + d * d, c, -(1.f - 2.f * d) * pv);
     v = fmaf(r, u, v);
     if constexpr (ORDER == kOrder10) {  // the 4th-order-in-time term r^2/12 Lap5(u2)
@@ -167,7 +251,7 @@ __global__ __launch_bounds__(kBT) void k_batch_step(BatchKernelArgs a) {
       vmax = fmaxf(vmax, fabsf(v));
       if (rec && a.dcal) a.dcal[E.dcal + (long long)it * E.ng + (gx - rx0)] = c;
       if constexpr (PREC != kSnapNone) {
-        if (snap_now) snap_store<PREC>(a.snap, snap_off + p, c, scale);
+        if (snap_now) snap_store<PREC>(snapb, p, c, scale);
       }
       if (a.tap_k >= 0 && ((gx | (gz - E.tap_oz)) & 1) == 0) {  // a coarse point
         const int xc = (gx + kBH) >> 1, zc = (gz + kBH - E.tap_oz) >> 1;
@@ -177,12 +261,12 @@ __global__ __launch_bounds__(kBT) void k_batch_step(BatchKernelArgs a) {
     } else {
       if (inject_next && rec) v += a.dcal[E.dcal + (long long)(it - 1) * E.ng + (gx - rx0)];
       if (image_now && PREC != kSnapNone) {
-        const float s = snap_load<PREC == kSnapNone ? kSnapF32 : PREC>(a.snap, snap_off + p, scale);
-        float* g = a.grad + E.grad + p;
-        *g = *g - wimg * s * c;
+        const float s = snap_load<PREC == kSnapNone ? kSnapF32 : PREC>(snapb, p, scale);
+        float& g = at(gradb, p);
+        g = g - wimg * s * c;
       }
     }
-    nxt[p] = v;
+    at(nxt, p) = v;
   }
   if (max_now) {  // the next step's field is snapshot (it+1)/4: fold its max
     vmax = wave_max(vmax);
