@@ -68,6 +68,13 @@ void Batch::build() {
   std::vector<float> amp, wts;
   long long model_len = 0, dcal_len = 0, grad_len = 0;
   int mbase = 0;
+  // tile width from the batch's size in 64 x 16 tiles
+  long long t16 = 0;
+  for (const BatchGroup& G : g_) {
+    const long long ux = G.nx + 2 * G.nb, uz = G.nz + G.nb;  // update region
+    t16 += (long long)G.nmodels * (long long)G.sx.size() * ((uz + kBatchTileZ - 1) / kBatchTileZ) * ((ux + 15) / 16);
+  }
+  tile_x_ = t16 >= kBatchWideTiles ? 32 : 16;
   for (size_t gi = 0; gi < g_.size(); gi++) {
     const BatchGroup& G = g_[gi];
     const int ns = (int)G.sx.size();
@@ -109,7 +116,7 @@ void Batch::build() {
         e.nxpad = nxpad;
         e.nzpad = nzpad;
         e.ntz = (nzpad - 2 * kHalo + kBatchTileZ - 1) / kBatchTileZ;
-        e.ntx = (nxpad - 2 * kHalo + kBatchTileX - 1) / kBatchTileX;
+        e.ntx = (nxpad - 2 * kHalo + tile_x_ - 1) / tile_x_;
         e.damp = bx0;
         e.dscale = G.dscale;
         e.nt = G.nt;
@@ -240,7 +247,7 @@ void Batch::run_loop(bool fwd) {
       a.tap_k = -1;
       for (size_t k = 0; k < taps_.size(); k++)
         if (taps_[k] == it) a.tap_k = (int)k;
-      launch_batch_fwd(a, prec_, order_, s);
+      launch_batch_fwd(a, prec_, order_, tile_x_, s);
       if (hook_ && (it + 2) % kSnapEvery == 0) {
         // after step it the buffer step it wrote holds the field at it+1
         float* cur = (it & 1) == 0 ? fa_.get() : fb_.get();
@@ -259,7 +266,7 @@ void Batch::run_loop(bool fwd) {
       a.it_or_k = k;
       a.rot = k;
       a.nblocks = tile0_[act];
-      launch_batch_bwd(a, prec_, order_, s);
+      launch_batch_bwd(a, prec_, order_, tile_x_, s);
     }
   }
 }
@@ -352,7 +359,7 @@ void Batch::step_from(const float* prev_host, const float* cur_host, int it0, in
     a.it_or_k = it;
     a.rot = k;
     a.nblocks = tile0_[act];
-    launch_batch_fwd(a, kSnapNone, order_, s);
+    launch_batch_fwd(a, kSnapNone, order_, tile_x_, s);
     if (hook_ && (it + 2) % kSnapEvery == 0) {  // the correction, as in forward runs
       float* cur = (k & 1) == 0 ? fa_.get() : fb_.get();
       float* prev = (k & 1) == 0 ? fb_.get() : fa_.get();

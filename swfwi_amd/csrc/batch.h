@@ -18,10 +18,13 @@
 namespace swfwi {
 
 constexpr int kSnapEvery = 4;
-// Step-kernel tile: 64 z x kBatchTileX x output points per block of 256
-// threads (each thread kBatchTileX / 4 consecutive x at one z).
+// Step-kernel tile: 64 z x 16 or 32 x output points per block of 256
+// threads (each thread 4 or 8 consecutive x at one z); 32 when the batch
+// holds at least kBatchWideTiles tiles of 64 x 16 (measured: 16 entries of
+// the 501x1501 grid, 14 112 such tiles, run 7% faster with 32; 16 entries of
+// the 793x277 padded coarse grid, 3 920, 8% slower).
 constexpr int kBatchTileZ = 64;
-constexpr int kBatchTileX = 32;
+constexpr int kBatchWideTiles = 8000;
 enum SnapPrec : int { kSnapF32 = 0, kSnapBF16 = 1, kSnapF16 = 2, kSnapNone = 3 };
 // Stencils: the reference's damped 10th-order-in-space, 4th-order-in-time
 // scheme (fd4t10s-damp-zjh.c), or a damped 4th-order-in-space, 2nd-order-in-
@@ -87,9 +90,9 @@ struct BatchKernelArgs {
 };
 
 void launch_batch_init_fwd(const BatchKernelArgs& a, int nentries, hipStream_t s);
-void launch_batch_fwd(const BatchKernelArgs& a, int prec, int order, hipStream_t s);
+void launch_batch_fwd(const BatchKernelArgs& a, int prec, int order, int tile_x, hipStream_t s);
 void launch_batch_init_bwd(const BatchKernelArgs& a, int nentries, hipStream_t s);
-void launch_batch_bwd(const BatchKernelArgs& a, int prec, int order, hipStream_t s);
+void launch_batch_bwd(const BatchKernelArgs& a, int prec, int order, int tile_x, hipStream_t s);
 void launch_fill_u32(unsigned* p, long long n, unsigned v, hipStream_t s);
 void launch_sub(const float* a, const float* b, float* out, long long n, hipStream_t s);  // out = a - b
 
@@ -183,6 +186,7 @@ class Batch {
   std::vector<int> run_of_;          // constructor order -> run index
   std::vector<Layout> lay_;
   std::vector<int> tile0_;           // per run index, prefix of tiles
+  int tile_x_ = 16;                  // step-kernel tile width
   std::vector<long long> model_off_; // per model (groups in order)
   std::vector<int> model_group_;
   long long fields_ = 0, models_len_ = 0, dcal_len_ = 0, grad_len_ = 0, snap_elems_ = 0;

@@ -1,10 +1,10 @@
 // Step kernels of the mixed-size batch (batch.h): one launch advances every
-// active entry by one time step, each block one 64 (z) x 32 (x) tile of one
+// active entry by one time step, each block one 64 (z) x 16 or 32 (x) tile of one
 // entry.  Per block: stage the entry's current field with a 6-cell halo in
 // LDS (whole rows per wavefront), form the 10th-order Laplacian u2 in LDS
 // (each lane sliding its z column over x, the x-direction terms in
 // registers), then the damped leapfrog update of fd4t10s-damp-zjh.c:14-115
-// in FP32 (fused multiply-adds), each thread 8 consecutive x at one z with
+// in FP32 (fused multiply-adds), each thread 4 or 8 consecutive x at one z with
 // its model and previous field loaded before the staging.
 //
 // Forward (FWD): the next step's source amplitude is added to `next`
@@ -29,13 +29,10 @@ namespace swfwi {
 namespace {
 
 constexpr int kBT = 256;
-constexpr int kBZ = kBatchTileZ, kBX = kBatchTileX, kBH = 6;
-constexpr int kPX = kBX / (kBT / kBZ);  // outputs per thread, consecutive in x
-static_assert(kBZ == 64 && kBX % (kBT / kBZ) == 0, "a wavefront covers one z column of the tile");
+constexpr int kBZ = kBatchTileZ, kBH = 6;
+static_assert(kBZ == 64, "a wavefront covers one z column of the tile");
 constexpr int kSZ = kBZ + 2 * kBH;  // staged z
-constexpr int kSX = kBX + 2 * kBH;  // staged x
 constexpr int kUZ = kBZ + 2;        // u2 z (outputs +-1)
-constexpr int kUX = kBX + 2;
 constexpr float kF16Scale = 16384.f;  // snapshot max maps to 2^14 in FP16
 
 // Zhang Jinhai coefficients as the reference stores them (fd4t10s-damp-zjh.c:24-29)
@@ -116,8 +113,13 @@ __device__ __forceinline__ float lap_at(const float* sc, int c) {
   return acc;
 }
 
-template <int PREC, bool FWD, int ORDER>
+template <int PREC, bool FWD, int ORDER, int TX>
 __global__ __launch_bounds__(kBT) void k_batch_step(BatchKernelArgs a) {
+  constexpr int kBX = TX;                 // tile x
+  constexpr int kPX = kBX / (kBT / kBZ);  // outputs per thread, consecutive in x
+  constexpr int kSX = kBX + 2 * kBH;      // staged x
+  constexpr int kUX = kBX + 2;            // u2 x (outputs +-1)
+  static_assert(kBX % (kBT / kBZ) == 0, "the tile's x splits over the 4 thread rows");
   __shared__ float sc[kSX * kSZ];
   __shared__ float su[kUX * kUZ];
   __shared__ float sm[kBT / 64];
@@ -328,35 +330,36 @@ void launch_batch_init_bwd(const BatchKernelArgs& a, int nentries, hipStream_t s
   hipLaunchKernelGGL(k_batch_inject0, dim3(nentries), dim3(kBT), 0, s, a, nentries, 0);
 }
 
-template <bool FWD, int ORDER>
+template <bool FWD, int ORDER, int TX>
 static void launch_step(const BatchKernelArgs& a, int prec, hipStream_t s) {
   if (a.nblocks <= 0) return;
   switch (prec) {
     case kSnapF32:
-      hipLaunchKernelGGL((k_batch_step<kSnapF32, FWD, ORDER>), dim3(a.nblocks), dim3(kBT), 0, s, a);
+      hipLaunchKernelGGL((k_batch_step<kSnapF32, FWD, ORDER, TX>), dim3(a.nblocks), dim3(kBT), 0, s, a);
       break;
     case kSnapBF16:
-      hipLaunchKernelGGL((k_batch_step<kSnapBF16, FWD, ORDER>), dim3(a.nblocks), dim3(kBT), 0, s, a);
+      hipLaunchKernelGGL((k_batch_step<kSnapBF16, FWD, ORDER, TX>), dim3(a.nblocks), dim3(kBT), 0, s, a);
       break;
     case kSnapF16:
-      hipLaunchKernelGGL((k_batch_step<kSnapF16, FWD, ORDER>), dim3(a.nblocks), dim3(kBT), 0, s, a);
+      hipLaunchKernelGGL((k_batch_step<kSnapF16, FWD, ORDER, TX>), dim3(a.nblocks), dim3(kBT), 0, s, a);
       break;
     default:
-      hipLaunchKernelGGL((k_batch_step<kSnapNone, FWD, ORDER>), dim3(a.nblocks), dim3(kBT), 0, s, a);
+      hipLaunchKernelGGL((k_batch_step<kSnapNone, FWD, ORDER, TX>), dim3(a.nblocks), dim3(kBT), 0, s, a);
       break;
   }
 }
-void launch_batch_fwd(const BatchKernelArgs& a, int prec, int order, hipStream_t s) {
+template <bool FWD>
+static void launch_dir(const BatchKernelArgs& a, int prec, int order, int tile_x, hipStream_t s) {
   if (order == kOrder4)
-    launch_step<true, kOrder4>(a, prec, s);
+    tile_x == 32 ? launch_step<FWD, kOrder4, 32>(a, prec, s) : launch_step<FWD, kOrder4, 16>(a, prec, s);
   else
-    launch_step<true, kOrder10>(a, prec, s);
+    tile_x == 32 ? launch_step<FWD, kOrder10, 32>(a, prec, s) : launch_step<FWD, kOrder10, 16>(a, prec, s);
 }
-void launch_batch_bwd(const BatchKernelArgs& a, int prec, int order, hipStream_t s) {
-  if (order == kOrder4)
-    launch_step<false, kOrder4>(a, prec, s);
-  else
-    launch_step<false, kOrder10>(a, prec, s);
+void launch_batch_fwd(const BatchKernelArgs& a, int prec, int order, int tile_x, hipStream_t s) {
+  launch_dir<true>(a, prec, order, tile_x, s);
+}
+void launch_batch_bwd(const BatchKernelArgs& a, int prec, int order, int tile_x, hipStream_t s) {
+  launch_dir<false>(a, prec, order, tile_x, s);
 }
 
 }  // namespace swfwi
