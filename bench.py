@@ -259,6 +259,7 @@ def learned_workload(args):
     iters = args.iters
     log = (lambda m: print(m, file=sys.stderr, flush=True))
     out = {}
+    finals = {}
     runs = (("plain_fine_fp32", fine, capi.ORDER_10, v0, dobs_f, None),
             ("learned_coarse", coarse, capi.ORDER_4, learned.restrict(v0), dobs_c, net),
             ("plain_coarse", coarse, capi.ORDER_4, learned.restrict(v0), dobs_c, None))
@@ -272,7 +273,40 @@ def learned_workload(args):
         out[name] = {"gradient_ms_per_shot": round(gms, 3), "gpts_per_s": round(pts / (gms * len(shots) * 1e-3) / 1e9, 2),
                      "fine_judged_misfit": float(f"{misfit:.6e}"), "own_grid_misfit": [float(f"{o:.5e}") for o in objs],
                      "wall_s": round(wall, 1)}
+        finals[name] = (vf, objs, misfit)
     m0 = learned.fine_misfit(case, fine, v0, dobs_f)
+    # equal misfit, two ways.  (1) The plain fine run's own objectives are
+    # the fine-judged misfits of its successive models: the first of them at
+    # or below the learned run's final misfit gives the fine iterations that
+    # reach the same misfit.  (2) Fine fallback: from the learned run's final
+    # model the fine propagator iterates on; the first model within 2% of the
+    # plain fine run's final misfit ends it, and the hybrid is charged the
+    # coarse and the fine gradients it used.
+    _, objs_f, m_plain = finals["plain_fine_fp32"]
+    m_learned = finals["learned_coarse"][2]
+    seq = list(objs_f) + [m_plain]  # misfit after k = 0 .. iters fine updates
+    k_eq = next((k for k, m in enumerate(seq) if m <= m_learned), None)
+    t_f, t_l = out["plain_fine_fp32"]["gradient_ms_per_shot"], out["learned_coarse"]["gradient_ms_per_shot"]
+    print(f"[bench] fine fallback from the learned model", file=sys.stderr, flush=True)
+    _, objs_h, gms_h, _ = learned.fwi_run(fine, capi.ORDER_10, finals["learned_coarse"][0], dobs_f, iters, case["vmin"],
+                                          case["vmax"], log=log)
+    k_h = next((k for k, m in enumerate(objs_h) if m <= 1.02 * m_plain), None)
+    # the same fallback from the coarse run without the network (the control)
+    print(f"[bench] fine fallback from the plain coarse model", file=sys.stderr, flush=True)
+    _, objs_c, gms_c, _ = learned.fwi_run(fine, capi.ORDER_10, finals["plain_coarse"][0], dobs_f, iters, case["vmin"],
+                                          case["vmax"], log=log)
+    k_c = next((k for k, m in enumerate(objs_c) if m <= 1.02 * m_plain), None)
+    t_c = out["plain_coarse"]["gradient_ms_per_shot"]
+    equal = {
+        "plain_fine_iterations_to_learned_misfit": k_eq,
+        "speedup_to_learned_misfit": round(k_eq * t_f / (iters * t_l), 3) if k_eq else None,
+        "fallback_fine_iterations_to_within_2pct": k_h,
+        "fallback_misfits": [float(f"{o:.5e}") for o in objs_h],
+        "speedup_with_fallback": round(iters * t_f / (iters * t_l + k_h * gms_h), 3) if k_h is not None else None,
+        "control_plain_coarse_fallback_fine_iterations": k_c,
+        "control_plain_coarse_fallback_misfits": [float(f"{o:.5e}") for o in objs_c],
+        "control_speedup_with_fallback": round(iters * t_f / (iters * t_c + k_c * gms_c), 3) if k_c is not None else None,
+        "note": "gradient time per shot summed over the iterations each method needs to reach the misfit"}
     # the network kernel alone: one launch over the 16 coarse fields of a
     # correction (the forward of the learned run holds 2 per 4 steps)
     from scipy.ndimage import gaussian_filter
@@ -305,6 +339,7 @@ def learned_workload(args):
             "equal_misfit_within_2pct": bool(abs(ratio - 1.0) <= 0.02),
             "fine_misfit_initial_model": float(f"{m0:.6e}"),
             "plain_coarse_fine_misfit_ratio": round(pc["fine_judged_misfit"] / pf["fine_judged_misfit"], 4),
+            "at_equal_misfit": equal,
         },
         "roofline": {"bound": "mfma", "achieved": round(tfs, 1), "peak": 2500.0, "unit": "TFLOP/s",
                      "frac": round(tfs / 2500.0, 4), "traffic": None, "kernel": "k_net_apply",
