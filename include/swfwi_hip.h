@@ -264,7 +264,8 @@ int swfwi_batch_step_from(swfwi_batch b, const float *prev, const float *cur, in
                           float *prev_out, float *cur_out);
 /* The learned correction p <- p + net(p) on both time levels of every entry
  * every 4 forward steps, over the physical interior (null: off).
- * Single-group batches. */
+ * Single-group batches.  The batch keeps a pointer to the network: destroy
+ * the network only after the batch, or after setting null here. */
 int swfwi_batch_set_correction(swfwi_batch b, swfwi_net net);
 
 /* ---- learned coarse-grid correction network --------------------------------
