@@ -84,9 +84,12 @@ class ConvNet {
   NetDev dev() const { return NetDev{w1a_.get(), w2f_.get(), w2b_.get(), w3a_.get(), w3_.get()}; }
   // p <- p + net(p) over the region, for a stack of fields (device)
   void correct(float* p, const NetGeom& g, float* yscratch, hipStream_t s) const;
-  // One Adam step on the batch (inputs, targets on the device): returns the
-  // loss sum_r r^2 / count before the step.
-  double train_step(const float* in, const float* target, const NetGeom& g, float lr, hipStream_t s);
+  // One Adam step on the batch (inputs, targets on the device); the loss
+  // sum_r r^2 before the step into *loss_dev (device, FP64).
+  void train_step(const float* in, const float* target, const NetGeom& g, float lr, double* loss_dev,
+                  hipStream_t s);
+  // loss sum into *loss_dev and the weight gradient, without a wait
+  void loss_grad_async(const float* in, const float* target, const NetGeom& g, double* loss_dev, hipStream_t s);
   // loss and gradient without an update (for checks)
   double loss_grad(const float* in, const float* target, const NetGeom& g, float* grad_host, hipStream_t s);
   int steps() const { return step_; }

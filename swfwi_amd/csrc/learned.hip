@@ -22,6 +22,7 @@
 // accumulator item v.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <utility>
 #include <vector>
@@ -350,13 +351,26 @@ __global__ __launch_bounds__(kT) void k_net_backward(const float* __restrict__ p
   }
 }
 
-// grad[j] = sum over blocks of parts[b][j], in block order (FP64)
+// grad[j] = sum over blocks of parts[b][j] in FP64, in a fixed order: each
+// block takes 16 weights; its 16 thread groups sum consecutive runs of the
+// partials in block order, then the 16 run sums are added in run order
+constexpr int kRedJ = 16, kRedRuns = kT / kRedJ;
 __global__ void k_net_reduce(const float* parts, long long nblocks, float* grad) {
-  const int j = blockIdx.x * kT + threadIdx.x;
-  if (j >= kNetParams) return;
+  __shared__ double run[kRedRuns][kRedJ];
+  const int jl = threadIdx.x % kRedJ, r = threadIdx.x / kRedJ;
+  const int j = blockIdx.x * kRedJ + jl;
+  const long long per = (nblocks + kRedRuns - 1) / kRedRuns;
+  const long long b0 = r * per, b1 = b0 + per < nblocks ? b0 + per : nblocks;
   double s = 0;
-  for (long long b = 0; b < nblocks; b++) s += parts[b * kNetParams + j];
-  grad[j] = (float)s;
+  if (j < kNetParams)
+    for (long long b = b0; b < b1; b++) s += parts[b * kNetParams + j];
+  run[r][jl] = s;
+  __syncthreads();
+  if (r == 0 && j < kNetParams) {
+    double t = 0;
+    for (int q = 0; q < kRedRuns; q++) t += run[q][jl];
+    grad[j] = (float)t;
+  }
 }
 
 __global__ void k_net_pack(const float* theta, uint16_t* w1a, uint16_t* w2f, uint16_t* w2b, uint16_t* w3a,
@@ -397,11 +411,33 @@ __global__ void k_net_adam(float* theta, const float* grad, float* m, float* v, 
   theta[i] -= lr * (mi * c1) / (sqrtf(vi * c2) + eps);
 }
 
+// *out = sum of parts[0..n) in FP64 in a fixed order: consecutive runs per
+// thread, then the run sums pairwise in a fixed tree
 __global__ void k_sum_parts(const double* parts, int n, double* out) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    double s = 0;
-    for (int i = 0; i < n; i++) s += parts[i];
-    *out = s;
+  __shared__ double red[kT];
+  const int per = (n + kT - 1) / kT, i0 = threadIdx.x * per;
+  double s = 0;
+  for (int i = i0; i < i0 + per && i < n; i++) s += parts[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = kT / 2; h > 0; h >>= 1) {
+    if (threadIdx.x < h) red[threadIdx.x] += red[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+// the minibatch: sample idx[b] of the training set into slot b, for the
+// inputs, models and targets at once
+__global__ void k_gather3(float* din, float* dr, float* dt, const float* sin, const float* sr, const float* st,
+                          const int* idx, int batch, long long stride) {
+  const long long n = (long long)batch * stride;
+  for (long long i = blockIdx.x * (long long)kT + threadIdx.x; i < n; i += (long long)gridDim.x * kT) {
+    const long long b = i / stride, o = i - b * stride;
+    const long long src = (long long)idx[b] * stride + o;
+    din[i] = sin[src];
+    dr[i] = sr[src];
+    dt[i] = st[src];
   }
 }
 
@@ -451,7 +487,7 @@ void net_backward(const float* p, const float* dy, const NetGeom& g, const NetDe
   check_geom(g, 2);
   const long long nb = tiles_of(g, 2);
   hipLaunchKernelGGL(k_net_backward, dim3((unsigned)nb), dim3(kT), 0, s, p, dy, g, w, parts);
-  hipLaunchKernelGGL(k_net_reduce, dim3((kNetParams + kT - 1) / kT), dim3(kT), 0, s, parts, nb, grad);
+  hipLaunchKernelGGL(k_net_reduce, dim3((kNetParams + kRedJ - 1) / kRedJ), dim3(kT), 0, s, parts, nb, grad);
 }
 
 void net_adam(float* theta, const float* grad, float* m, float* v, int step, float lr, float b1, float b2, float eps,
@@ -462,7 +498,7 @@ void net_adam(float* theta, const float* grad, float* m, float* v, int step, flo
 }
 
 void net_reduce_parts(const double* parts, int nparts, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, s, parts, nparts, out);
+  hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(kT), 0, s, parts, nparts, out);
 }
 
 // ---- host handle -------------------------------------------------------------
@@ -501,7 +537,9 @@ void ConvNet::correct(float* p, const NetGeom& g, float* yscratch, hipStream_t s
   net_add(p, yscratch, g, s);
 }
 
-double ConvNet::loss_grad(const float* in, const float* target, const NetGeom& g, float* grad_host, hipStream_t s) {
+// loss into *loss_dev (sum of r^2, FP64) and the weight gradient into grad_
+void ConvNet::loss_grad_async(const float* in, const float* target, const NetGeom& g, double* loss_dev,
+                              hipStream_t s) {
   const size_t n = (size_t)g.stride * g.entries;
   y_.resize(n);
   dy_.resize(n);
@@ -509,8 +547,12 @@ double ConvNet::loss_grad(const float* in, const float* target, const NetGeom& g
   net_apply(in, y_.get(), g, dev(), s);
   const int np = 1024;
   net_loss(in, y_.get(), target, dy_.get(), g, lparts_.get(), np, s);
-  net_reduce_parts(lparts_.get(), np, lsum_.get(), s);
+  net_reduce_parts(lparts_.get(), np, loss_dev, s);
   net_backward(in, dy_.get(), g, dev(), parts_.get(), grad_.get(), s);
+}
+
+double ConvNet::loss_grad(const float* in, const float* target, const NetGeom& g, float* grad_host, hipStream_t s) {
+  loss_grad_async(in, target, g, lsum_.get(), s);
   double loss = 0;
   SWFWI_HIP(hipMemcpyAsync(&loss, lsum_.get(), sizeof(double), hipMemcpyDeviceToHost, s));
   if (grad_host) SWFWI_HIP(hipMemcpyAsync(grad_host, grad_.get(), sizeof(float) * kNetParams, hipMemcpyDeviceToHost, s));
@@ -519,12 +561,12 @@ double ConvNet::loss_grad(const float* in, const float* target, const NetGeom& g
   return loss / count;
 }
 
-double ConvNet::train_step(const float* in, const float* target, const NetGeom& g, float lr, hipStream_t s) {
-  const double loss = loss_grad(in, target, g, nullptr, s);
+void ConvNet::train_step(const float* in, const float* target, const NetGeom& g, float lr, double* loss_dev,
+                         hipStream_t s) {
+  loss_grad_async(in, target, g, loss_dev, s);
   step_++;
   net_adam(theta_.get(), grad_.get(), m_.get(), v_.get(), step_, lr, 0.9f, 0.999f, 1e-8f, s);
   repack(s);
-  return loss;
 }
 
 void ConvNet::set_data(const float* in_host, const float* r_host, const float* target_host, int n,
@@ -546,6 +588,7 @@ void ConvNet::set_data(const float* in_host, const float* r_host, const float* t
 void ConvNet::train(int batch, int steps, float lr, unsigned long long seed, double* losses, hipStream_t s) {
   if (!dn_) fail(kErrState, "no training set");
   if (batch < 1 || batch > dn_) fail(kErrArg, "minibatch size out of range");
+  if (steps < 0) fail(kErrArg, "negative step count");
   NetGeom g = dg_;
   g.entries = batch;
   bin_.resize((size_t)batch * g.stride);
@@ -554,8 +597,9 @@ void ConvNet::train(int batch, int steps, float lr, unsigned long long seed, dou
   g.r = br_.get();
   g.rstride = g.stride;
   g.rdiv = 1;
-  // seeded shuffle of the sample order (splitmix64 + Fisher-Yates), refreshed each pass
-  std::vector<int> order(dn_);
+  // seeded shuffle of the sample order (splitmix64 + Fisher-Yates), refreshed
+  // each pass; the minibatch of step k is idx[k * batch .. (k + 1) * batch)
+  std::vector<int> order(dn_), idx((size_t)steps * batch);
   for (int i = 0; i < dn_; i++) order[i] = i;
   unsigned long long st = seed;
   auto next = [&]() {
@@ -566,24 +610,31 @@ void ConvNet::train(int batch, int steps, float lr, unsigned long long seed, dou
     return z ^ (z >> 31);
   };
   int pos = dn_;
-  for (int k = 0; k < steps; k++) {
-    for (int b = 0; b < batch; b++) {
-      if (pos == dn_) {
-        for (int i = dn_ - 1; i > 0; i--) std::swap(order[i], order[(int)(next() % (unsigned long long)(i + 1))]);
-        pos = 0;
-      }
-      const int smp = order[pos++];
-      SWFWI_HIP(hipMemcpyAsync(bin_.get() + (size_t)b * g.stride, din_.get() + (size_t)smp * g.stride,
-                               sizeof(float) * g.stride, hipMemcpyDeviceToDevice, s));
-      SWFWI_HIP(hipMemcpyAsync(btgt_.get() + (size_t)b * g.stride, dtgt_.get() + (size_t)smp * g.stride,
-                               sizeof(float) * g.stride, hipMemcpyDeviceToDevice, s));
-      SWFWI_HIP(hipMemcpyAsync(br_.get() + (size_t)b * g.stride, dr_.get() + (size_t)smp * g.stride,
-                               sizeof(float) * g.stride, hipMemcpyDeviceToDevice, s));
+  for (size_t q = 0; q < idx.size(); q++) {
+    if (pos == dn_) {
+      for (int i = dn_ - 1; i > 0; i--) std::swap(order[i], order[(int)(next() % (unsigned long long)(i + 1))]);
+      pos = 0;
     }
-    const double l = train_step(bin_.get(), btgt_.get(), g, lr, s);
-    if (losses) losses[k] = l;
+    idx[q] = order[pos++];
   }
+  if (steps == 0) return;
+  DBuf<int> didx;
+  DBuf<double> dloss;
+  didx.upload(idx.data(), idx.size(), s);
+  dloss.resize((size_t)steps);
+  const long long total = (long long)batch * g.stride;
+  const unsigned gblocks = (unsigned)std::min<long long>((total + kT - 1) / kT, 2048);
+  for (int k = 0; k < steps; k++) {
+    hipLaunchKernelGGL(k_gather3, dim3(gblocks), dim3(kT), 0, s, bin_.get(), br_.get(), btgt_.get(), din_.get(),
+                       dr_.get(), dtgt_.get(), didx.get() + (size_t)k * batch, batch, g.stride);
+    train_step(bin_.get(), btgt_.get(), g, lr, dloss.get() + k, s);
+  }
+  std::vector<double> l((size_t)steps);
+  dloss.download(l.data(), l.size(), s);
   SWFWI_HIP(hipStreamSynchronize(s));
+  const double count = (double)(g.x1 - g.x0) * (g.z1 - g.z0) * g.entries;
+  if (losses)
+    for (int k = 0; k < steps; k++) losses[k] = l[k] / count;
 }
 
 }  // namespace swfwi
