@@ -143,3 +143,39 @@ def test_batch_gradient_against_fp64_oracle(oracle, run, prec):
     assert info["entries"] == len(b.entries) and info["snapshot_bytes"] > 0
     if prec != capi.SNAP_F32:
         b.close()
+
+
+def test_wide_tile_batch_against_oracles(oracle):
+    """A batch big enough for the 64 x 32 step tiles (24 entries of the
+    301x921 grid: 8928 tiles of 64 x 16, above the 8000 at which the batch
+    switches to 32-wide tiles), nt=800: seismograms of the first and last
+    entry against the FP32 oracle, their gradients from FP32 checkpoints
+    against the FP64 oracle run."""
+    g = Group(nx=921, nz=301, nb=30, dx=12.5, dt=0.001, fm=10.0, nt=800,
+              sx=np.linspace(0, 920, 12).astype(np.int32), sz=np.ones(12, np.int32), nmodels=2)
+    ux, uz = g.nx + 2 * g.nb, g.nz + g.nb
+    assert g.nmodels * len(g.sx) * ((uz + 63) // 64) * ((ux + 15) // 16) >= 8000
+    b = Batch([g], snap_prec=capi.SNAP_F32)
+    mods = [models_for(g, 7 + m) for m in range(2)]
+    for m in range(2):
+        b.bind_model(0, m, mods[m][0])
+    dobs = b.forward()
+    for m in range(2):
+        b.bind_model(0, m, mods[m][1])
+    D = b.split_data(b.forward())
+    V = b.split_data(b.adjoint_source(dobs))
+    GE = b.split_grid(b.backward(None, want_entries=True, want_models=False)[0])
+    og = oracle_geom(oracle, g)
+    wlt = oracle.ricker(g.nt, g.fm, g.dt, 1.0)
+    mask = inner_mask(g)
+    for k in (0, len(b.entries) - 1):
+        _, m, s = b.entries[k]
+        want = og.forward_modeling(mods[m][1], wlt, shot=s)
+        e = rel_l2(D[k], want)
+        print(f"  entry {k} (model {m}, shot {s}): seismogram rel-L2 {e:.2e}", flush=True)
+        assert e <= SEIS_TOL
+        gw = og.calgradient64(mods[m][1], wlt, V[k], shot=s)
+        eg = rel_l2(GE[k][mask], gw[mask])
+        print(f"  entry {k}: gradient rel-L2 vs FP64 {eg:.2e}", flush=True)
+        assert eg <= GRAD_TOL
+    b.close()
