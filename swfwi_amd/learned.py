@@ -246,6 +246,101 @@ def train(case, models, rounds=3, steps_per_round=3000, starts=None, batch=16, l
     return net, hist
 
 
+def crop_pairs(case, n_models, rnd, net=None, seed=0, chunk=100, starts=2, crop=112, crops=2, floor=0.1, log=None):
+    """Training pairs of round `rnd` from full-size fine runs, as crops.
+
+    n_models seeded random layered models (`seed + k`, the config-0 recipe)
+    in chunks of `chunk`: each chunk is one fine batch (one shot per model, at
+    a seeded even x on fine row 3, the same for the chunk) tapped at the
+    coarse points; from each of `starts` seeded coarse start steps j (3 mod 4)
+    the coarse leapfrog runs 4 rnd corrected steps and then 4 plain ones from
+    the fine wavefield at (j-1, j), and the fine wavefield at (j+4rnd+3,
+    j+4rnd+4) is the target, as in rollout_pairs.  From every (model, start,
+    time level) `crops` windows of crop x crop coarse cells are cut, centred
+    on cells where the target's magnitude is at least `floor` times its maximum
+    (the network is a 7x7-cell local operator: a window with a 5-cell margin
+    trains it as the whole field would).  Returns inputs, 1 / vel_t and
+    targets [n][crop][crop]."""
+    nt_c = case["nt"] // 2
+    ins, rvs, tgts = [], [], []
+    for c0 in range(0, n_models, chunk):
+        m = min(chunk, n_models - c0)
+        rng = np.random.default_rng(seed + 7919 * (c0 // chunk + 1))
+        nx = case["nx"]
+        sx = int(rng.integers(nx // 8, 7 * nx // 8)) & ~1
+        js = sorted(int(j) for j in rng.choice(np.arange(11, nt_c - 4 * rnd - 8, 4), size=starts, replace=False))
+        models = random_models(case, m, seed + c0)
+        fine, coarse = grids(case, [sx])
+        slot, taps = {}, []
+        for j in js:
+            for k in (j - 1, j, j + 4 * rnd + 3, j + 4 * rnd + 4):
+                if k not in slot:
+                    slot[k] = len(taps)
+                    taps.append(2 * k)
+        fb = _batch(with_models(fine, m), capi.SNAP_NONE, capi.ORDER_10)
+        for q in range(m):
+            fb.bind_model(0, q, expand(models[q], fine))
+        fb.set_taps(taps, oz=OZ)
+        fb.forward(want_data=False)
+        T = np.stack(fb.taps())  # [model][tap][x][z]
+        fb.close()
+        cb = _batch(with_models(coarse, m), capi.SNAP_NONE, capi.ORDER_4)
+        rv = []
+        for q in range(m):
+            vc = expand(restrict(models[q]), coarse)
+            cb.bind_model(0, q, vc)
+            rv.append((1.0 / vc).astype(np.float32))
+        shape = T.shape[2:]
+        for j in js:
+            prev, cur = T[:, slot[j - 1]], T[:, slot[j]]
+            if rnd > 0:
+                cb.set_correction(net)
+                prev, cur = cb.step_from(prev, cur, it0=j, nsteps=4 * rnd)
+                cb.set_correction(None)
+            prev, cur = cb.step_from(prev, cur, it0=j + 4 * rnd, nsteps=4)
+            pairs = ((prev.reshape((m,) + shape), T[:, slot[j + 4 * rnd + 3]]),
+                     (cur.reshape((m,) + shape), T[:, slot[j + 4 * rnd + 4]]))
+            for q in range(m):
+                for inp, tgt in pairs:
+                    t = tgt[q]
+                    mag = np.abs(t)
+                    cand = np.argwhere(mag >= floor * mag.max()) if mag.max() > 0 else np.argwhere(mag >= 0)
+                    for ci in rng.choice(len(cand), size=crops):
+                        x0 = int(np.clip(cand[ci][0] - crop // 2, 0, shape[0] - crop))
+                        z0 = int(np.clip(cand[ci][1] - crop // 2, 0, shape[1] - crop))
+                        w = (slice(x0, x0 + crop), slice(z0, z0 + crop))
+                        ins.append(inp[q][w])
+                        rvs.append(rv[q][w])
+                        tgts.append(t[w])
+        cb.close()
+        if log:
+            log(f"  round {rnd}: models {c0 + m}/{n_models}, {len(ins)} crops")
+    f = lambda a: np.ascontiguousarray(np.stack(a), np.float32)  # noqa: E731
+    return f(ins), f(rvs), f(tgts)
+
+
+def train_crops(case, n_models, rounds=3, steps_per_round=3000, batch=16, lr=1e-3, noise=0.02, copies=2, seed=0,
+                log=None, **kw):
+    """train() on crop_pairs: round 0 from the fine wavefield, later rounds
+    from the network's own corrected propagation, noise-augmented inputs,
+    everything gathered so far trained on.  Returns (net, losses per round)."""
+    rng = np.random.default_rng(seed + 7)
+    net = Net(Net.initial(seed=seed, scale=1e-3))
+    data, hist = None, []
+    for r in range(rounds):
+        ins, rvs, tgts = crop_pairs(case, n_models, r, net if r else None, seed=seed, log=log, **kw)
+        new = (_noisy(ins, noise, copies, rng), np.concatenate([rvs] * copies), np.concatenate([tgts] * copies))
+        data = new if data is None else tuple(np.concatenate([a, b]) for a, b in zip(data, new))
+        data = tuple(np.ascontiguousarray(a, np.float32) for a in data)
+        c = ins.shape[1]
+        net.set_data(*data, (5, c - 5, 5, c - 5))
+        losses = net.train(batch=min(batch, len(data[0])), steps=steps_per_round, lr=lr, seed=seed + r)
+        hist.append(losses)
+        if log:
+            log(f"round {r}: {len(data[0])} crops, loss {losses[0]:.3e} -> {losses[-50:].mean():.3e}")
+    return net, hist
+
+
 def with_gain(net, gain):
     """The network with its output scaled by `gain` (the last layer's
     weights times gain; the network is linear in them)."""
