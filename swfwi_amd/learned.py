@@ -405,15 +405,18 @@ def _refill(vt, grid):
     return np.ascontiguousarray(vt[np.ix_(ix, iz)])
 
 
-def fwi_run(grid, order, v0, dobs, iters, vmin, vmax, net=None, dv=30.0, snap_prec=None, log=None):
+def fwi_run(grid, order, v0, dobs, iters, vmin, vmax, net=None, dv=30.0, snap_prec=None, log=None, keep=(),
+            stop_at=None):
     """`iters` FWI iterations on one grid: every shot's gradient in one
     batched forward (checkpoints) + adjoint source + backward, PR+ conjugate
     gradient, a two-trial parabolic step on the summed objective, clamped
     update of the transformed model and boundary refill.  dobs: [shot][nt][ng]
     on this grid's receivers and samples.  Returns the final physical model,
     the objectives, the gradient time per shot (forward + backward on the
-    device, ms; median over the iterations) and the stencil point updates of
-    one gradient of all shots."""
+    device, ms; median over the iterations), the stencil point updates of
+    one gradient of all shots, and {k: physical model after k updates} for
+    the k in `keep`.  With `stop_at`, the run ends at the first model whose
+    objective is at or below it (the objectives then end with that one)."""
     prec = capi.SNAP_BF16 if snap_prec is None else snap_prec
     S = len(grid["sx"])
     b = _batch(grid, prec, order)
@@ -434,10 +437,20 @@ def fwi_run(grid, order, v0, dobs, iters, vmin, vmax, net=None, dv=30.0, snap_pr
 
     objs, gms, pts = [], [], 0.0
     g_prev = d_prev = None
+    x0, x1, z0, z1 = region(grid)
+
+    def physical(m):
+        v = dx / (dt * np.sqrt(m.astype(np.float64)))
+        return np.ascontiguousarray(v[x0:x1, z0:z1], np.float32)
+
+    kept = {}
     for it in range(iters):
         b.bind_model(0, 0, vt)
         dcal = b.forward()
         J = float(np.sum((obs.astype(np.float64) - dcal) ** 2))
+        if stop_at is not None and J <= stop_at:
+            objs.append(J)
+            break
         b.adjoint_source(obs, want=False)
         _, gm = b.backward(None, want_entries=False)
         info = b.info()
@@ -468,11 +481,11 @@ def fwi_run(grid, order, v0, dobs, iters, vmin, vmax, net=None, dv=30.0, snap_pr
         objs.append(J)
         if log:
             log(f"  iter {it}: objective {J:.5e}, gradient {gms[-1]:.2f} ms/shot, step {alpha:.3e}")
+        if it + 1 in keep:
+            kept[it + 1] = physical(vt)
     b.close()
     tr.close()
-    vfin = dx / (dt * np.sqrt(vt.astype(np.float64)))
-    x0, x1, z0, z1 = region(grid)
-    return np.ascontiguousarray(vfin[x0:x1, z0:z1], np.float32), objs, float(np.median(gms)), pts
+    return physical(vt), objs, float(np.median(gms)) if gms else 0.0, pts, kept
 
 
 def upsample(vc, shape):
