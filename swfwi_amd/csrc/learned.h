@@ -49,9 +49,9 @@ struct NetGeom {
 // W3 rounded to BF16, as FP32.
 struct NetDev {
   const uint16_t* w1a;  // layer-1 A operands, [2][64 lanes][4]
-  const uint16_t* w2f;  // layer-2 A operands, [9][64][4] (item v of lane l: ci = 4 (l / 16) + v, co = l % 16)
   const uint16_t* w2b;  // backward B operands, [9][64][4] (co = 4 (l / 16) + v, ci = l % 16)
   const uint16_t* w3a;  // layer-3 A operands, [64][4]
+  const uint16_t* w2p;  // layer-2 A operands of the K = 32 products (taps 2q, 2q+1; tap 8, zeros), [5][64][8]
   const float* w3;      // W3 rounded to BF16
 };
 
@@ -81,7 +81,7 @@ class ConvNet {
   ConvNet();
   void set_weights(const float* host);
   void get_weights(float* host) const;
-  NetDev dev() const { return NetDev{w1a_.get(), w2f_.get(), w2b_.get(), w3a_.get(), w3_.get()}; }
+  NetDev dev() const { return NetDev{w1a_.get(), w2b_.get(), w3a_.get(), w2p_.get(), w3_.get()}; }
   // p <- p + net(p) over the region, for a stack of fields (device)
   void correct(float* p, const NetGeom& g, float* yscratch, hipStream_t s) const;
   // One Adam step on the batch (inputs, targets on the device); the loss
@@ -103,7 +103,7 @@ class ConvNet {
  private:
   void repack(hipStream_t s);
   DBuf<float> theta_, m_, v_, grad_, w3_;
-  DBuf<uint16_t> w1a_, w2f_, w2b_, w3a_;
+  DBuf<uint16_t> w1a_, w2b_, w3a_, w2p_;
   DBuf<float> y_, dy_, parts_, din_, dr_, dtgt_, bin_, br_, btgt_;
   NetGeom dg_{};
   int dn_ = 0;

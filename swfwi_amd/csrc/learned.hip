@@ -47,6 +47,7 @@ typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 hbf16x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 hbf16x8 __attribute__((ext_vector_type(8)));
 
 __device__ __host__ __forceinline__ uint16_t bf16_bits(unsigned u) {  // round to nearest even
   u += 0x7fffu + ((u >> 16) & 1u);
@@ -141,20 +142,35 @@ __device__ bool forward_tile(const float* __restrict__ p, const float* __restric
   }
   __syncthreads();
   // layers 2 and 3
-  bf16x4 a2[9];
+  // five K = 32 products (v_mfma_f32_16x16x32_bf16: lane l supplies
+  // A[i = l%16][k = 8*(l/16) + v], B[k = 8*(l/16) + v][j = l%16]): product q
+  // takes tap 2q at k < 16 and tap 2q+1 at k >= 16 (channel k%16); the fifth
+  // takes tap 8 at k < 16 and zeros.  One opcode for the whole accumulation
+  // chain: a 16x16x16 product that read the accumulator of a 16x16x32 one
+  // issued just before it has given wrong sums on gfx950 (no wait states are
+  // inserted between the two).
+  hbf16x8 a2p[5];
 #pragma unroll
-  for (int t = 0; t < 9; t++) a2[t] = *reinterpret_cast<const bf16x4*>(w.w2f + (t * 64 + lane) * 4);
+  for (int q = 0; q < 5; q++) a2p[q] = *reinterpret_cast<const hbf16x8*>(w.w2p + (q * 64 + lane) * 8);
   const bf16x4 a3 = *reinterpret_cast<const bf16x4*>(w.w3a + lane * 4);
+  const int khalf = kq >> 1, kc = 8 * (kq & 1);  // this lane's tap of the pair, first channel
   for (int grp = wave; grp < kGroups2; grp += kT / 64) {
     const int qa = grp * 16 + (lane & 15);     // this lane's point
     const int qs = qa < kNH2 ? qa : kNH2 - 1;  // points past the region read a valid one
     const int x = qs / kH2, z = qs % kH2;      // layer-2 coords; layer-1 = +1
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int t = 0; t < 9; t++) {
+    for (int q = 0; q < 4; q++) {
+      const int t = 2 * q + khalf;
       const int q1 = (x + 1 + t / 3 - 1) * kH1 + (z + 1 + t % 3 - 1);
-      const bf16x4 b = *reinterpret_cast<const bf16x4*>(h1 + q1 * kNetC + 4 * kq);
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2[t], b, acc, 0, 0, 0);
+      const hbf16x8 b = *reinterpret_cast<const hbf16x8*>(h1 + q1 * kNetC + kc);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2p[q], b, acc, 0, 0, 0);
+    }
+    {
+      const int q1 = (x + 2) * kH1 + (z + 2);  // tap 8: (+1, +1)
+      hbf16x8 b = *reinterpret_cast<const hbf16x8*>(h1 + q1 * kNetC + kc);
+      if (khalf) b = hbf16x8{};
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2p[4], b, acc, 0, 0, 0);
     }
     const bf16x4 hb = relu_bf16(acc);  // h2[4 kq + v][qa]
     if (h2 && qa < kNH2) *reinterpret_cast<bf16x4*>(h2 + qa * kNetC + 4 * kq) = hb;
@@ -175,7 +191,7 @@ __device__ bool forward_tile(const float* __restrict__ p, const float* __restric
 __global__ __launch_bounds__(kT) void k_net_apply(const float* __restrict__ p_all, float* __restrict__ y_all,
                                                   NetGeom g, NetDev w) {
   __shared__ unsigned sx[kP * kP];
-  __shared__ uint16_t h1[kNH1 * kNetC];
+  __shared__ __align__(16) uint16_t h1[kNH1 * kNetC];
   __shared__ float u[9 * kNH2];
   const TileMap m = tile_of(g, 0);
   const float* p = p_all + (long long)m.e * g.stride;
@@ -247,7 +263,7 @@ __global__ void k_net_loss(const float* p, const float* y, const float* t, float
 __global__ __launch_bounds__(kT) void k_net_backward(const float* __restrict__ p_all, const float* __restrict__ dy_all,
                                                      NetGeom g, NetDev w, float* __restrict__ parts) {
   __shared__ unsigned sx[kP * kP];
-  __shared__ uint16_t h1[kNH1 * kNetC];
+  __shared__ __align__(16) uint16_t h1[kNH1 * kNetC];
   __shared__ uint16_t h2[kNH2 * kNetC];
   __shared__ float sdy[kNH1];                 // dy over the tile + 2
   __shared__ uint16_t dz2[kNH2 * kNetC];      // BF16 (MFMA operand)
@@ -373,9 +389,16 @@ __global__ void k_net_reduce(const float* parts, long long nblocks, float* grad)
   }
 }
 
-__global__ void k_net_pack(const float* theta, uint16_t* w1a, uint16_t* w2f, uint16_t* w2b, uint16_t* w3a,
-                           float* w3) {
+__global__ void k_net_pack(const float* theta, uint16_t* w1a, uint16_t* w2b, uint16_t* w3a,
+                           uint16_t* w2p, float* w3) {
   const int i = blockIdx.x * kT + threadIdx.x;
+  if (i < 5 * 64 * 8) {
+    // layer 2, product q: A[co = l%16][k = 8*(l/16) + v] = W2[2q + k/16][k%16][co],
+    // zero past tap 8
+    const int q = i / 512, l = (i / 8) % 64, v = i % 8;
+    const int k = 8 * (l >> 4) + v, co = l & 15, t = 2 * q + (k >> 4);
+    w2p[i] = t < 9 ? bf16_of(theta[kNetW1 + (t * kNetC + (k & 15)) * kNetC + co]) : (uint16_t)0;
+  }
   if (i < kNetW3) w3[i] = f32_of(bf16_of(theta[kNetW1 + kNetW2 + i]));
   if (i < 2 * 64 * 4) {
     // layer 1, product h: A[c = l%16][k = 16 h + 4*(l/16) + v] = W1 row k (= 2 t + input) or 0 past k = 17
@@ -394,7 +417,6 @@ __global__ void k_net_pack(const float* theta, uint16_t* w1a, uint16_t* w2f, uin
     const int t = i / 256, l = (i / 4) % 64, v = i % 4;
     const int k = 4 * (l >> 4) + v, n = l & 15;
     const float* W2 = theta + kNetW1;
-    w2f[i] = bf16_of(W2[(t * kNetC + k) * kNetC + n]);  // k = ci, n = co
     w2b[i] = bf16_of(W2[(t * kNetC + n) * kNetC + k]);  // k = co, n = ci
   }
 }
@@ -460,9 +482,9 @@ void check_geom(const NetGeom& g, int margin) {
 }  // namespace
 
 void net_pack(const float* theta, const NetDev& w, hipStream_t s) {
-  hipLaunchKernelGGL(k_net_pack, dim3((9 * 64 * 4 + kT - 1) / kT), dim3(kT), 0, s, theta, const_cast<uint16_t*>(w.w1a),
-                     const_cast<uint16_t*>(w.w2f), const_cast<uint16_t*>(w.w2b), const_cast<uint16_t*>(w.w3a),
-                     const_cast<float*>(w.w3));
+  hipLaunchKernelGGL(k_net_pack, dim3((5 * 64 * 8 + kT - 1) / kT), dim3(kT), 0, s, theta, const_cast<uint16_t*>(w.w1a),
+                     const_cast<uint16_t*>(w.w2b), const_cast<uint16_t*>(w.w3a),
+                     const_cast<uint16_t*>(w.w2p), const_cast<float*>(w.w3));
 }
 
 void net_apply(const float* p, float* y, const NetGeom& g, const NetDev& w, hipStream_t s) {
@@ -510,9 +532,9 @@ ConvNet::ConvNet() {
   grad_.resize(kNetParams);
   w3_.resize(kNetW3);
   w1a_.resize(2 * 64 * 4);
-  w2f_.resize(9 * 64 * 4);
   w2b_.resize(9 * 64 * 4);
   w3a_.resize(64 * 4);
+  w2p_.resize(5 * 64 * 8);
   lparts_.resize(1024);
   lsum_.resize(1);
 }

@@ -67,6 +67,33 @@ def test_net_forward_and_gradient_against_oracle():
     net.close()
 
 
+def test_net_forward_each_layer2_tap_alone():
+    """Layer 2 with all but one of its nine taps zeroed, tap by tap: each
+    product of the kernel's layer-2 accumulation chain must contribute on its
+    own (a product whose sum is lost or read stale shows up as one wrong tap,
+    which the all-taps check can blur into a small overall error)."""
+    rng = np.random.default_rng(3)
+    nx, nz = 70, 53
+    region = (9, 61, 7, 46)
+    p = smooth_fields(rng, 1, nx, nz)
+    rv = rng.uniform(0.05, 0.3, p.shape).astype(np.float32)
+    base = Net.initial(seed=5, scale=0.1)
+    x0, x1, z0, z1 = region
+    net = Net(base)
+    for t in range(9):
+        theta = base.copy()
+        w2 = theta[cn.NW1:cn.NW1 + 9 * 16 * 16].reshape(9, 16, 16)
+        w2[np.arange(9) != t] = 0
+        net.set_weights(theta)
+        got = net.apply(p, rv, region)[0]
+        want, _ = cn.forward(p[0], rv[0], theta, emulate=True)
+        err = rel_l2(got[x0:x1, z0:z1], want[x0:x1, z0:z1])
+        print(f"layer-2 tap {t} alone: rel-L2 {err:.2e}", flush=True)
+        assert np.linalg.norm(want[x0:x1, z0:z1]) > 0
+        assert err < 1e-4, t
+    net.close()
+
+
 def test_net_training_lowers_loss():
     """Targets p + F p for a fixed 3x3 filter F: the network can represent the
     correction (F p = relu(F p) - relu(-F p)), so Adam on the device must
