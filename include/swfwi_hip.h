@@ -128,11 +128,18 @@ int swfwi_forward_shots(swfwi_solver h, const float *wlt, float *dobs);
  * step kernel, [1] backward step kernel. */
 int swfwi_solver_counters(swfwi_solver h, double *point_updates, long long *launches);
 /* Capture time loops into HIP graphs and replay them when the same buffers
- * recur (on by default inside the inversion drivers' epochs). */
+ * recur: 0 off, 1 batches of up to 8 entries (larger batches are not
+ * launch-bound; on by default inside the inversion drivers' epochs), 2 every
+ * batch size. */
 int swfwi_solver_set_graphs(swfwi_solver h, int on);
 /* Stencil tile width along x: 8, 16 or 32, or 0 for the measured rule
  * by batch size (the default). */
 int swfwi_solver_set_tile_x(swfwi_solver h, int tile_x);
+/* The tile tuner (1, the default) or the measured rule alone (0): with
+ * tile_x 0, the first forward run of each batch size times every width on
+ * scratch wavefields and keeps the rule's width unless another is faster by
+ * 3% in both timed rounds.  Every width gives the same results bit for bit. */
+int swfwi_solver_set_tuning(swfwi_solver h, int on);
 /* Tile width in use for a kernel kind (0 forward, 1 backward) and batch
  * size, 0 when that batch size has not run yet. */
 int swfwi_solver_get_tile_x(swfwi_solver h, int kind, int entries, int *tile_x);

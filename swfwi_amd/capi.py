@@ -53,6 +53,7 @@ SIGNATURES = {
     "swfwi_forward_shots": (_I, [_VP, _f32, _f32]),
     "swfwi_solver_counters": (_I, [_VP, _f64, np.ctypeslib.ndpointer(np.int64, flags="C_CONTIGUOUS")]),
     "swfwi_solver_set_graphs": (_I, [_VP, _I]),
+    "swfwi_solver_set_tuning": (_I, [_VP, _I]),
     "swfwi_solver_set_tile_x": (_I, [_VP, _I]),
     "swfwi_solver_get_tile_x": (_I, [_VP, _I, _I, C.POINTER(_I)]),
     "swfwi_solver_set_timing": (_I, [_VP, _I]),
@@ -272,7 +273,11 @@ class Solver:
         return pu.tolist(), la.tolist()
 
     def set_graphs(self, on):
-        check(load().swfwi_solver_set_graphs(self.h, int(bool(on))))
+        """False/0 off, True/1 batches of up to 8 entries, 2 every batch size."""
+        check(load().swfwi_solver_set_graphs(self.h, int(on)))
+
+    def set_tuning(self, on):
+        check(load().swfwi_solver_set_tuning(self.h, int(bool(on))))
 
     def set_tile_x(self, tx):
         check(load().swfwi_solver_set_tile_x(self.h, int(tx)))

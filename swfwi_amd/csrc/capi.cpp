@@ -376,7 +376,15 @@ int swfwi_solver_counters(swfwi_solver h, double* point_updates, long long* laun
 int swfwi_solver_set_graphs(swfwi_solver h, int on) {
   return guarded([&] {
     NEED(h);
-    h->s->prop().set_graphs(on != 0);
+    if (on < 0 || on > 2) fail(kErrArg, "graph mode must be 0, 1 or 2");
+    h->s->prop().set_graphs(on);
+  });
+}
+
+int swfwi_solver_set_tuning(swfwi_solver h, int on) {
+  return guarded([&] {
+    NEED(h);
+    h->s->prop().set_tuning(on != 0);
   });
 }
 

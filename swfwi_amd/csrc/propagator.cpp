@@ -242,10 +242,17 @@ float Propagator::time_steps(int kind, int E, const float* rv, long long rvstrid
   hipStream_t s = stream();
   const long long np = g_.npts();
   w.ensure(E, np, true);
-  DBuf<float> gscratch;
+  DBuf<float> own;
+  float* gscratch = nullptr;
   if (kind == 1) {
-    gscratch.resize((size_t)E * (size_t)np);
-    launch_fill(gscratch.get(), (long long)E * np, 0.f, s);
+    const size_t n = (size_t)E * (size_t)np;
+    if (tune_grad_.size() >= n) {
+      gscratch = tune_grad_.get();
+    } else {
+      own.resize(n);
+      gscratch = own.get();
+    }
+    launch_fill(gscratch, (long long)n, 0.f, s);
   }
   if (clear) {
     launch_fill(w.a.get(), (long long)E * np, 0.f, s);
@@ -279,7 +286,7 @@ float Propagator::time_steps(int kind, int E, const float* rv, long long rvstrid
       a.rv = rv;
       a.rvstride = rvstride;
       a.rvdiv = rvdiv;
-      a.grad = gscratch.get();
+      a.grad = gscratch;
       a.gstride = np;
       a.weight = 1.f;
       a.do_xcorr = 1;
@@ -318,23 +325,26 @@ int Propagator::tile_choice(int kind, int E, const float* rv, long long rvstride
   const long long key = tile_key(kind, E);
   for (const auto& kv : tile_cache_)
     if (kv.first == key) return kv.second;
-  int best = 1;
-  if (!tune_ || !may_time) {
-    best = fixed_width(kind, E);
-  } else {
+  const int rule = fixed_width(kind, E);  // the measured rule is the incumbent
+  int best = rule;
+  if (tune_ && may_time) {
     // timed trial on scratch wavefields (the caller's workspace is free at
     // this point: forward() tunes before it zeroes and runs); three
-    // interleaved rounds (the first warms clocks and caches), the best of
-    // the last two per width
-    float best_of[kNumWidths] = {};
+    // interleaved rounds, the first warming clocks and caches; a width
+    // replaces the rule's only when it is faster by kTuneMargin in both of
+    // the last two, so that noise between near-equal widths keeps the rule
+    float t[2][kNumWidths] = {};
+    if (kind == 1) tune_grad_.resize((size_t)E * (size_t)g_.npts());
     for (int round = 0; round < 3; round++)
       for (int i = 0; i < kNumWidths; i++) {
         const float us = time_steps(kind, E, rv, rvstride, rvdiv, w, i, 8, round == 0 && i == 0);
-        if (round == 1 || (round == 2 && us < best_of[i])) best_of[i] = us;
+        if (round > 0) t[round - 1][i] = us;
       }
-    best = 0;
-    for (int i = 1; i < kNumWidths; i++)
-      if (best_of[i] < best_of[best]) best = i;
+    tune_grad_ = DBuf<float>();
+    for (int i = 0; i < kNumWidths; i++) {
+      const bool faster = t[0][i] < (1.f - kTuneMargin) * t[0][rule] && t[1][i] < (1.f - kTuneMargin) * t[1][rule];
+      if (faster && std::max(t[0][i], t[1][i]) < std::max(t[0][best], t[1][best])) best = i;
+    }
   }
   tile_cache_.push_back({key, best});
   return best;
@@ -539,7 +549,7 @@ void Propagator::run_maybe_graph(const std::vector<uintptr_t>& key, int E, int k
     SWFWI_HIP(hipEventRecord(t0, s));
   }
   // Large batches are not launch-bound: a graph would only add its capture.
-  if (!graphs_ || E > kGraphMaxEntries) {
+  if (graphs_ == 0 || (graphs_ == 1 && E > kGraphMaxEntries)) {
     body();
   } else {
     GraphEntry* hit = nullptr;

@@ -134,11 +134,23 @@ class Propagator {
 
   // Time loops are captured into HIP graphs and replayed when the same
   // buffers come back (the drivers' epochs); off by default for one-shot calls.
-  void set_graphs(bool on) { graphs_ = on; }
+  // Mode 1 captures batches of up to kGraphMaxEntries entries (larger ones
+  // are not launch-bound: measured on the 301x921 grid at 100 members, an
+  // ensemble FWI epoch ran 2585 ms launched directly against 2612 ms
+  // replayed), mode 2 captures every batch size.
+  void set_graphs(int mode) { graphs_ = mode; }
   // Tile width override (8, 16, 32) or 0 for the tuner; tuning off falls back
-  // to a size heuristic.
+  // to the measured rule by batch size.
   void set_tile_x(int tx);
-  void set_tuning(bool on) { tune_ = on; }
+  // The tile tuner (on by default): the first forward() of a batch size
+  // times every width on scratch wavefields and keeps the measured rule's
+  // width unless another is faster by kTuneMargin in both timed rounds.
+  void set_tuning(bool on) {
+    tune_ = on;
+    tile_cache_.clear();
+  }
+  bool tuning() const { return tune_; }
+  static constexpr float kTuneMargin = 0.03f;
   // Backward iterations as one fused block per tile (0), as separate source
   // and receiver blocks (1), or by batch size (-1, the default: split while
   // the fused tiles would not fill the chip).
@@ -154,7 +166,8 @@ class Propagator {
   // wavefields of model rv (entry e uses rv + (e / rvdiv) * rvstride).
   float time_steps(int kind, int E, const float* rv, long long rvstride, int rvdiv, Fields& w, int width, int reps,
                    bool clear);
-  bool graphs() const { return graphs_; }
+  bool graphs() const { return graphs_ != 0; }
+  int graphs_mode() const { return graphs_; }
   // Timing: one HIP event pair around every forward (0) / backward (1) time
   // loop on the handle's stream, graph replays included; collect_timing()
   // waits for the loops in flight and adds them to kernel_ms / kernel_n
@@ -217,7 +230,8 @@ class Propagator {
     long long launches;
   };
   std::vector<GraphEntry> graph_cache_;
-  bool graphs_ = false, timing_ = false;
+  int graphs_ = 0;
+  bool timing_ = false;
   std::vector<TimedLoop> pending_;
   std::vector<hipEvent_t> event_pool_;
 
@@ -230,7 +244,8 @@ class Propagator {
   DampTables damp_;
   DevPoints src_[kNumWidths], rec_[kNumWidths];  // points per tile, per tile width
   int tile_override_ = 0;                 // 0 = tune, else a width from kTileWidths
-  bool tune_ = false;
+  bool tune_ = true;
+  DBuf<float> tune_grad_;  // scratch gradient of the tuner's backward trials
   int bwd_split_ = -1;
   std::vector<std::pair<long long, int>> tile_cache_;  // (kind*2^32 + E) -> width index
   DBuf<int> sxp_, szp_, gxp_, gzp_;

@@ -222,7 +222,7 @@ Inversion::Inversion(Solver& s, const float* wlt_host, const float* dobs_host, c
   tmax_ = (g.dx / g.dt / p.vmin) * (g.dx / g.dt / p.vmin);
   if (tmax_ <= tmin_) fail(kErrArg, "vmax must exceed vmin");
   // epochs re-run the same time loops on the same buffers: replay graphs
-  s.prop().set_graphs(true);
+  s.prop().set_graphs(std::max(s.prop().graphs_mode(), 1));
   s.set_wavelet(wlt_host);
   const size_t m = (size_t)g.ns() * g.nt * g.ng();
   dobs_.upload(dobs_host, m, s.stream());

@@ -136,6 +136,48 @@ def test_forward_shots_bitexact(oracle, small):
         assert np.array_equal(bits(dobs[shot]), bits(g.forward_modeling(vt, wlt, shot=shot)))
 
 
+def test_tile_tuner_choice_and_results(oracle):
+    """The tuner (on by default) picks a width per kernel kind and batch size
+    on the first forward run; with it off the measured rule decides (64x8
+    tiles for this small grid, fewer than 600 64x16 tiles); the seismograms
+    are the oracle's either way, bit for bit."""
+    case = workloads.small_case(nx=60, nz=40, nb=8, nt=400, ns=3)
+    s, g, vt, vi, wlt = setup_case(oracle, case)
+    s.bind_velocity(vt)
+    want = np.stack([g.forward_modeling(vt, wlt, shot=k) for k in range(case.ns)])
+    for tuning in (True, False):
+        print(f"tuning={tuning}", flush=True)
+        s.set_tuning(tuning)
+        d = s.forward_shots(wlt)
+        assert np.array_equal(bits(d), bits(want))
+        widths = (s.tile_x(0, case.ns), s.tile_x(1, case.ns))
+        print(f"  widths forward/backward at {case.ns} entries: {widths}", flush=True)
+        assert all(w in (8, 16, 32) for w in widths)
+        if not tuning:
+            assert widths == (8, 8)
+    s.close()
+
+
+def test_graph_capture_at_every_batch_size_is_bitexact(oracle):
+    """Graph mode 2 captures the 12-shot batch (past the 8-entry limit of
+    the default mode) and replays it on the second call: both calls give the
+    directly launched seismograms and the oracle's, bit for bit."""
+    case = workloads.small_case(nx=60, nz=40, nb=8, nt=400, ns=12)
+    s, g, vt, vi, wlt = setup_case(oracle, case)
+    s.bind_velocity(vt)
+    s.set_graphs(0)
+    direct = s.forward_shots(wlt)
+    s.set_graphs(2)
+    for call in range(2):
+        print(f"graph mode 2, call {call}", flush=True)
+        assert np.array_equal(bits(s.forward_shots(wlt)), bits(direct))
+    for shot in (0, case.ns - 1):
+        assert np.array_equal(bits(direct[shot]), bits(g.forward_modeling(vt, wlt, shot=shot)))
+    with pytest.raises(capi.SwfwiError):
+        s.set_graphs(3)
+    s.close()
+
+
 @pytest.mark.parametrize("shot", [1, -1])
 def test_calgradient_bitexact(oracle, small, shot):
     case, s, g, vt, vi, wlt = small
