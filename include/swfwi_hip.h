@@ -239,6 +239,15 @@ int swfwi_batch_destroy(swfwi_batch b);
 int swfwi_batch_set_snapshot_precision(swfwi_batch b, int snap_prec);
 /* Capture each time loop into a HIP graph on first use (on by default). */
 int swfwi_batch_set_graphs(swfwi_batch b, int on);
+/* The tile tuner (1, the default) or the size rule alone (0): the first
+ * forward run times the step kernels at tile widths 16 and 32 over every
+ * entry and keeps the rule's width unless the other is faster by 3% in both
+ * timed rounds.  tile_x receives the width in use. */
+int swfwi_batch_set_tuning(swfwi_batch b, int on);
+/* A fixed tile width, 16 or 32 (the tuner is skipped), or 0 for the rule
+ * and the tuner. */
+int swfwi_batch_set_tile_x(swfwi_batch b, int tile_x);
+int swfwi_batch_tile_x(swfwi_batch b, int *tile_x);
 /* model `model` of group `group`: padded transformed velocity (Damp4t10d) */
 int swfwi_batch_bind_model(swfwi_batch b, int group, int model, const float *vel_t);
 /* Forward run of every entry: seismograms into dcal[entry][nt][ng] (may be

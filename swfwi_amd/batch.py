@@ -81,6 +81,18 @@ class Batch:
     def set_graphs(self, on):
         check(load().swfwi_batch_set_graphs(self.h, int(bool(on))))
 
+    def set_tuning(self, on):
+        check(load().swfwi_batch_set_tuning(self.h, int(bool(on))))
+
+    def set_tile_x(self, tx):
+        """16 or 32 (fixed, no tuning), or 0 for the size rule and the tuner."""
+        check(load().swfwi_batch_set_tile_x(self.h, int(tx)))
+
+    def tile_x(self):
+        t = C.c_int(0)
+        check(load().swfwi_batch_tile_x(self.h, C.byref(t)))
+        return t.value
+
     def bind_model(self, group, model, vel_t):
         g = self.groups[group]
         v = _f(vel_t)

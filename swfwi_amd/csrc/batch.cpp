@@ -43,6 +43,118 @@ Batch::~Batch() {
     if (g) (void)hipGraphExecDestroy(g);
 }
 
+void Batch::drop_graphs() {
+  for (hipGraphExec_t& g : graph_)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+
+void Batch::set_tuning(bool on) {
+  tune_ = on;
+  tuned_ = false;
+  const int tx = fixed_x_ ? fixed_x_ : rule_x_;
+  if (tile_x_ != tx) layout_tiles(tx);
+}
+
+void Batch::set_tile_x(int tx) {
+  if (tx != 0 && tx != 16 && tx != 32) fail(kErrArg, "batch tile width must be 0 (rule and tuner), 16 or 32");
+  fixed_x_ = tx;
+  tuned_ = false;
+  const int w = tx ? tx : rule_x_;
+  if (tile_x_ != w) layout_tiles(w);
+}
+
+void Batch::layout_tiles(int tx) {
+  hipStream_t s = stream_.get();
+  SWFWI_HIP(hipStreamSynchronize(s));  // no launch in flight reads the old layout
+  tile_x_ = tx;
+  std::vector<int> blk;
+  tile0_.clear();
+  int tiles = 0;
+  for (size_t r = 0; r < ent_.size(); r++) {
+    BatchEntry& e = ent_[r];
+    e.ntx = (e.nxpad - 2 * kHalo + tx - 1) / tx;
+    tile0_.push_back(tiles);
+    blk.insert(blk.end(), (size_t)e.ntz * e.ntx, (int)r);
+    tiles += e.ntz * e.ntx;
+  }
+  tile0_.push_back(tiles);
+  dent_.upload(ent_.data(), ent_.size(), s);
+  dblk_.upload(blk, s);
+  dtile0_.upload(tile0_, s);
+  SWFWI_HIP(hipStreamSynchronize(s));
+  drop_graphs();
+}
+
+void Batch::tune() {
+  // Timed trial on the batch's own buffers before its first forward run,
+  // which clears or overwrites everything the trial writes (fields, scales
+  // and taps are cleared; seismogram rows and checkpoint slots are written
+  // step by step; the backward run clears the gradient).  kReps steps from
+  // step 0 (forward) and iteration 0 (backward), where every entry is
+  // active; three interleaved rounds, the first warming clocks and caches.
+  tuned_ = true;
+  constexpr int kReps = 8;
+  const int n = (int)ent_.size();
+  int min_fwd = ent_[0].nt, min_bwd = ent_[0].nt;
+  for (const BatchEntry& e : ent_) {
+    min_fwd = std::min(min_fwd, e.nt);
+    min_bwd = std::min(min_bwd, e.nt - e.it_last);
+  }
+  if (min_fwd < kReps) return;
+  const bool bwd = prec_ != kSnapNone && min_bwd >= kReps;
+  hipStream_t s = stream_.get();
+  launch_fill(fa_.get(), fields_, 0.f, s);
+  launch_fill(fb_.get(), fields_, 0.f, s);
+  launch_fill_u32(scale_.get(), std::max(scales_, 1), 0u, s);
+  if (bwd) {
+    launch_fill(vsrc_.get(), dcal_len_, 0.f, s);
+    launch_fill(grad_.get(), grad_len_, 0.f, s);
+    if (snap_bytes_ >= 4) launch_fill_u32(reinterpret_cast<unsigned*>(snap_.get()), snap_bytes_ / 4, 0u, s);
+  }
+  const int widths[2] = {16, 32};
+  float t[2][2] = {};
+  hipEvent_t e0, e1;
+  SWFWI_HIP(hipEventCreate(&e0));
+  SWFWI_HIP(hipEventCreate(&e1));
+  for (int round = 0; round < 3; round++)
+    for (int wi = 0; wi < 2; wi++) {
+      layout_tiles(widths[wi]);
+      BatchKernelArgs a = args(true);
+      a.tap = nullptr;
+      SWFWI_HIP(hipEventRecord(e0, s));
+      for (int r = 0; r < kReps; r++) {
+        a.it_or_k = r;
+        a.rot = r;
+        a.nblocks = tile0_[n];
+        launch_batch_fwd(a, prec_, order_, tile_x_, s);
+      }
+      if (bwd) {
+        BatchKernelArgs b = args(false);
+        for (int r = 0; r < kReps; r++) {
+          b.it_or_k = r;
+          b.rot = r;
+          b.nblocks = tile0_[n];
+          launch_batch_bwd(b, prec_, order_, tile_x_, s);
+        }
+      }
+      SWFWI_HIP(hipEventRecord(e1, s));
+      SWFWI_HIP(hipEventSynchronize(e1));
+      float ms = 0;
+      SWFWI_HIP(hipEventElapsedTime(&ms, e0, e1));
+      if (round > 0) t[round - 1][wi] = ms;
+    }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  SWFWI_HIP(hipGetLastError());
+  const int rule = rule_x_ == widths[1] ? 1 : 0, other = 1 - rule;
+  const bool faster = t[0][other] < (1.f - kBatchTuneMargin) * t[0][rule] &&
+                      t[1][other] < (1.f - kBatchTuneMargin) * t[1][rule];
+  layout_tiles(widths[faster ? other : rule]);
+}
+
 void Batch::set_snap_prec(int p) {
   if (p != kSnapF32 && p != kSnapBF16 && p != kSnapF16 && p != kSnapNone) fail(kErrArg, "unknown snapshot precision");
   if (p == prec_) return;
@@ -51,11 +163,8 @@ void Batch::set_snap_prec(int p) {
   if (prec_ == kSnapNone || (p == kSnapF32 && prec_ != kSnapF32))
     fail(kErrArg, "the batch's checkpoint storage was created for a narrower format");
   prec_ = p;
-  for (hipGraphExec_t& g : graph_)
-    if (g) {
-      (void)hipGraphExecDestroy(g);
-      g = nullptr;
-    }
+  drop_graphs();
+  tuned_ = false;  // the kernels change with the checkpoint format
 }
 
 void Batch::build() {
@@ -74,7 +183,7 @@ void Batch::build() {
     const long long ux = G.nx + 2 * G.nb, uz = G.nz + G.nb;  // update region
     t16 += (long long)G.nmodels * (long long)G.sx.size() * ((uz + kBatchTileZ - 1) / kBatchTileZ) * ((ux + 15) / 16);
   }
-  tile_x_ = t16 >= kBatchWideTiles ? 32 : 16;
+  tile_x_ = rule_x_ = t16 >= kBatchWideTiles ? 32 : 16;
   for (size_t gi = 0; gi < g_.size(); gi++) {
     const BatchGroup& G = g_[gi];
     const int ns = (int)G.sx.size();
@@ -399,6 +508,7 @@ static void timed(hipStream_t s, double& ms_out, const std::function<void()>& bo
 
 void Batch::forward(float* dcal_host) {
   hipStream_t s = stream_.get();
+  if (tune_ && !tuned_ && !fixed_x_) tune();
   auto body = [&] {
     if (!graphs_) {
       run_loop(true);

@@ -25,6 +25,7 @@ constexpr int kSnapEvery = 4;
 // the 793x277 padded coarse grid, 3 920, 8% slower).
 constexpr int kBatchTileZ = 64;
 constexpr int kBatchWideTiles = 8000;
+constexpr float kBatchTuneMargin = 0.03f;
 enum SnapPrec : int { kSnapF32 = 0, kSnapBF16 = 1, kSnapF16 = 2, kSnapNone = 3 };
 // Stencils: the reference's damped 10th-order-in-space, 4th-order-in-time
 // scheme (fd4t10s-damp-zjh.c), or a damped 4th-order-in-space, 2nd-order-in-
@@ -113,6 +114,15 @@ class Batch {
   int snap_prec() const { return prec_; }
   void set_snap_prec(int p);
   void set_graphs(bool on) { graphs_ = on; }
+  // Tile tuner (on by default): the first forward run first times the step
+  // kernels (forward and, with checkpoints, backward) at both tile widths
+  // over every entry, and keeps the width of the size rule unless the other
+  // is faster by kBatchTuneMargin in both timed rounds.  Off: the rule.
+  void set_tuning(bool on);
+  // A fixed tile width (16 or 32; the tuner is skipped), or 0 for the rule
+  // and the tuner.
+  void set_tile_x(int tx);
+  int tile_x() const { return tile_x_; }
   // model m of group g: padded transformed velocity (Damp4t10d layout)
   void bind_model(int g, int m, const float* vel_t);
   // Forward run of every entry: records the seismograms and stores the
@@ -170,6 +180,9 @@ class Batch {
 
  private:
   void build();
+  void layout_tiles(int tx);  // entry tile counts and the block list for width tx
+  void tune();
+  void drop_graphs();
   void run_loop(bool fwd);
   BatchKernelArgs args(bool fwd);
   std::vector<BatchGroup> g_;
@@ -187,6 +200,9 @@ class Batch {
   std::vector<Layout> lay_;
   std::vector<int> tile0_;           // per run index, prefix of tiles
   int tile_x_ = 16;                  // step-kernel tile width
+  int rule_x_ = 16;                  // the size rule's width
+  bool tune_ = true, tuned_ = false;
+  int fixed_x_ = 0;                  // a width set by the caller, 0 for none
   std::vector<long long> model_off_; // per model (groups in order)
   std::vector<int> model_group_;
   long long fields_ = 0, models_len_ = 0, dcal_len_ = 0, grad_len_ = 0, snap_elems_ = 0;

@@ -730,6 +730,28 @@ int swfwi_batch_set_graphs(swfwi_batch b, int on) {
   });
 }
 
+int swfwi_batch_set_tuning(swfwi_batch b, int on) {
+  return guarded([&] {
+    NEED(b);
+    b->b->set_tuning(on != 0);
+  });
+}
+
+int swfwi_batch_set_tile_x(swfwi_batch b, int tile_x) {
+  return guarded([&] {
+    NEED(b);
+    b->b->set_tile_x(tile_x);
+  });
+}
+
+int swfwi_batch_tile_x(swfwi_batch b, int* tile_x) {
+  return guarded([&] {
+    NEED(b);
+    NEED(tile_x);
+    *tile_x = b->b->tile_x();
+  });
+}
+
 int swfwi_batch_bind_model(swfwi_batch b, int group, int model, const float* vel_t) {
   return guarded([&] {
     NEED(b);

@@ -156,10 +156,12 @@ def test_wide_tile_batch_against_oracles(oracle):
     ux, uz = g.nx + 2 * g.nb, g.nz + g.nb
     assert g.nmodels * len(g.sx) * ((uz + 63) // 64) * ((ux + 15) // 16) >= 8000
     b = Batch([g], snap_prec=capi.SNAP_F32)
+    b.set_tile_x(32)  # the rule's width here; fixed so the tuner cannot pick 16
     mods = [models_for(g, 7 + m) for m in range(2)]
     for m in range(2):
         b.bind_model(0, m, mods[m][0])
     dobs = b.forward()
+    assert b.tile_x() == 32
     for m in range(2):
         b.bind_model(0, m, mods[m][1])
     D = b.split_data(b.forward())
@@ -179,3 +181,38 @@ def test_wide_tile_batch_against_oracles(oracle):
         print(f"  entry {k}: gradient rel-L2 vs FP64 {eg:.2e}", flush=True)
         assert eg <= GRAD_TOL
     b.close()
+
+
+def test_batch_tile_widths_and_tuner():
+    """The step kernels at tile widths 16 and 32 give the mixed-size batch's
+    seismograms and gradients (BF16 checkpoints) bit for bit alike: each
+    output point is formed by the same operations in the same order whatever
+    the tile.  With the tuner (the default) the first forward run picks one
+    of them; with it off the size rule decides (16 for this small batch)."""
+    gs = groups()
+    mods = {(gi, m): models_for(g, 2 * gi + m) for gi, g in enumerate(gs) for m in range(g.nmodels)}
+    out = {}
+    for mode in ("16", "32", "tuner", "rule"):
+        b = Batch(gs, snap_prec=capi.SNAP_BF16)
+        if mode in ("16", "32"):
+            b.set_tile_x(int(mode))
+        if mode == "rule":
+            b.set_tuning(False)
+        for (gi, m), (vt, vi) in mods.items():
+            b.bind_model(gi, m, vi)
+        d = b.forward()
+        tx = b.tile_x()
+        print(f"{mode}: tile width {tx}", flush=True)
+        assert tx in (16, 32)
+        if mode in ("16", "32"):
+            assert tx == int(mode)
+        if mode == "rule":
+            assert tx == 16
+        vsrc = (np.random.default_rng(5).standard_normal(d.size) * 1e-3).astype(np.float32)
+        ge, gm = b.backward(vsrc)
+        assert np.count_nonzero(ge) > 0
+        out[mode] = (d, ge, gm)
+        b.close()
+    for mode in ("32", "tuner", "rule"):
+        for k in range(3):
+            assert np.array_equal(out[mode][k].view(np.uint32), out["16"][k].view(np.uint32)), (mode, k)
