@@ -486,6 +486,7 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(name, args.workload), "kernel": name,
             "algorithmic_bytes_per_launch": round(byts / max(n, 1)),
             "avg_launch_us": round(1e3 * ms / max(n, 1), 3), "launches": n,
+            "tile_x": s.tile_x(0 if name == "k_fwd_step" else 1, 1) if ess else None,
             "note": ("one-entry launches (the two line-search trials as two loops side by side on two streams, "
                      "each launch timed on its own stream)" if ess else "batches of 11-22 small-grid entries") +
                     ": the live set fits the 256 MB last-level cache, so achieved is an effective (algorithmic) "
@@ -499,6 +500,7 @@ def main():
                     "traffic": measured_traffic("k_fwd_step_batched", args.workload), "kernel": "k_fwd_step",
                     "entries": case.ns, "algorithmic_bytes_per_launch": round(obs_bytes / max(obs_n, 1)),
                     "avg_launch_us": round(1e3 * obs_ms / max(obs_n, 1), 3), "launches": obs_n,
+                    "tile_x": s.tile_x(0, case.ns),
                     "note": "the observed-data generation before the timed epochs (fm-damp): every shot in one "
                             "batched forward loop, untimed for value; algorithmic bytes charge the shared model "
                             "per shot"}
