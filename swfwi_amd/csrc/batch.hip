@@ -141,7 +141,7 @@ __global__ __launch_bounds__(kBT) void k_batch_step(BatchKernelArgs a) {
   float* nxt = (even ? a.fa : a.fb) + E.f;  // holds prev; next is written over it
 
   const bool inside = ox - kBH + kSX <= E.nxpad && oz - kBH + kSZ <= E.nzpad;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   // this thread's kPX outputs (consecutive x at one z): the model and prev
   // are loaded before the staging so that their latency overlaps it
   const int zl = threadIdx.x & (kBZ - 1), xg = threadIdx.x / kBZ;
@@ -171,37 +171,53 @@ __global__ __launch_bounds__(kBT) void k_batch_step(BatchKernelArgs a) {
     if (lane < kSZ - 64) sc[xx * kSZ + 64 + lane] = v1;
   }
   __syncthreads();
-  // u2 on the tile + 1 ring: each wavefront takes a run of rows and slides
-  // its lane's column (tile z = lane) over x in registers, so the x-direction
-  // terms are read from LDS once per row instead of once per tap; the ring's
-  // z = -1 and z = 64 points follow, one per thread of wave 0.  The terms
-  // are summed in the same order as lap_at below.
+  // u2 on the tile + 1 ring: each wavefront takes a run of rows (kRows or
+  // kRows - 1) and holds its lane's staged column (tile z = lane) over those
+  // rows and their x reach in registers, so the x-direction terms are read
+  // from LDS once per row instead of once per tap; the loop is unrolled over
+  // the rows, so the window never shifts.  The ring's z = -1 and z = 64
+  // points follow, one per thread of waves 0-1.  The terms are summed in the
+  // same order as lap_at below.
   {
     constexpr int R = ORDER == kOrder10 ? 5 : 2;
-    const int r0 = wave * kUX / (kBT / 64), r1 = (wave + 1) * kUX / (kBT / 64);
-    const float* col = sc + lane + kBH;  // staged column of tile z = lane
-    float w[2 * R + 1];                  // staged rows xs - R .. xs + R of the column
+    constexpr int kW = kBT / 64;                // wavefronts
+    constexpr int kRows = (kUX + kW - 1) / kW;  // most u2 rows of a wavefront
+    const int r0 = wave * kUX / kW, nrows = (wave + 1) * kUX / kW - r0;  // wave-uniform
+    // staged rows r0 - 1 - R + k of the column of tile z = lane, k < kRows +
+    // 2R (a wavefront with kRows - 1 rows reads one row it does not use: it
+    // is not the last wavefront, which takes kRows, so that row lies inside
+    // the staged tile); every read is an immediate offset from one register
+    int base = lane + kBH + (r0 + kBH - 1 - R) * kSZ;
+    asm("" : "+v"(base));
+    const float* col = sc + base;
+    float w[kRows + 2 * R];
 #pragma unroll
-    for (int k = 0; k <= 2 * R; k++) w[k] = col[(r0 + kBH - 1 - R + k) * kSZ];
-    for (int xu = r0; xu < r1; xu++) {
-      const float* row = col + (xu + kBH - 1) * kSZ;
-      float acc;
-      if constexpr (ORDER == kOrder10) {
-        acc = -4.f * kA0 * w[5];
-        acc = fmaf(kA1, (row[-1] + row[1]) + (w[4] + w[6]), acc);
-        acc = fmaf(kA2, (row[-2] + row[2]) + (w[3] + w[7]), acc);
-        acc = fmaf(kA3, (row[-3] + row[3]) + (w[2] + w[8]), acc);
-        acc = fmaf(kA4, (row[-4] + row[4]) + (w[1] + w[9]), acc);
-        acc = fmaf(kA5, (row[-5] + row[5]) + (w[0] + w[10]), acc);
-      } else {
-        acc = -5.f * w[2];
-        acc = fmaf(4.f / 3.f, (row[-1] + row[1]) + (w[1] + w[3]), acc);
-        acc = fmaf(-1.f / 12.f, (row[-2] + row[2]) + (w[0] + w[4]), acc);
+    for (int k = 0; k < kRows + 2 * R; k++) w[k] = col[k * kSZ];
+#pragma unroll
+    for (int i = 0; i < kRows; i++) {
+      if (i < kRows - 1 || nrows == kRows) {
+        const int xu = r0 + i;
+        // staged row of u2 row xu, its z reads as immediate offsets 0..2R
+        // from one register (opaque, so the reads are not re-based in pairs)
+        int ro = (i + R) * kSZ - R;
+        asm("" : "+v"(ro));
+        const float* row = col + ro + R;
+        const float* x = w + i;  // x[R]: the row's own staged value
+        float acc;
+        if constexpr (ORDER == kOrder10) {
+          acc = -4.f * kA0 * x[5];
+          acc = fmaf(kA1, (row[-1] + row[1]) + (x[4] + x[6]), acc);
+          acc = fmaf(kA2, (row[-2] + row[2]) + (x[3] + x[7]), acc);
+          acc = fmaf(kA3, (row[-3] + row[3]) + (x[2] + x[8]), acc);
+          acc = fmaf(kA4, (row[-4] + row[4]) + (x[1] + x[9]), acc);
+          acc = fmaf(kA5, (row[-5] + row[5]) + (x[0] + x[10]), acc);
+        } else {
+          acc = -5.f * x[2];
+          acc = fmaf(4.f / 3.f, (row[-1] + row[1]) + (x[1] + x[3]), acc);
+          acc = fmaf(-1.f / 12.f, (row[-2] + row[2]) + (x[0] + x[4]), acc);
+        }
+        su[xu * kUZ + lane + 1] = acc;
       }
-      su[xu * kUZ + lane + 1] = acc;
-#pragma unroll
-      for (int k = 0; k < 2 * R; k++) w[k] = w[k + 1];
-      if (xu + 1 < r1) w[2 * R] = col[(xu + kBH + R) * kSZ];
     }
     if (threadIdx.x < 2 * kUX) {
       const int xu = threadIdx.x >> 1, zu = (threadIdx.x & 1) ? kUZ - 1 : 0;
@@ -239,9 +255,14 @@ __global__ __launch_bounds__(kBT) void k_batch_step(BatchKernelArgs a) {
     const float u = su[uc];
     const float r = rvj[j];
     const float c = sc[(xl + kBH) * kSZ + zl + kBH];
-    const float d = sponge ? E.dscale * damping(gx, gz, E.nxpad, E.nzpad, E.damp) : 0.f;
     const float pv = pvj[j];
-    float v = fmaf(2.f - 2.f * d + d * d, c, -(1.f - 2.f * d) * pv);
+    float v;
+    if (sponge) {
+      const float d = E.dscale * damping(gx, gz, E.nxpad, E.nzpad, E.damp);
+      v = fmaf(2.f - 2.f * d + d * d, c, -(1.f - 2.f * d) * pv);
+    } else {
+      v = fmaf(2.f, c, -pv);  // d = 0: the coefficients are exactly 2 and 1
+    }
     v = fmaf(r, u, v);
     if constexpr (ORDER == kOrder10) {  // the 4th-order-in-time term r^2/12 Lap5(u2)
       const float lap = (su[uc - 1] + su[uc + 1]) + (su[uc - kUZ] + su[uc + kUZ]) - 4.f * u;
