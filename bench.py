@@ -182,13 +182,13 @@ def mixed_workload(args):
         timing[name] = {"wall_ms": 1e3 * wall, "device_ms": dev_ms, "forward_ms": float(np.mean(fms)),
                         "backward_ms": float(np.mean(bms)), "points": pts, "snapshot_bytes": inf["snapshot_bytes"],
                         "entries": inf["entries"], "gpts": pts / wall / 1e9, "bytes": byts,
-                        "tile_x": b.tile_x(),
+                        "tile_x": b.tile_x(), "block_order": b.block_order(),
                         # one launch per step of the longest entry, forward, and per backward
                         # iteration (the taper ends the backward ~0.3 s early)
                         "launches": max(g.nt for g in groups) + max(g.nt - int(round(0.3 / g.dt)) for g in groups)}
         b.close()
     by_format = {k: {"gpts_per_s": round(v["gpts"], 2), "gradient_ms_per_shot": round(v["device_ms"] / v["entries"], 4),
-                     "tile_x": v["tile_x"]} for k, v in timing.items()}
+                     "tile_x": v["tile_x"], "block_order": v["block_order"]} for k, v in timing.items()}
     timing = timing["bf16"]
     gpts = timing["gpts"]
     line = {
@@ -212,10 +212,11 @@ def mixed_workload(args):
                      "algorithmic_bytes_per_launch": round(timing["bytes"] / timing["launches"]),
                      "avg_launch_us": round(1e3 * timing["device_ms"] / timing["launches"], 2),
                      "launches": timing["launches"],
-                     "tile_x": timing["tile_x"],
+                     "tile_x": timing["tile_x"], "block_order": timing["block_order"],
                      "note": "BF16 checkpoints; forward and backward loops timed with HIP events on the batch's "
                              "stream; the active entries shrink as the shorter grids finish (112, 48, then 16); "
-                             "tile_x: the step kernels' tile width picked by the batch's tile tuner"},
+                             "tile_x, block_order: the step kernels' tile width and block order (1: a model's "
+                             "shots tile by tile) picked by the batch's tuner"},
         "gradient_rel_l2_vs_fp64": res,
         "gradient_tolerance": 1e-3,
     }

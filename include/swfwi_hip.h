@@ -247,6 +247,12 @@ int swfwi_batch_set_tuning(swfwi_batch b, int on);
 /* A fixed tile width, 16 or 32 (the tuner is skipped), or 0 for the rule
  * and the tuner. */
 int swfwi_batch_set_tile_x(swfwi_batch b, int tile_x);
+/* Block order of the step launches: 0 entry by entry, 1 the shots of each
+ * model tile by tile (a model tile is read once from HBM and then from L2),
+ * or -1 for the rule (1) and the tuner, which times both orders with the
+ * widths.  Results are the same bit for bit in every order. */
+int swfwi_batch_set_block_order(swfwi_batch b, int order);
+int swfwi_batch_block_order(swfwi_batch b, int *order);
 int swfwi_batch_tile_x(swfwi_batch b, int *tile_x);
 /* model `model` of group `group`: padded transformed velocity (Damp4t10d) */
 int swfwi_batch_bind_model(swfwi_batch b, int group, int model, const float *vel_t);

@@ -88,6 +88,15 @@ class Batch:
         """16 or 32 (fixed, no tuning), or 0 for the size rule and the tuner."""
         check(load().swfwi_batch_set_tile_x(self.h, int(tx)))
 
+    def set_block_order(self, order):
+        """0 entry by entry, 1 a model's shots tile by tile, -1 the rule and the tuner."""
+        check(load().swfwi_batch_set_block_order(self.h, int(order)))
+
+    def block_order(self):
+        t = C.c_int(0)
+        check(load().swfwi_batch_block_order(self.h, C.byref(t)))
+        return t.value
+
     def tile_x(self):
         t = C.c_int(0)
         check(load().swfwi_batch_tile_x(self.h, C.byref(t)))

@@ -93,6 +93,8 @@ SIGNATURES = {
     "swfwi_batch_set_tuning": (_I, [_VP, _I]),
     "swfwi_batch_tile_x": (_I, [_VP, C.POINTER(_I)]),
     "swfwi_batch_set_tile_x": (_I, [_VP, _I]),
+    "swfwi_batch_set_block_order": (_I, [_VP, _I]),
+    "swfwi_batch_block_order": (_I, [_VP, C.POINTER(_I)]),
     "swfwi_batch_bind_model": (_I, [_VP, _I, _I, _f32]),
     "swfwi_batch_forward": (_I, [_VP, C.c_void_p]),
     "swfwi_batch_adjoint_source": (_I, [_VP, _f32, C.c_void_p]),

@@ -54,36 +54,50 @@ void Batch::drop_graphs() {
 void Batch::set_tuning(bool on) {
   tune_ = on;
   tuned_ = false;
-  const int tx = fixed_x_ ? fixed_x_ : rule_x_;
-  if (tile_x_ != tx) layout_tiles(tx);
+  const int tx = fixed_x_ ? fixed_x_ : rule_x_, order = fixed_order_ >= 0 ? fixed_order_ : kRuleOrder;
+  if (tile_x_ != tx || blk_order_ != order) layout_tiles(tx, order);
 }
 
 void Batch::set_tile_x(int tx) {
   if (tx != 0 && tx != 16 && tx != 32) fail(kErrArg, "batch tile width must be 0 (rule and tuner), 16 or 32");
   fixed_x_ = tx;
-  tuned_ = false;
-  const int w = tx ? tx : rule_x_;
-  if (tile_x_ != w) layout_tiles(w);
+  set_tuning(tune_);
 }
 
-void Batch::layout_tiles(int tx) {
+void Batch::set_block_order(int order) {
+  if (order < -1 || order > 1) fail(kErrArg, "batch block order must be -1 (rule and tuner), 0 or 1");
+  fixed_order_ = order;
+  set_tuning(tune_);
+}
+
+void Batch::layout_tiles(int tx, int order) {
   hipStream_t s = stream_.get();
   SWFWI_HIP(hipStreamSynchronize(s));  // no launch in flight reads the old layout
   tile_x_ = tx;
+  blk_order_ = order;
+  // entries run longest first and a model's shots are consecutive entries of
+  // one grid and one step count, so every launch's active entries (a prefix)
+  // hold whole runs of shots of a model: each run's blocks are contiguous
   std::vector<int> blk;
-  tile0_.clear();
-  int tiles = 0;
-  for (size_t r = 0; r < ent_.size(); r++) {
-    BatchEntry& e = ent_[r];
-    e.ntx = (e.nxpad - 2 * kHalo + tx - 1) / tx;
-    tile0_.push_back(tiles);
-    blk.insert(blk.end(), (size_t)e.ntz * e.ntx, (int)r);
-    tiles += e.ntz * e.ntx;
+  tile0_.assign(ent_.size() + 1, 0);
+  for (BatchEntry& e : ent_) e.ntx = (e.nxpad - 2 * kHalo + tx - 1) / tx;
+  for (size_t r = 0; r < ent_.size();) {
+    size_t r1 = r + 1;
+    while (r1 < ent_.size() && ent_[r1].model == ent_[r].model) r1++;
+    const int nt_e = ent_[r].ntz * ent_[r].ntx;
+    if (nt_e >= (1 << kBlkTileBits)) fail(kErrRange, "too many tiles per batch entry");
+    if (order == 1) {
+      for (int t = 0; t < nt_e; t++)
+        for (size_t q = r; q < r1; q++) blk.push_back((int)(q << kBlkTileBits) | t);
+    } else {
+      for (size_t q = r; q < r1; q++)
+        for (int t = 0; t < nt_e; t++) blk.push_back((int)(q << kBlkTileBits) | t);
+    }
+    for (size_t q = r; q < r1; q++) tile0_[q + 1] = tile0_[q] + nt_e;
+    r = r1;
   }
-  tile0_.push_back(tiles);
   dent_.upload(ent_.data(), ent_.size(), s);
   dblk_.upload(blk, s);
-  dtile0_.upload(tile0_, s);
   SWFWI_HIP(hipStreamSynchronize(s));
   drop_graphs();
 }
@@ -114,14 +128,22 @@ void Batch::tune() {
     launch_fill(grad_.get(), grad_len_, 0.f, s);
     if (snap_bytes_ >= 4) launch_fill_u32(reinterpret_cast<unsigned*>(snap_.get()), snap_bytes_ / 4, 0u, s);
   }
-  const int widths[2] = {16, 32};
-  float t[2][2] = {};
+  // candidates: the free widths x the free block orders
+  std::vector<std::pair<int, int>> cand;
+  for (int w : {16, 32})
+    for (int o : {0, 1})
+      if ((!fixed_x_ || w == fixed_x_) && (fixed_order_ < 0 || o == fixed_order_)) cand.push_back({w, o});
+  const std::pair<int, int> rule{fixed_x_ ? fixed_x_ : rule_x_, fixed_order_ >= 0 ? fixed_order_ : kRuleOrder};
+  if (cand.size() < 2) return;
+  std::vector<float> t[2];
+  t[0].assign(cand.size(), 0.f);
+  t[1].assign(cand.size(), 0.f);
   hipEvent_t e0, e1;
   SWFWI_HIP(hipEventCreate(&e0));
   SWFWI_HIP(hipEventCreate(&e1));
   for (int round = 0; round < 3; round++)
-    for (int wi = 0; wi < 2; wi++) {
-      layout_tiles(widths[wi]);
+    for (size_t wi = 0; wi < cand.size(); wi++) {
+      layout_tiles(cand[wi].first, cand[wi].second);
       BatchKernelArgs a = args(true);
       a.tap = nullptr;
       SWFWI_HIP(hipEventRecord(e0, s));
@@ -149,10 +171,15 @@ void Batch::tune() {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   SWFWI_HIP(hipGetLastError());
-  const int rule = rule_x_ == widths[1] ? 1 : 0, other = 1 - rule;
-  const bool faster = t[0][other] < (1.f - kBatchTuneMargin) * t[0][rule] &&
-                      t[1][other] < (1.f - kBatchTuneMargin) * t[1][rule];
-  layout_tiles(widths[faster ? other : rule]);
+  size_t ri = 0;
+  for (size_t i = 0; i < cand.size(); i++)
+    if (cand[i] == rule) ri = i;
+  size_t best = ri;
+  for (size_t i = 0; i < cand.size(); i++) {
+    const bool faster = t[0][i] < (1.f - kBatchTuneMargin) * t[0][ri] && t[1][i] < (1.f - kBatchTuneMargin) * t[1][ri];
+    if (faster && std::max(t[0][i], t[1][i]) < std::max(t[0][best], t[1][best])) best = i;
+  }
+  layout_tiles(cand[best].first, cand[best].second);
 }
 
 void Batch::set_snap_prec(int p) {
@@ -257,9 +284,9 @@ void Batch::build() {
     if (list[i].e.nt - list[i].e.it_last > list[i - 1].e.nt - list[i - 1].e.it_last)
       fail(kErrArg, "entries with fewer steps must not need more backward iterations");
   run_of_.assign(list.size(), 0);
+  if (list.size() >= (1u << (31 - kBlkTileBits))) fail(kErrRange, "too many batch entries");
   long long f = 0, snap = 0;
-  int scale = 0, tiles = 0;
-  std::vector<int> blk;
+  int scale = 0;
   for (size_t r = 0; r < list.size(); r++) {
     BatchEntry& e = list[r].e;
     const long long npts = (long long)e.nxpad * e.nzpad;
@@ -270,13 +297,8 @@ void Batch::build() {
     e.scale0 = scale;
     scale += e.nsnap;
     run_of_[list[r].ctor] = (int)r;
-    tile0_.push_back(tiles);
-    const int nt_e = e.ntz * e.ntx;
-    blk.insert(blk.end(), nt_e, (int)r);
-    tiles += nt_e;
     ent_.push_back(e);
   }
-  tile0_.push_back(tiles);
   fields_ = f;
   models_len_ = model_len;
   dcal_len_ = dcal_len;
@@ -286,9 +308,6 @@ void Batch::build() {
   const size_t esz = prec_ == kSnapF32 ? 4 : (prec_ == kSnapNone ? 0 : 2);
   snap_bytes_ = (long long)(snap * esz);
 
-  dent_.upload(ent_.data(), ent_.size(), s);
-  dblk_.upload(blk, s);
-  dtile0_.upload(tile0_, s);
   amp_.upload(amp, s);
   wts_.upload(wts, s);
   rv_.resize((size_t)models_len_);
@@ -301,7 +320,7 @@ void Batch::build() {
   mgrad_.resize((size_t)models_len_);
   snap_.resize((size_t)std::max(snap_bytes_, 1LL));
   scale_.resize((size_t)std::max(scales_, 1));
-  SWFWI_HIP(hipStreamSynchronize(s));
+  layout_tiles(tile_x_, kRuleOrder);  // uploads the entries and the block list
 }
 
 void Batch::bind_model(int g, int m, const float* vel_t) {
@@ -320,8 +339,7 @@ void Batch::bind_model(int g, int m, const float* vel_t) {
 BatchKernelArgs Batch::args(bool fwd) {
   BatchKernelArgs a{};
   a.ent = dent_.get();
-  a.blk_entry = dblk_.get();
-  a.tile0 = dtile0_.get();
+  a.blk = dblk_.get();
   a.rv = rv_.get();
   a.amp = amp_.get();
   a.wts = wts_.get();

@@ -25,6 +25,8 @@ constexpr int kSnapEvery = 4;
 // the 793x277 padded coarse grid, 3 920, 8% slower).
 constexpr int kBatchTileZ = 64;
 constexpr int kBatchWideTiles = 8000;
+constexpr int kBlkTileBits = 20;  // tiles per entry < 2^20, entries < 2^11
+constexpr int kRuleOrder = 1;      // block order without tuning: a model's shots tile by tile
 constexpr float kBatchTuneMargin = 0.03f;
 enum SnapPrec : int { kSnapF32 = 0, kSnapBF16 = 1, kSnapF16 = 2, kSnapNone = 3 };
 // Stencils: the reference's damped 10th-order-in-space, 4th-order-in-time
@@ -72,8 +74,7 @@ struct BatchEntry {
 
 struct BatchKernelArgs {
   const BatchEntry* ent;
-  const int* blk_entry;  // entry of each block of a launch over all entries
-  const int* tile0;      // first block of each entry
+  const int* blk;        // per block of a launch: entry << kBlkTileBits | its tile
   const float* rv;
   const float* amp;
   const float* wts;
@@ -119,10 +120,15 @@ class Batch {
   // over every entry, and keeps the width of the size rule unless the other
   // is faster by kBatchTuneMargin in both timed rounds.  Off: the rule.
   void set_tuning(bool on);
-  // A fixed tile width (16 or 32; the tuner is skipped), or 0 for the rule
-  // and the tuner.
+  // A fixed tile width (16 or 32), or 0 for the rule and the tuner.
   void set_tile_x(int tx);
   int tile_x() const { return tile_x_; }
+  // Block order of a launch: 0 entry by entry (the tiles of one entry, then
+  // the next), 1 the shots of each model tile by tile (a model tile is read
+  // from HBM once and from L2 by the model's other shots), or -1 for the
+  // rule (1) and the tuner.
+  void set_block_order(int order);
+  int block_order() const { return blk_order_; }
   // model m of group g: padded transformed velocity (Damp4t10d layout)
   void bind_model(int g, int m, const float* vel_t);
   // Forward run of every entry: records the seismograms and stores the
@@ -180,7 +186,7 @@ class Batch {
 
  private:
   void build();
-  void layout_tiles(int tx);  // entry tile counts and the block list for width tx
+  void layout_tiles(int tx, int order);  // entry tile counts and the block list
   void tune();
   void drop_graphs();
   void run_loop(bool fwd);
@@ -203,6 +209,8 @@ class Batch {
   int rule_x_ = 16;                  // the size rule's width
   bool tune_ = true, tuned_ = false;
   int fixed_x_ = 0;                  // a width set by the caller, 0 for none
+  int blk_order_ = 1;                // block order in use
+  int fixed_order_ = -1;             // an order set by the caller, -1 for none
   std::vector<long long> model_off_; // per model (groups in order)
   std::vector<int> model_group_;
   long long fields_ = 0, models_len_ = 0, dcal_len_ = 0, grad_len_ = 0, snap_elems_ = 0;
@@ -210,7 +218,7 @@ class Batch {
   long long snap_bytes_ = 0;
   Stream stream_;
   DBuf<BatchEntry> dent_;
-  DBuf<int> dblk_, dtile0_;
+  DBuf<int> dblk_;
   DBuf<float> rv_, amp_, wts_, fa_, fb_, dcal_, vsrc_, grad_, mgrad_;
   DBuf<unsigned char> snap_;
   DBuf<unsigned> scale_;

@@ -128,9 +128,10 @@ __global__ __launch_bounds__(kBT) void k_batch_step(BatchKernelArgs a) {
   const int n = a.nblocks, per = n / 8, rem = n % 8;
   const int xcd = blockIdx.x % 8, local = blockIdx.x / 8;
   const int l = xcd * per + (xcd < rem ? xcd : rem) + local;
-  const int e = a.blk_entry[l];
+  const int pk = a.blk[l];
+  const int e = pk >> kBlkTileBits;
   const BatchEntry E = a.ent[e];
-  const int tile = l - a.tile0[e];
+  const int tile = pk & ((1 << kBlkTileBits) - 1);
   const int it = FWD ? a.it_or_k : E.nt - 1 - a.it_or_k;
   const int tx = tile / E.ntz, tz = tile - tx * E.ntz;
   const int ox = kBH + tx * kBX, oz = kBH + tz * kBZ;

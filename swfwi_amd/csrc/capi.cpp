@@ -744,6 +744,21 @@ int swfwi_batch_set_tile_x(swfwi_batch b, int tile_x) {
   });
 }
 
+int swfwi_batch_set_block_order(swfwi_batch b, int order) {
+  return guarded([&] {
+    NEED(b);
+    b->b->set_block_order(order);
+  });
+}
+
+int swfwi_batch_block_order(swfwi_batch b, int* order) {
+  return guarded([&] {
+    NEED(b);
+    NEED(order);
+    *order = b->b->block_order();
+  });
+}
+
 int swfwi_batch_tile_x(swfwi_batch b, int* tile_x) {
   return guarded([&] {
     NEED(b);

@@ -183,36 +183,40 @@ def test_wide_tile_batch_against_oracles(oracle):
     b.close()
 
 
-def test_batch_tile_widths_and_tuner():
-    """The step kernels at tile widths 16 and 32 give the mixed-size batch's
-    seismograms and gradients (BF16 checkpoints) bit for bit alike: each
-    output point is formed by the same operations in the same order whatever
-    the tile.  With the tuner (the default) the first forward run picks one
-    of them; with it off the size rule decides (16 for this small batch)."""
+def test_batch_tile_widths_block_orders_and_tuner():
+    """The step kernels at tile widths 16 and 32, with the blocks of a launch
+    entry by entry (order 0) or a model's shots tile by tile (order 1), give
+    the mixed-size batch's seismograms and gradients (BF16 checkpoints) bit
+    for bit alike: each output point is formed by the same operations in the
+    same order whatever the tile and the block order.  With the tuner (the
+    default) the first forward run picks one of them; with it off the rule
+    decides (width 16 for this small batch, order 1)."""
     gs = groups()
     mods = {(gi, m): models_for(g, 2 * gi + m) for gi, g in enumerate(gs) for m in range(g.nmodels)}
     out = {}
-    for mode in ("16", "32", "tuner", "rule"):
+    modes = [(16, 0), (16, 1), (32, 0), (32, 1), "tuner", "rule"]
+    for mode in modes:
         b = Batch(gs, snap_prec=capi.SNAP_BF16)
-        if mode in ("16", "32"):
-            b.set_tile_x(int(mode))
         if mode == "rule":
             b.set_tuning(False)
+        elif mode != "tuner":
+            b.set_tile_x(mode[0])
+            b.set_block_order(mode[1])
         for (gi, m), (vt, vi) in mods.items():
             b.bind_model(gi, m, vi)
         d = b.forward()
-        tx = b.tile_x()
-        print(f"{mode}: tile width {tx}", flush=True)
-        assert tx in (16, 32)
-        if mode in ("16", "32"):
-            assert tx == int(mode)
+        got = (b.tile_x(), b.block_order())
+        print(f"{mode}: tile width {got[0]}, block order {got[1]}", flush=True)
+        assert got[0] in (16, 32) and got[1] in (0, 1)
+        if isinstance(mode, tuple):
+            assert got == mode
         if mode == "rule":
-            assert tx == 16
+            assert got == (16, 1)
         vsrc = (np.random.default_rng(5).standard_normal(d.size) * 1e-3).astype(np.float32)
         ge, gm = b.backward(vsrc)
         assert np.count_nonzero(ge) > 0
         out[mode] = (d, ge, gm)
         b.close()
-    for mode in ("32", "tuner", "rule"):
+    for mode in modes[1:]:
         for k in range(3):
-            assert np.array_equal(out[mode][k].view(np.uint32), out["16"][k].view(np.uint32)), (mode, k)
+            assert np.array_equal(out[mode][k].view(np.uint32), out[(16, 0)][k].view(np.uint32)), (mode, k)
