@@ -59,7 +59,8 @@ void Batch::set_tuning(bool on) {
 }
 
 void Batch::set_tile_x(int tx) {
-  if (tx != 0 && tx != 16 && tx != 32) fail(kErrArg, "batch tile width must be 0 (rule and tuner), 16 or 32");
+  if (tx != 0 && tx != 16 && tx != 24 && tx != 32)
+    fail(kErrArg, "batch tile width must be 0 (rule and tuner), 16, 24 or 32");
   fixed_x_ = tx;
   set_tuning(tune_);
 }
@@ -130,7 +131,7 @@ void Batch::tune() {
   }
   // candidates: the free widths x the free block orders
   std::vector<std::pair<int, int>> cand;
-  for (int w : {16, 32})
+  for (int w : {16, 24, 32})
     for (int o : {0, 1})
       if ((!fixed_x_ || w == fixed_x_) && (fixed_order_ < 0 || o == fixed_order_)) cand.push_back({w, o});
   const std::pair<int, int> rule{fixed_x_ ? fixed_x_ : rule_x_, fixed_order_ >= 0 ? fixed_order_ : kRuleOrder};
@@ -210,7 +211,7 @@ void Batch::build() {
     const long long ux = G.nx + 2 * G.nb, uz = G.nz + G.nb;  // update region
     t16 += (long long)G.nmodels * (long long)G.sx.size() * ((uz + kBatchTileZ - 1) / kBatchTileZ) * ((ux + 15) / 16);
   }
-  tile_x_ = rule_x_ = t16 >= kBatchWideTiles ? 32 : 16;
+  tile_x_ = rule_x_ = t16 >= kBatchWideTiles ? 24 : 16;
   for (size_t gi = 0; gi < g_.size(); gi++) {
     const BatchGroup& G = g_[gi];
     const int ns = (int)G.sx.size();

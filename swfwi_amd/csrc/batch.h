@@ -18,11 +18,15 @@
 namespace swfwi {
 
 constexpr int kSnapEvery = 4;
-// Step-kernel tile: 64 z x 16 or 32 x output points per block of 256
-// threads (each thread 4 or 8 consecutive x at one z); 32 when the batch
-// holds at least kBatchWideTiles tiles of 64 x 16 (measured: 16 entries of
-// the 501x1501 grid, 14 112 such tiles, run 7% faster with 32; 16 entries of
-// the 793x277 padded coarse grid, 3 920, 8% slower).
+// Step-kernel tile: 64 z x 16, 24 or 32 x output points per block of 256
+// threads (each thread 4, 6 or 8 consecutive x at one z).  The rule: 24 when
+// the batch holds at least kBatchWideTiles tiles of 64 x 16, else 16
+// (measured with a model's shots tile by tile: 16 entries of the 501x1501
+// grid, 14 112 such tiles, forward + backward 300.6 ms at 24 against 310.4 at
+// 32 and 329.4 at 16; the 112 mixed-size entries 1033 ms at 24 against 1068 at
+// 32; 16 entries of the 793x277 padded coarse grid, 3 920 tiles, 54.4 ms at
+// 16, 55.1 at 24, 61.1 at 32).  At 24 a block's LDS (17.8 KB) leaves room for
+// 8 blocks per CU, at 32 (22.4 KB) for 7.
 constexpr int kBatchTileZ = 64;
 constexpr int kBatchWideTiles = 8000;
 constexpr int kBlkTileBits = 20;  // tiles per entry < 2^20, entries < 2^11

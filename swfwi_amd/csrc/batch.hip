@@ -370,12 +370,21 @@ static void launch_step(const BatchKernelArgs& a, int prec, hipStream_t s) {
       break;
   }
 }
+template <bool FWD, int ORDER>
+static void launch_width(const BatchKernelArgs& a, int prec, int tile_x, hipStream_t s) {
+  if (tile_x == 32)
+    launch_step<FWD, ORDER, 32>(a, prec, s);
+  else if (tile_x == 24)
+    launch_step<FWD, ORDER, 24>(a, prec, s);
+  else
+    launch_step<FWD, ORDER, 16>(a, prec, s);
+}
 template <bool FWD>
 static void launch_dir(const BatchKernelArgs& a, int prec, int order, int tile_x, hipStream_t s) {
   if (order == kOrder4)
-    tile_x == 32 ? launch_step<FWD, kOrder4, 32>(a, prec, s) : launch_step<FWD, kOrder4, 16>(a, prec, s);
+    launch_width<FWD, kOrder4>(a, prec, tile_x, s);
   else
-    tile_x == 32 ? launch_step<FWD, kOrder10, 32>(a, prec, s) : launch_step<FWD, kOrder10, 16>(a, prec, s);
+    launch_width<FWD, kOrder10>(a, prec, tile_x, s);
 }
 void launch_batch_fwd(const BatchKernelArgs& a, int prec, int order, int tile_x, hipStream_t s) {
   launch_dir<true>(a, prec, order, tile_x, s);

@@ -146,9 +146,9 @@ def test_batch_gradient_against_fp64_oracle(oracle, run, prec):
 
 
 def test_wide_tile_batch_against_oracles(oracle):
-    """A batch big enough for the 64 x 32 step tiles (24 entries of the
-    301x921 grid: 8928 tiles of 64 x 16, above the 8000 at which the batch
-    switches to 32-wide tiles), nt=800: seismograms of the first and last
+    """A batch big enough for the wide step tiles (24 entries of the 301x921
+    grid: 8928 tiles of 64 x 16, above the 8000 from which the batch's rule
+    takes 24-wide tiles), run with 32-wide tiles, nt=800: seismograms of the first and last
     entry against the FP32 oracle, their gradients from FP32 checkpoints
     against the FP64 oracle run."""
     g = Group(nx=921, nz=301, nb=30, dx=12.5, dt=0.001, fm=10.0, nt=800,
@@ -156,7 +156,7 @@ def test_wide_tile_batch_against_oracles(oracle):
     ux, uz = g.nx + 2 * g.nb, g.nz + g.nb
     assert g.nmodels * len(g.sx) * ((uz + 63) // 64) * ((ux + 15) // 16) >= 8000
     b = Batch([g], snap_prec=capi.SNAP_F32)
-    b.set_tile_x(32)  # the rule's width here; fixed so the tuner cannot pick 16
+    b.set_tile_x(32)  # fixed: the widest tile, which neither the rule nor the tuner need pick
     mods = [models_for(g, 7 + m) for m in range(2)]
     for m in range(2):
         b.bind_model(0, m, mods[m][0])
@@ -184,7 +184,7 @@ def test_wide_tile_batch_against_oracles(oracle):
 
 
 def test_batch_tile_widths_block_orders_and_tuner():
-    """The step kernels at tile widths 16 and 32, with the blocks of a launch
+    """The step kernels at tile widths 16, 24 and 32, with the blocks of a launch
     entry by entry (order 0) or a model's shots tile by tile (order 1), give
     the mixed-size batch's seismograms and gradients (BF16 checkpoints) bit
     for bit alike: each output point is formed by the same operations in the
@@ -194,7 +194,7 @@ def test_batch_tile_widths_block_orders_and_tuner():
     gs = groups()
     mods = {(gi, m): models_for(g, 2 * gi + m) for gi, g in enumerate(gs) for m in range(g.nmodels)}
     out = {}
-    modes = [(16, 0), (16, 1), (32, 0), (32, 1), "tuner", "rule"]
+    modes = [(16, 0), (16, 1), (24, 1), (32, 0), (32, 1), "tuner", "rule"]
     for mode in modes:
         b = Batch(gs, snap_prec=capi.SNAP_BF16)
         if mode == "rule":
@@ -207,7 +207,7 @@ def test_batch_tile_widths_block_orders_and_tuner():
         d = b.forward()
         got = (b.tile_x(), b.block_order())
         print(f"{mode}: tile width {got[0]}, block order {got[1]}", flush=True)
-        assert got[0] in (16, 32) and got[1] in (0, 1)
+        assert got[0] in (16, 24, 32) and got[1] in (0, 1)
         if isinstance(mode, tuple):
             assert got == mode
         if mode == "rule":
