@@ -157,6 +157,27 @@ __global__ __launch_bounds__(kBT) void k_batch_step(BatchKernelArgs a) {
     rvj[j] = ok ? at(rvb, p) : 0.f;
     pvj[j] = ok ? at(nxt, p) : 0.f;
   }
+  // backward imaging steps: the checkpoint and the gradient of the thread's
+  // outputs are loaded here too, ahead of the staging
+  const bool image_now =
+      !FWD && PREC != kSnapNone && it >= kSnapEvery && (it % kSnapEvery == 0) && a.wts[E.wts0 + it] != 0.f;
+  const long long snap_off = E.snap + (long long)(FWD ? it / kSnapEvery : it / kSnapEvery - 1) * npts;
+  void* snapb = PREC == kSnapNone ? a.snap
+                : PREC == kSnapF32 ? static_cast<void*>(static_cast<float*>(a.snap) + snap_off)
+                                   : static_cast<void*>(static_cast<unsigned short*>(a.snap) + snap_off);
+  float* gradb = a.grad + E.grad;
+  float svj[kPX], gvj[kPX];
+  if (image_now) {
+    const float iscale = __uint_as_float(a.scale[E.scale0 + it / kSnapEvery - 1]);
+#pragma unroll
+    for (int j = 0; j < kPX; j++) {
+      const int gx = ox + xg * kPX + j;
+      const bool ok = gx < E.nxpad - kBH && gz < E.nzpad - kBH;
+      const int p = gx * ld + gz;
+      svj[j] = ok ? snap_load<PREC == kSnapNone ? kSnapF32 : PREC>(snapb, p, iscale) : 0.f;
+      gvj[j] = ok ? at(gradb, p) : 0.f;
+    }
+  }
   // stage cur + 6-cell halo: each wavefront takes whole rows (one z per
   // lane, the last 12 z of a row by lanes 0-11), so a row is two coalesced
   // loads and no lane divides its index
@@ -233,18 +254,11 @@ __global__ __launch_bounds__(kBT) void k_batch_step(BatchKernelArgs a) {
   const bool sponge = ox < E.damp || ox + kBX > E.nxpad - E.damp || oz + kBZ > E.nzpad - E.damp;
   const bool snap_now = FWD && PREC != kSnapNone && (it % kSnapEvery == kSnapEvery - 1);
   const bool max_now = FWD && ((it + 1) % kSnapEvery == kSnapEvery - 1) && (it + 1) / kSnapEvery < E.nsnap;
-  const bool image_now =
-      !FWD && PREC != kSnapNone && it >= kSnapEvery && (it % kSnapEvery == 0) && a.wts[E.wts0 + it] != 0.f;
   const bool inject_next = FWD ? it + 1 < E.nt : it - 1 >= E.it_last;
   float scale = 0.f;
   if (snap_now) scale = __uint_as_float(a.scale[E.scale0 + it / kSnapEvery]);
   if (image_now) scale = __uint_as_float(a.scale[E.scale0 + it / kSnapEvery - 1]);
   const float wimg = image_now ? a.wts[E.wts0 + it] : 0.f;
-  const long long snap_off = E.snap + (long long)(FWD ? it / kSnapEvery : it / kSnapEvery - 1) * npts;
-  void* snapb = PREC == kSnapNone ? a.snap
-                : PREC == kSnapF32 ? static_cast<void*>(static_cast<float*>(a.snap) + snap_off)
-                                   : static_cast<void*>(static_cast<unsigned short*>(a.snap) + snap_off);
-  float* gradb = a.grad + E.grad;
   float vmax = 0.f;
 #pragma unroll
   for (int j = 0; j < kPX; j++) {
@@ -284,11 +298,7 @@ __global__ __launch_bounds__(kBT) void k_batch_step(BatchKernelArgs a) {
       }
     } else {
       if (inject_next && rec) v += a.dcal[E.dcal + (long long)(it - 1) * E.ng + (gx - rx0)];
-      if (image_now && PREC != kSnapNone) {
-        const float s = snap_load<PREC == kSnapNone ? kSnapF32 : PREC>(snapb, p, scale);
-        float& g = at(gradb, p);
-        g = g - wimg * s * c;
-      }
+      if (image_now && PREC != kSnapNone) at(gradb, p) = gvj[j] - wimg * svj[j] * c;
     }
     at(nxt, p) = v;
   }
