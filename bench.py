@@ -245,7 +245,7 @@ def learned_workload(args):
 
     learned.GRAPHS = not args.no_graphs
     t0 = time.perf_counter()
-    case = learned.config4_case()
+    case = learned.fwi_501x1501_case()
     val = learned.random_models(case, 2, 10 ** 6)  # validation models, seeds far from every training seed
     gains = (1.0, 0.5, 0.25)
     if args.train == "small":

@@ -22,7 +22,7 @@ def small_case():
     return dict(nx=161, nz=80, nb=20, dx=10.0, dt=0.001, fm=8.0, nt=1200, vmin=1500.0, vmax=3500.0)
 
 
-def config4_case():
+def fwi_501x1501_case():
     """The 501 x 1501 fine grid at 12.5 m (nb=30, dt=1 ms, 8 Hz, nt=3000)
     and its 250 x 751 coarse grid at 25 m (nb=15, dt=2 ms)."""
     return dict(nx=1501, nz=501, nb=30, dx=12.5, dt=0.001, fm=8.0, nt=3000, vmin=1500.0, vmax=4500.0)
@@ -68,7 +68,8 @@ def grids(case, shots=None):
 
 
 def random_models(case, n, seed):
-    """n seeded random layered models on the fine grid (config-0 recipe),
+    """n seeded random layered models on the fine grid (the layered recipe of
+    workloads.layered_model),
     [n][nx][nz] in m/s."""
     return np.stack([workloads.layered_model(case["nz"], case["nx"], case["vmin"], case["vmax"], seed + k)
                      for k in range(n)])
@@ -249,7 +250,7 @@ def train(case, models, rounds=3, steps_per_round=3000, starts=None, batch=16, l
 def crop_pairs(case, n_models, rnd, net=None, seed=0, chunk=100, starts=2, crop=112, crops=2, floor=0.1, log=None):
     """Training pairs of round `rnd` from full-size fine runs, as crops.
 
-    n_models seeded random layered models (`seed + k`, the config-0 recipe)
+    n_models seeded random layered models (`seed + k`, workloads.layered_model)
     in chunks of `chunk`: each chunk is one fine batch (one shot per model, at
     a seeded even x on fine row 3, the same for the chunk) tapped at the
     coarse points; from each of `starts` seeded coarse start steps j (3 mod 4)

@@ -20,7 +20,7 @@ from oracle import pyoracle  # noqa: E402
 
 # (nx, nz, nb, seed): padded dims and the damping width handed to the kernel
 CASES = [(48, 40, 14, 11), (20, 16, 8, 12), (37, 29, 20, 13)]
-# multi-step case on the config-0 padded grid: point source, damped steps
+# multi-step case on the 201x301 (nb=20) padded grid: point source, damped steps
 C0 = dict(nx=353, nz=233, nb=26, steps=40, seed=14)
 
 
@@ -40,7 +40,7 @@ def main():
         ref.fd4t10s_zjh_2d_vtrans(pu, cur, vel, uu, nx, nz)
         out.update({f"c{k}_cur": cur, f"c{k}_prev": prev, f"c{k}_vel": vel, f"c{k}_nb": np.int32(nb),
                     f"c{k}_damp_next": pd, f"c{k}_damp_u2": ud, f"c{k}_undamp_next": pu, f"c{k}_undamp_u2": uu})
-    # config-0 grid: a unit spike stepped C0["steps"] times with the damped kernel
+    # 201x301 (nb=20) grid: a unit spike stepped C0["steps"] times with the damped kernel
     nx, nz, nb = C0["nx"], C0["nz"], C0["nb"]
     rng = np.random.default_rng(C0["seed"])
     vel = rng.uniform(4.9, 44.0, (nx, nz)).astype(np.float32)

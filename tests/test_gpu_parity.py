@@ -3,8 +3,8 @@ same seeded inputs.
 
 Tolerances: in exact mode the kernels reproduce the reference's expression
 trees, so seismograms and gradients must be bit-identical; in FP32 mode the
-north-star budgets apply (seismogram relative L2 <= 1e-5, gradient relative
-L2 <= 1e-4)."""
+FP32 budgets apply (seismogram relative L2 <= 1e-5, gradient relative L2 <=
+1e-4)."""
 import numpy as np
 import pytest
 
