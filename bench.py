@@ -405,7 +405,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: --gpus {args.gpus} is not supported (shots/members shard across GPUs "
-              "with one gradient all-reduce per iteration; that path is not part of this build)", file=sys.stderr)
+              "with one gradient all-reduce per iteration; that path is not implemented)", file=sys.stderr)
         sys.exit(2)
 
     from swfwi_amd import capi, workloads

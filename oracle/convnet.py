@@ -1,8 +1,8 @@
 """numpy restatement of the learned coarse-grid correction network: forward
 pass, loss and its gradient with respect to every weight.
 
-TEST INFRASTRUCTURE ONLY (the checker of the network's HIP kernels); the
-product never imports it.
+TEST INFRASTRUCTURE ONLY (what the network's HIP kernels are checked
+against); the product never imports it.
 
 Network: y = sum_t sum_ci W3[t][ci] h2[ci](x + o_t),
 h2 = relu(sum_t sum_ci W2[t][ci][co] h1[ci](x + o_t)),

@@ -2,8 +2,8 @@
 pAnalyze, enkfanalyze.cpp:113-183,343-495,578-597; clipPosition,
 Matrix.cpp:170-185) in FP64 with LAPACK's SVD.
 
-TEST INFRASTRUCTURE ONLY (the checker of the GPU analysis).  The data-row
-order is the build's [nt][ng]; the reference stores [ng][nt], a row
+TEST INFRASTRUCTURE ONLY (what the GPU analysis is checked against).  The
+data-row order is [nt][ng] here; the reference stores [ng][nt], a row
 permutation that leaves every product unchanged."""
 import numpy as np
 

@@ -1,6 +1,6 @@
 /*
  * oracle.c -- CPU restatement of the swfwi acoustic FD propagation and
- * adjoint-gradient path (the checker).
+ * adjoint-gradient path, the reference results of the parity tests.
  *
  * TEST INFRASTRUCTURE ONLY.  This file is the parity oracle: it is built into
  * oracle/liboracle.so and loaded by tests/, by __graft_entry__.smoke() and by

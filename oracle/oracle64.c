@@ -1,6 +1,6 @@
 /*
  * oracle64.c -- FP64 run of the acoustic propagation and adjoint gradient
- * (the checker of the low-precision checkpoint path).
+ * (what the low-precision checkpoint path is measured against).
  *
  * TEST INFRASTRUCTURE ONLY.  Built into oracle/liboracle.so beside oracle.c;
  * loaded by tests/ and by bench.py's CPU legs, never by the product library.

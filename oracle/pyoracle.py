@@ -2,8 +2,8 @@
 own stencil kernels compiled unchanged (oracle/_ref/libref.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by __graft_entry__.smoke() and by
-the CPU-baseline leg of bench.py, as the checker.  The product package never
-imports it.
+the CPU-baseline leg of bench.py, as the results the GPU path is checked
+against.  The product package never imports it.
 """
 import ctypes as C
 import os
