@@ -154,7 +154,15 @@ __device__ bool forward_tile(const float* __restrict__ p, const float* __restric
   for (int q = 0; q < 5; q++) a2p[q] = *reinterpret_cast<const hbf16x8*>(w.w2p + (q * 64 + lane) * 8);
   const bf16x4 a3 = *reinterpret_cast<const bf16x4*>(w.w3a + lane * 4);
   const int khalf = kq >> 1, kc = 8 * (kq & 1);  // this lane's tap of the pair, first channel
-  for (int grp = wave; grp < kGroups2; grp += kT / 64) {
+  // layer 2 per group, its result kept in registers; then (apply) layer 3
+  // into u, which may share its memory with h1: every layer-2 read of h1 is
+  // done at the barrier between the two
+  constexpr int kMaxG = (kGroups2 + kT / 64 - 1) / (kT / 64);  // groups per wavefront, at most
+  bf16x4 hbs[kMaxG];
+#pragma unroll
+  for (int i = 0; i < kMaxG; i++) {
+    const int grp = wave + i * (kT / 64);
+    if (grp >= kGroups2) break;                // wave-uniform
     const int qa = grp * 16 + (lane & 15);     // this lane's point
     const int qs = qa < kNH2 ? qa : kNH2 - 1;  // points past the region read a valid one
     const int x = qs / kH2, z = qs % kH2;      // layer-2 coords; layer-1 = +1
@@ -172,11 +180,18 @@ __device__ bool forward_tile(const float* __restrict__ p, const float* __restric
       if (khalf) b = hbf16x8{};
       acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2p[4], b, acc, 0, 0, 0);
     }
-    const bf16x4 hb = relu_bf16(acc);  // h2[4 kq + v][qa]
-    if (h2 && qa < kNH2) *reinterpret_cast<bf16x4*>(h2 + qa * kNetC + 4 * kq) = hb;
-    if (u) {
+    hbs[i] = relu_bf16(acc);  // h2[4 kq + v][qa]
+    if (h2 && qa < kNH2) *reinterpret_cast<bf16x4*>(h2 + qa * kNetC + 4 * kq) = hbs[i];
+  }
+  if (u) {  // uniform over the grid
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < kMaxG; i++) {
+      const int grp = wave + i * (kT / 64);
+      if (grp >= kGroups2) break;
+      const int qa = grp * 16 + (lane & 15);
       f32x4 y3 = {0.f, 0.f, 0.f, 0.f};
-      y3 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a3, hb, y3, 0, 0, 0);  // U[4 kq + v][qa]
+      y3 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a3, hbs[i], y3, 0, 0, 0);  // U[4 kq + v][qa]
 #pragma unroll
       for (int v = 0; v < 4; v++) {
         const int t = 4 * kq + v;
@@ -191,8 +206,10 @@ __device__ bool forward_tile(const float* __restrict__ p, const float* __restric
 __global__ __launch_bounds__(kT) void k_net_apply(const float* __restrict__ p_all, float* __restrict__ y_all,
                                                   NetGeom g, NetDev w) {
   __shared__ unsigned sx[kP * kP];
+  // h1 (layer 1) and then u (layer 3): u overwrites h1 after layer 2
   __shared__ __align__(16) uint16_t h1[kNH1 * kNetC];
-  __shared__ float u[9 * kNH2];
+  static_assert(9 * kNH2 * sizeof(float) <= kNH1 * kNetC * sizeof(uint16_t), "u fits in h1");
+  float* u = reinterpret_cast<float*>(h1);
   const TileMap m = tile_of(g, 0);
   const float* p = p_all + (long long)m.e * g.stride;
   const float* r = g.r + (long long)(m.e / g.rdiv) * g.rstride;
