@@ -239,13 +239,13 @@ int swfwi_batch_destroy(swfwi_batch b);
 int swfwi_batch_set_snapshot_precision(swfwi_batch b, int snap_prec);
 /* Capture each time loop into a HIP graph on first use (on by default). */
 int swfwi_batch_set_graphs(swfwi_batch b, int on);
-/* The tile tuner (1, the default) or the size rule alone (0): the first
- * forward run times the step kernels at tile widths 16 and 32 over every
- * entry and keeps the rule's width unless the other is faster by 3% in both
- * timed rounds.  tile_x receives the width in use. */
+/* The tuner (1, the default) or the rules alone (0): the first forward run
+ * times the step kernels at tile widths 16, 24 and 32 and both block orders
+ * over every entry, and keeps the rules' choice (width 24 from 8000 tiles of
+ * 64 x 16, else 16; order 1) unless another is faster by 3% in both timed
+ * rounds.  swfwi_batch_tile_x receives the width in use. */
 int swfwi_batch_set_tuning(swfwi_batch b, int on);
-/* A fixed tile width, 16 or 32 (the tuner is skipped), or 0 for the rule
- * and the tuner. */
+/* A fixed tile width, 16, 24 or 32, or 0 for the rule and the tuner. */
 int swfwi_batch_set_tile_x(swfwi_batch b, int tile_x);
 /* Block order of the step launches: 0 entry by entry, 1 the shots of each
  * model tile by tile (a model tile is read once from HBM and then from L2),

@@ -85,7 +85,7 @@ class Batch:
         check(load().swfwi_batch_set_tuning(self.h, int(bool(on))))
 
     def set_tile_x(self, tx):
-        """16 or 32 (fixed, no tuning), or 0 for the size rule and the tuner."""
+        """16, 24 or 32 (fixed), or 0 for the size rule and the tuner."""
         check(load().swfwi_batch_set_tile_x(self.h, int(tx)))
 
     def set_block_order(self, order):

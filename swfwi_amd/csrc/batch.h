@@ -119,12 +119,13 @@ class Batch {
   int snap_prec() const { return prec_; }
   void set_snap_prec(int p);
   void set_graphs(bool on) { graphs_ = on; }
-  // Tile tuner (on by default): the first forward run first times the step
-  // kernels (forward and, with checkpoints, backward) at both tile widths
-  // over every entry, and keeps the width of the size rule unless the other
-  // is faster by kBatchTuneMargin in both timed rounds.  Off: the rule.
+  // Tuner (on by default): the first forward run first times the step
+  // kernels (forward and, with checkpoints, backward) over every entry for
+  // each free tile width (16, 24, 32) and block order, and keeps the rules'
+  // choice unless another is faster by kBatchTuneMargin in both timed
+  // rounds.  Off: the rules.
   void set_tuning(bool on);
-  // A fixed tile width (16 or 32), or 0 for the rule and the tuner.
+  // A fixed tile width (16, 24 or 32), or 0 for the rule and the tuner.
   void set_tile_x(int tx);
   int tile_x() const { return tile_x_; }
   // Block order of a launch: 0 entry by entry (the tiles of one entry, then
