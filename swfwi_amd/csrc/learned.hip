@@ -1,19 +1,21 @@
 // Kernels of the learned coarse-grid correction (learned.h).
 //
 // One block per 16 x 16 tile of output points (z fastest), 256 threads (4
-// waves).  Forward, per tile, every layer on v_mfma_f32_16x16x16_bf16 with
-// the 16 points of a group on the product's N side:
+// waves).  Forward, per tile, every layer on BF16 MFMA with the 16 points of
+// a group on the product's N side:
 //   stage   p and r p (BF16 pairs) with a 3-cell halo (22 x 22) in LDS;
 //   layer 1 D1[c][pt] = sum_k W1^T[c][k] X[k][pt], k = 2 t + (p | r p),
-//           over the tile + 2 (20 x 20): two products (K = 18 padded to 32),
-//           ReLU, BF16 into LDS;
+//           over the tile + 2 (20 x 20): two 16x16x16 products (K = 18
+//           padded to 32), ReLU, BF16 into LDS;
 //   layer 2 D2[co][pt] = sum_t sum_ci W2_t^T[co][ci] h1[ci](pt + o_t) over
-//           the tile + 1 (18 x 18): nine products, ReLU, BF16;
+//           the tile + 1 (18 x 18): five 16x16x32 products (taps in pairs),
+//           ReLU, BF16, kept in registers;
 //   layer 3 U[t][pt] = sum_c W3[t][c] h2[c](pt) on the same 18 x 18 points,
-//           one product whose B operand is layer 2's accumulator as it
-//           stands (its D layout is the B layout), U into LDS;
+//           one 16x16x16 product whose B operand is layer 2's result as it
+//           stands (its D layout is the B layout), U into LDS over h1's
+//           memory once every wave is past layer 2;
 //   output  y(x) = sum_t U[t](x + o_t) on the tile.
-// So a 16-point group costs 2 + 9 + 1 products and h2 never leaves the
+// So a 16-point group costs 2 + 5 + 1 products and h2 never leaves the
 // registers.
 //
 // v_mfma_f32_16x16x16_bf16 lane layout (measured on the device): lane l
@@ -203,7 +205,9 @@ __device__ bool forward_tile(const float* __restrict__ p, const float* __restric
   return true;
 }
 
-__global__ __launch_bounds__(kT) void k_net_apply(const float* __restrict__ p_all, float* __restrict__ y_all,
+// 8 waves per SIMD (63 VGPRs, no spills): 5-6% faster than the 7 the
+// compiler picks by itself (58 VGPRs plus the accumulators)
+__global__ __launch_bounds__(kT) __attribute__((amdgpu_waves_per_eu(8))) void k_net_apply(const float* __restrict__ p_all, float* __restrict__ y_all,
                                                   NetGeom g, NetDev w) {
   __shared__ unsigned sx[kP * kP];
   // h1 (layer 1) and then u (layer 3): u overwrites h1 after layer 2
