@@ -7,6 +7,7 @@
     python -m swfwi_amd.profile_workloads enkf      # EnKF analyses, 301x921, N members
     python -m swfwi_amd.profile_workloads enfwi     # ensemble-Kalman FWI epochs, 301x921, N members
     python -m swfwi_amd.profile_workloads sweep     # step-kernel time per tile width and batch size
+    python -m swfwi_amd.profile_workloads net       # network apply kernel over 16 coarse fields of the 501x1501 case
 """
 import argparse
 import sys
@@ -192,9 +193,33 @@ def run_enfwi(nt, members, epochs, graphs=True):
               f"{fwi.mean_objective():.4e}, steps {min(steps):.3g}..{max(steps):.3g}", flush=True)
 
 
+def run_net(reps):
+    """The learned correction's forward kernel alone: 16 smooth random fields
+    of the coarse grid of the 501x1501 learned-FWI case (793 x 277 padded),
+    1 / vel_t = 0.1, random weights; `reps` back-to-back launches, twice."""
+    from scipy.ndimage import gaussian_filter
+
+    from . import learned
+    from .batch import Net
+
+    case = learned.fwi_501x1501_case()
+    _, coarse = learned.grids(case)
+    shape = (coarse["nx"] + 2 * coarse["nb"] + 12, coarse["nz"] + coarse["nb"] + 12)
+    rng = np.random.default_rng(1)
+    fields = np.stack([gaussian_filter(rng.standard_normal(shape), 3.0) for _ in range(16)]).astype(np.float32)
+    rv = np.full(fields.shape, 0.1, np.float32)
+    reg = learned.region(coarse)
+    net = Net(Net.initial(seed=3, scale=0.1))
+    npts = (reg[1] - reg[0]) * (reg[3] - reg[2]) * 16
+    for _ in range(2):
+        ms = net.time_apply(fields, rv, reg, reps=reps)
+        print(f"net apply {shape} x16: {1e3 * ms:.2f} us per launch, {5472 * npts / (ms * 1e-3) / 1e12:.1f} TFLOP/s",
+              flush=True)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("mode", choices=["copy", "shots", "ess", "fwi", "enkf", "enfwi", "sweep"])
+    ap.add_argument("mode", choices=["copy", "shots", "ess", "fwi", "enkf", "enfwi", "sweep", "net"])
     ap.add_argument("--entries", default="1,2,3,4,8,16,60,100,200")
     ap.add_argument("--reps", type=int, default=100)
     ap.add_argument("--precision", choices=["exact", "fp32"], default="exact")
@@ -220,6 +245,8 @@ def main(argv=None):
         run_enkf(a.nt or 4000, a.members, graphs=not a.no_graphs)
     elif a.mode == "fwi":
         run_fwi(a.nt or 2000, a.epochs)
+    elif a.mode == "net":
+        run_net(a.reps)
     else:
         run_ess(a.nt or 1000, a.epochs, graphs=not a.no_graphs)
     return 0
