@@ -147,7 +147,7 @@ __device__ bool forward_tile(const float* __restrict__ p, const float* __restric
   // five K = 32 products (v_mfma_f32_16x16x32_bf16: lane l supplies
   // A[i = l%16][k = 8*(l/16) + v], B[k = 8*(l/16) + v][j = l%16]): product q
   // takes tap 2q at k < 16 and tap 2q+1 at k >= 16 (channel k%16); the fifth
-  // takes tap 8 at k < 16 and zeros.  One opcode for the whole accumulation
+  // takes tap 8 at k < 16 and zero weights.  One opcode for the whole accumulation
   // chain: a 16x16x16 product that read the accumulator of a 16x16x32 one
   // issued just before it has given wrong sums on gfx950 (no wait states are
   // inserted between the two).
@@ -156,6 +156,16 @@ __device__ bool forward_tile(const float* __restrict__ p, const float* __restric
   for (int q = 0; q < 5; q++) a2p[q] = *reinterpret_cast<const hbf16x8*>(w.w2p + (q * 64 + lane) * 8);
   const bf16x4 a3 = *reinterpret_cast<const bf16x4*>(w.w3a + lane * 4);
   const int khalf = kq >> 1, kc = 8 * (kq & 1);  // this lane's tap of the pair, first channel
+  // B operand of product q < 4: channels kc .. kc + 7 of tap 2q + khalf of
+  // the point's layer-1 neighbourhood; of product 4: those of tap 8 for every
+  // lane (the weights past k = 15 are zero and layer-1 values finite, so the
+  // lanes of k >= 16 add exact zeros)
+  int boff[5];
+#pragma unroll
+  for (int q = 0; q < 5; q++) {
+    const int t = q < 4 ? 2 * q + khalf : 8;
+    boff[q] = ((t / 3 - 1) * kH1 + (t % 3 - 1)) * kNetC + kc;
+  }
   // layer 2 per group, its result kept in registers; then (apply) layer 3
   // into u, which may share its memory with h1: every layer-2 read of h1 is
   // done at the barrier between the two
@@ -168,19 +178,12 @@ __device__ bool forward_tile(const float* __restrict__ p, const float* __restric
     const int qa = grp * 16 + (lane & 15);     // this lane's point
     const int qs = qa < kNH2 ? qa : kNH2 - 1;  // points past the region read a valid one
     const int x = qs / kH2, z = qs % kH2;      // layer-2 coords; layer-1 = +1
+    const uint16_t* c1 = h1 + ((x + 1) * kH1 + (z + 1)) * kNetC;  // the point in layer 1
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
-      const int t = 2 * q + khalf;
-      const int q1 = (x + 1 + t / 3 - 1) * kH1 + (z + 1 + t % 3 - 1);
-      const hbf16x8 b = *reinterpret_cast<const hbf16x8*>(h1 + q1 * kNetC + kc);
+    for (int q = 0; q < 5; q++) {
+      const hbf16x8 b = *reinterpret_cast<const hbf16x8*>(c1 + boff[q]);
       acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2p[q], b, acc, 0, 0, 0);
-    }
-    {
-      const int q1 = (x + 2) * kH1 + (z + 2);  // tap 8: (+1, +1)
-      hbf16x8 b = *reinterpret_cast<const hbf16x8*>(h1 + q1 * kNetC + kc);
-      if (khalf) b = hbf16x8{};
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2p[4], b, acc, 0, 0, 0);
     }
     hbs[i] = relu_bf16(acc);  // h2[4 kq + v][qa]
     if (h2 && qa < kNH2) *reinterpret_cast<bf16x4*>(h2 + qa * kNetC + 4 * kq) = hbs[i];
