@@ -197,10 +197,11 @@ __device__ bool forward_tile(const float* __restrict__ p, const float* __restric
       const int qa = grp * 16 + (lane & 15);
       f32x4 y3 = {0.f, 0.f, 0.f, 0.f};
       y3 = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a3, hbs[i], y3, 0, 0, 0);  // U[4 kq + v][qa]
+      if (qa < kNH2) {
+        float* ub = u + 4 * kq * kNH2 + qa;  // row 4 kq; rows past 8 are not kept
 #pragma unroll
-      for (int v = 0; v < 4; v++) {
-        const int t = 4 * kq + v;
-        if (t < 9 && qa < kNH2) u[t * kNH2 + qa] = y3[v];
+        for (int v = 0; v < 4; v++)
+          if (4 * kq + v < 9) ub[v * kNH2] = y3[v];
       }
     }
   }
