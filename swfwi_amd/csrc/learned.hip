@@ -123,18 +123,24 @@ __device__ bool forward_tile(const float* __restrict__ p, const float* __restric
   const bf16x4 a1b = *reinterpret_cast<const bf16x4*>(w.w1a + (64 + lane) * 4);
   if (!__syncthreads_or(nonzero) && skip_zero) return false;  // block-uniform
   // layer 1: lane's B items are (p, r p) at taps 2 kq and 2 kq + 1 (first
-  // product) and at tap 8 for kq = 0 (second product, the rest zero)
+  // product) and at tap 8 (second product: k = 16 + 4 kq + v, of which only
+  // k = 16, 17 have non-zero weights; the other lanes' finite values add
+  // exact zeros)
   {
     const int t0 = 2 * kq, t1 = 2 * kq + 1;
     const int o0 = (t0 / 3 - 1) * kP + (t0 % 3 - 1), o1 = (t1 / 3 - 1) * kP + (t1 % 3 - 1);
     const int o8 = kP + 1;
-    for (int grp = wave; grp < kGroups1; grp += kT / 64) {
+    constexpr int kMaxG1 = (kGroups1 + kT / 64 - 1) / (kT / 64);  // groups per wavefront, at most
+#pragma unroll
+    for (int i = 0; i < kMaxG1; i++) {
+      const int grp = wave + i * (kT / 64);
+      if (grp >= kGroups1) break;  // wave-uniform
       const int q = grp * 16 + (lane & 15);
       const int s = (q / kH1 + 1) * kP + (q % kH1 + 1);  // staged centre
       union { unsigned u2[2]; bf16x4 b; } b0, b1;
       b0.u2[0] = sx[s + o0];
       b0.u2[1] = sx[s + o1];
-      b1.u2[0] = kq == 0 ? sx[s + o8] : 0u;
+      b1.u2[0] = sx[s + o8];
       b1.u2[1] = 0u;
       f32x4 acc = {0.f, 0.f, 0.f, 0.f};
       acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1a, b0.b, acc, 0, 0, 0);
