@@ -112,7 +112,7 @@ __device__ bool forward_tile(const float* __restrict__ p, const float* __restric
   for (int q = tid; q < kP * kP; q += kT) {
     const int x = m.ox - 3 + q / kP, z = m.oz - 3 + q % kP;
     const bool in = x >= 0 && x < g.nx && z >= 0 && z < g.nz;
-    const long long o = (long long)x * g.nz + z;
+    const int o = x * g.nz + z;  // a field holds fewer than 2^31 points (checked)
     const float v = in ? p[o] : 0.f;
     const float vq = in ? r[o] * v : 0.f;
     sx[q] = pack_bf16(v, vq);
@@ -508,6 +508,7 @@ void check_geom(const NetGeom& g, int margin) {
   if (g.x0 < r || g.z0 < r || g.x1 > g.nx - r || g.z1 > g.nz - r || g.x1 <= g.x0 || g.z1 <= g.z0 || g.entries < 1)
     fail(kErrArg, "network region must lie 5 cells inside the field");
   if (tiles_of(g, margin) > 0x7fffffffLL) fail(kErrRange, "network launch too large");
+  if ((long long)g.nx * g.nz >= (1LL << 31)) fail(kErrRange, "network field too large");
 }
 
 }  // namespace
