@@ -82,6 +82,13 @@ __device__ __forceinline__ bf16x4 relu_bf16(const f32x4& a) {
   return r.b;
 }
 
+// Element i of a field whose base is block-uniform, the byte offset formed in
+// 32 bits (a field holds fewer than 2^29 points, checked), so the load takes
+// the base from scalar registers and one 32-bit offset
+__device__ __forceinline__ float ld_at(const float* base, unsigned i) {
+  return *reinterpret_cast<const float*>(reinterpret_cast<const char*>(base) + i * 4u);
+}
+
 struct TileMap {
   int e, ox, oz;  // entry, tile origin (global padded coords)
 };
@@ -112,9 +119,9 @@ __device__ bool forward_tile(const float* __restrict__ p, const float* __restric
   for (int q = tid; q < kP * kP; q += kT) {
     const int x = m.ox - 3 + q / kP, z = m.oz - 3 + q % kP;
     const bool in = x >= 0 && x < g.nx && z >= 0 && z < g.nz;
-    const int o = x * g.nz + z;  // a field holds fewer than 2^31 points (checked)
-    const float v = in ? p[o] : 0.f;
-    const float vq = in ? r[o] * v : 0.f;
+    const unsigned o = (unsigned)(x * g.nz + z);
+    const float v = in ? ld_at(p, o) : 0.f;
+    const float vq = in ? ld_at(r, o) * v : 0.f;
     sx[q] = pack_bf16(v, vq);
     nonzero |= v != 0.f;
   }
@@ -508,7 +515,7 @@ void check_geom(const NetGeom& g, int margin) {
   if (g.x0 < r || g.z0 < r || g.x1 > g.nx - r || g.z1 > g.nz - r || g.x1 <= g.x0 || g.z1 <= g.z0 || g.entries < 1)
     fail(kErrArg, "network region must lie 5 cells inside the field");
   if (tiles_of(g, margin) > 0x7fffffffLL) fail(kErrRange, "network launch too large");
-  if ((long long)g.nx * g.nz >= (1LL << 31)) fail(kErrRange, "network field too large");
+  if ((long long)g.nx * g.nz >= (1LL << 29)) fail(kErrRange, "network field too large");
 }
 
 }  // namespace
